@@ -1,0 +1,287 @@
+"""Benchmark: amp-updates/s and %HBM peak of the statevector gate kernels on
+one MI355X.
+
+Workload (one step): the 28-qubit random-SU(4) volume circuit, depth 28
+(392 Haar 4x4 gates on seeded random pairs, complex64 state of 2 GiB), run
+from |0...0> through the plain per-gate kernels.  value = 392 * 2^28 /
+seconds per step.  The same circuit is also run as fused 5-qubit blocks on
+FP32 MFMA ("fused"), and both results are checked against an FP64 CPU
+statevector of the same circuit family at a size the CPU finishes quickly.
+
+Extra figures in the same JSON line:
+  roofline      dominant kernel (dense 2Q gate), 16 B per amplitude update,
+                average launch duration from HIP events on the launch stream
+  per_gate_30q  each gate class on a 30-qubit state (8 GiB)
+  cpu_baseline  FP64 C/OpenMP statevector (oracle/sv_fp64.c) on host cores,
+                bounded sample of the same circuit
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "qiskit-blackwell_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+import torch  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+MFMA_FP32_PEAK_TFLOPS = 157.3
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+class EventTimer:
+    """HIP events on the stream the kernels are launched on."""
+
+    def __init__(self, stream):
+        self.stream = stream
+
+    def __enter__(self):
+        self.a = torch.cuda.Event(enable_timing=True)
+        self.b = torch.cuda.Event(enable_timing=True)
+        self.a.record(self.stream)
+        return self
+
+    def __exit__(self, *exc):
+        self.b.record(self.stream)
+
+    def ms(self):
+        self.b.synchronize()
+        return self.a.elapsed_time(self.b)
+
+
+def circuit_plan(sim, qc, mode):
+    cc = sim.compile(qc)
+    return cc, cc.plan(cc.resolve_mode(mode), sim.device)
+
+
+def run_steps(sim, qc, mode, steps, warmup, stream):
+    """Median/mean seconds per step and the kernel-only event time per step."""
+    import blackwell_kernels as bk
+    cc, plan = circuit_plan(sim, qc, mode)
+    state = torch.empty(1 << cc.n_qubits, dtype=torch.complex64, device=sim.device)
+    for _ in range(warmup):
+        bk.cuquantum._lib.call("qsim_state_init", bk.cuquantum.stream_handle(sim.device),
+                               bk.cuquantum.ptr(state), cc.n_qubits)
+        plan.run(state)
+    torch.cuda.synchronize()
+    wall, kern = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        bk.cuquantum._lib.call("qsim_state_init", bk.cuquantum.stream_handle(sim.device),
+                               bk.cuquantum.ptr(state), cc.n_qubits)
+        with EventTimer(stream) as ev:
+            plan.run(state)
+        torch.cuda.synchronize()
+        wall.append(time.perf_counter() - s0)
+        kern.append(ev.ms() / 1e3)
+    total = time.perf_counter() - t0
+    return state, total, wall, kern, plan.n_launches
+
+
+def fidelity_check(sim, n, depth, seed, modes):
+    """Fidelity / max|d amp| of the GPU result vs the FP64 C statevector."""
+    from blackwell_backend import workloads as W
+    from blackwell_backend.circuit import instructions
+    from oracle import build_ref
+    from oracle import statevector as O
+    lib = build_ref.load()
+    qc = W.su4_volume(n, depth, seed)
+    psi = np.zeros(1 << n, dtype=np.complex128)
+    lib.sv_init(psi.ctypes.data, n)
+    for name, qs, _, params in instructions(qc):
+        u = np.asarray(params[0], dtype=np.complex128)
+        v = np.empty(32)
+        v[0::2] = u.reshape(-1).real
+        v[1::2] = u.reshape(-1).imag
+        lib.sv_apply_2q(psi.ctypes.data, n, v.ctypes.data_as(_pd()), qs[0], qs[1])
+    out = {}
+    for mode in modes:
+        sv, _ = sim.run_statevector(qc, mode=mode)
+        out[mode] = {"fidelity": O.fidelity(psi, sv),
+                     "max_abs_damp": O.max_abs_diff_phase_aligned(psi, sv)}
+    return out
+
+
+def _pd():
+    import ctypes
+    return ctypes.POINTER(ctypes.c_double)
+
+
+def per_gate_sweep(n=30, reps=4):
+    """amp-updates/s and GB/s of each gate class on an n-qubit state."""
+    import blackwell_kernels as bk
+    from blackwell_kernels import _lib
+    dev = torch.device("cuda", 0)
+    st = bk.state_init(n, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    N = 1 << n
+    u2 = np.array([[0.6, 0.8j], [0.8j, 0.6]], dtype=np.complex64)
+    u4 = np.linalg.qr(np.random.default_rng(1).standard_normal((4, 4)) + 0j)[0].astype(np.complex64)
+    g2 = torch.from_numpy(u2).to(dev)
+    g4 = torch.from_numpy(u4).to(dev)
+    d2 = torch.tensor([np.exp(-0.3j), np.exp(0.3j)], dtype=torch.complex64, device=dev)
+    cases = [
+        ("1q_dense_t0", lambda: bk.apply_gate(st, g2, 0), N, 16 * N),
+        ("1q_dense_t12", lambda: bk.apply_gate(st, g2, 12), N, 16 * N),
+        ("1q_dense_t29", lambda: bk.apply_gate(st, g2, n - 1), N, 16 * N),
+        ("2q_dense_0_1", lambda: bk.apply_gate_2q(st, g4, 0, 1), N, 16 * N),
+        ("2q_dense_7_21", lambda: bk.apply_gate_2q(st, g4, 7, 21), N, 16 * N),
+        ("2q_dense_28_29", lambda: bk.apply_gate_2q(st, g4, n - 2, n - 1), N, 16 * N),
+        ("diag_t5", lambda: bk.apply_diagonal(st, d2, 5), N, 16 * N),
+        ("phase_t9", lambda: bk.apply_diagonal_phase(st, 1j, 9), N // 2, 8 * N),
+        ("cnot_3_17", lambda: bk.apply_mcx(st, [3], 17), N // 2, 8 * N),
+        ("mcx3_2_11_25_to_19", lambda: bk.apply_mcx(st, [2, 11, 25], 19), N // 8, 2 * N),
+        ("swap_4_23", lambda: bk.apply_mcswap(st, [], 4, 23), N // 2, 8 * N),
+        ("norm2", lambda: bk.norm2(st), N, 8 * N),
+    ]
+    out = {}
+    for name, fn, upd, nbytes in cases:
+        fn()
+        torch.cuda.synchronize()
+        with EventTimer(stream) as ev:
+            for _ in range(reps):
+                fn()
+        ms = ev.ms() / reps
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out[name] = {"ms": round(ms, 4), "amp_updates_per_s": upd / (ms * 1e-3),
+                     "GB_per_s": round(gbs, 1), "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+    # sampling 2^20 shots
+    u = torch.rand(1 << 20, dtype=torch.float64, device=dev)
+    bk.sample_with_uniforms(st, u)
+    torch.cuda.synchronize()
+    with EventTimer(stream) as ev:
+        bk.sample_with_uniforms(st, u)
+    out["sample_2^20_shots"] = {"ms": round(ev.ms(), 3)}
+    del st
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_baseline(n, depth, seed, budget_s=15.0):
+    """FP64 C/OpenMP statevector on host cores: amp-updates/s on a bounded
+    prefix of the same circuit (gates applied until budget_s elapses)."""
+    from blackwell_backend import workloads as W
+    from oracle import build_ref
+    lib = build_ref.load()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    threads = lib.sv_set_threads(threads)
+    psi = np.empty(1 << n, dtype=np.complex128)
+    lib.sv_init(psi.ctypes.data, n)
+    layers = W.su4_layers(n, depth, seed)
+    done = 0
+    t0 = time.perf_counter()
+    for pairs, us in layers:
+        for (a, b), u in zip(pairs, us):
+            v = np.empty(32)
+            v[0::2] = u.reshape(-1).real
+            v[1::2] = u.reshape(-1).imag
+            lib.sv_apply_2q(psi.ctypes.data, n, v.ctypes.data_as(_pd()), a, b)
+            done += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    del psi
+    return {"value": done * (1 << n) / el, "unit": "amp-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"first {done} of {n * depth // 2} gates of the {n}-qubit depth-{depth} "
+                      f"SU(4) circuit, FP64 C/OpenMP statevector (oracle/sv_fp64.c), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--qubits", type=int, default=28)
+    ap.add_argument("--no-extras", action="store_true", help="skip the 30q sweep and CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the statevector path here runs on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
+    from blackwell_backend import BlackwellSimulator, workloads as W
+    n = args.qubits
+    depth = n
+    seed = 2024
+    sim = BlackwellSimulator("cuda:0")
+    stream = torch.cuda.current_stream(sim.device)
+    qc = W.su4_volume(n, depth, seed)
+    n_gates = (n // 2) * depth
+    log(f"[bench] compile + warmup plain ({n} qubits, {n_gates} gates)")
+    _, total, wall, kern, launches = run_steps(sim, qc, "plain", args.steps, args.warmup, stream)
+    ms_step = total / args.steps * 1e3
+    updates = n_gates * float(1 << n)
+    value = updates / (total / args.steps)
+    per_launch_s = float(np.median(kern)) / launches
+    achieved = 16.0 * (1 << n) / per_launch_s / 1e9
+    log(f"[bench] plain: {ms_step:.1f} ms/step, {value:.3e} amp-updates/s, "
+        f"{achieved:.0f} GB/s per 2Q launch")
+    log("[bench] fused 5-qubit MFMA blocks")
+    _, ftotal, fwall, fkern, flaunch = run_steps(sim, qc, "blocks", args.steps, args.warmup, stream)
+    fused_value = updates / (ftotal / args.steps)
+    blk_s = float(np.median(fkern)) / flaunch
+    blk_bytes = 16.0 * (1 << n)
+    blk_flops = 256.0 * (1 << n)
+    log(f"[bench] fused: {ftotal / args.steps * 1e3:.1f} ms/step, {fused_value:.3e} gate-amp-updates/s,"
+        f" {flaunch} blocks")
+    line = {
+        "metric": "amp-updates/s (plain per-gate kernels, 28q random-SU(4) depth 28)",
+        "value": value,
+        "unit": "amp-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 (fp32 compensated)",
+        "data": "synthetic (seeded Haar SU(4) gates, seed 2024)",
+        "config": {"workload": f"random-SU(4) volume circuit, {n} qubits, depth {depth}, "
+                               f"{n_gates} gates, complex64 state {(8 << n) >> 30} GiB, plain per-gate",
+                   "qubits": n, "depth": depth, "gates": n_gates, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_gate2q (dense 4x4, 16 B/amp-update)"},
+        "fused": {"value": fused_value, "unit": "gate-equivalent amp-updates/s",
+                  "ms_per_step": ftotal / args.steps * 1e3, "blocks": flaunch,
+                  "speedup_vs_plain": fused_value / value,
+                  "block_GB_per_s": blk_bytes / blk_s / 1e9,
+                  "block_mfma_TFLOP_per_s": blk_flops / blk_s / 1e12,
+                  "frac_hbm": blk_bytes / blk_s / 1e9 / HBM_PEAK_GBS,
+                  "frac_mfma_fp32": blk_flops / blk_s / 1e12 / MFMA_FP32_PEAK_TFLOPS},
+    }
+    try:
+        log("[bench] FP64 parity check (22 qubits, depth 22)")
+        line["parity"] = fidelity_check(sim, 22, 22, seed, ("plain", "blocks"))
+    except Exception as e:  # reported, not hidden
+        line["parity"] = {"error": repr(e)}
+    if not args.no_extras:
+        try:
+            log("[bench] per-gate sweep at 30 qubits")
+            line["per_gate_30q"] = per_gate_sweep(30)
+        except Exception as e:
+            line["per_gate_30q"] = {"error": repr(e)}
+        try:
+            log("[bench] CPU baseline")
+            line["cpu_baseline"] = cpu_baseline(n, depth, seed)
+        except Exception as e:
+            line["cpu_baseline"] = {"error": repr(e)}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,90 @@
+"""FP64 CPU oracle for the SparsePauliOp -> CSR -> Chebyshev path —
+TEST INFRASTRUCTURE ONLY (tests/, smoke() and bench.py's cpu_baseline leg).
+
+* pauli_dense: brute-force Kronecker products of 2x2 Paulis in Qiskit label
+  order (rightmost character = qubit 0), the definition behind
+  SparsePauliOp.to_matrix used at blackwell_backend/hamiltonian.py:58.
+* chebyshev_coefficients: restates blackwell_backend/chebyshev.py:26-78
+  (c_k = (2 - delta_k0) (-i)^k J_k(alpha), K_est = alpha + 2 ln(1/eps) + 20,
+  tail trimmed while the cumulative tail is below eps, K >= 2, conjugated for
+  t < 0), with J_k from scipy.special.jv.
+* chebyshev_evolve_fp64: the three-term recurrence of chebyshev.py:111-156
+  in complex128.
+* expm_reference: scipy.sparse.linalg.expm_multiply, the independent check.
+"""
+import numpy as np
+import scipy.sparse as sp
+from scipy.special import jv
+
+_P = {
+    "I": np.eye(2, dtype=np.complex128),
+    "X": np.array([[0, 1], [1, 0]], dtype=np.complex128),
+    "Y": np.array([[0, -1j], [1j, 0]], dtype=np.complex128),
+    "Z": np.array([[1, 0], [0, -1]], dtype=np.complex128),
+}
+
+
+def pauli_dense(labels, coeffs):
+    """Dense matrix of sum_k coeffs[k] * P(labels[k])."""
+    n = len(labels[0])
+    m = np.zeros((1 << n, 1 << n), dtype=np.complex128)
+    for lab, c in zip(labels, coeffs):
+        t = np.array([[1.0 + 0j]])
+        for ch in lab:
+            t = np.kron(t, _P[ch])
+        m += c * t
+    return m
+
+
+def pauli_sparse(labels, coeffs):
+    """Same operator as a scipy CSR matrix (built term by term from sparse
+    Kronecker products; exact zeros removed)."""
+    n = len(labels[0])
+    m = sp.csr_matrix((1 << n, 1 << n), dtype=np.complex128)
+    for lab, c in zip(labels, coeffs):
+        t = sp.csr_matrix(np.array([[1.0 + 0j]]))
+        for ch in lab:
+            t = sp.kron(t, sp.csr_matrix(_P[ch]), format="csr")
+        m = m + c * t
+    m.eliminate_zeros()
+    return m.tocsr()
+
+
+def chebyshev_coefficients(t, spectral_bound, precision=1e-10):
+    alpha = spectral_bound * abs(t)
+    k_est = max(int(alpha + 2.0 * np.log(1.0 / precision) + 20), 10)
+    ks = np.arange(k_est + 1)
+    coeffs = np.where(ks > 0, 2.0, 1.0) * ((-1j) ** ks) * jv(ks, alpha)
+    cumtail = np.cumsum(np.abs(coeffs)[::-1])[::-1]
+    K = k_est
+    for k in range(k_est, 0, -1):
+        if cumtail[k] < precision:
+            K = k
+        else:
+            break
+    K = max(K, 2)
+    coeffs = coeffs[:K + 1]
+    if t < 0:
+        coeffs = np.conj(coeffs)
+    tail = float(cumtail[min(K, len(cumtail) - 1)])
+    return coeffs, K, tail
+
+
+def chebyshev_evolve_fp64(h, psi, t, spectral_bound, precision=1e-10):
+    """exp(-iHt) psi by the Chebyshev recurrence in complex128 (H sparse or dense)."""
+    coeffs, K, _ = chebyshev_coefficients(t, spectral_bound, precision)
+    scale = 1.0 / spectral_bound
+    t0 = np.asarray(psi, dtype=np.complex128)
+    t1 = scale * (h @ t0)
+    acc = coeffs[0] * t0 + coeffs[1] * t1
+    prev, cur = t0, t1
+    for k in range(2, K + 1):
+        nxt = 2.0 * scale * (h @ cur) - prev
+        acc = acc + coeffs[k] * nxt
+        prev, cur = cur, nxt
+    return acc
+
+
+def expm_reference(h, psi, t):
+    from scipy.sparse.linalg import expm_multiply
+    return expm_multiply(-1j * t * sp.csr_matrix(h), np.asarray(psi, dtype=np.complex128))

@@ -1,0 +1,110 @@
+"""Chebyshev evolution exp(-iHt)|psi> on the gfx950 library
+(blackwell_backend/chebyshev.py:26-172).
+
+Coefficients are FP64 on the host: c_k = (2 - delta_k0) (-i)^k J_k(alpha),
+alpha = spectral_bound * |t|, J_k from scipy.special.jv, conjugated for
+t < 0.  The degree is found by expanding until the coefficient tail is
+negligible and then trimming while the cumulative tail stays below the
+requested precision (the reference trims from a fixed estimate
+K_est = alpha + 2 ln(1/eps) + 20, which can sit only a few terms above the
+need at large alpha; here K_est only seeds the search).
+
+Device work per term is ONE launch (qsim_cheb_spmm_step): compensated
+SpMM of the scaled Hamiltonian against the current term, the recurrence
+T_{k+1} = 2 H~ T_k - T_{k-1} written over T_{k-1}, and the Kahan/Neumaier
+accumulation acc += c_k T_{k+1} with its compensation vector.  Any number
+of right-hand sides runs in the same launch, stored RHS-contiguous.
+"""
+import math
+import time
+
+import numpy as np
+import torch
+from scipy.special import jv as bessel_j
+
+from blackwell_kernels import _lib
+from blackwell_kernels._lib import call, ptr, stream_handle
+
+
+def chebyshev_coefficients(t, spectral_bound, precision=1e-10):
+    """(coeffs complex128 [K+1], K, tail_bound)."""
+    alpha = float(spectral_bound) * abs(float(t))
+    k_est = max(int(alpha + 2.0 * np.log(1.0 / precision) + 20), 10)
+    # grow until the last 8 coefficients are far below the precision
+    kmax = k_est
+    while True:
+        ks = np.arange(kmax + 1)
+        c = np.where(ks > 0, 2.0, 1.0) * ((-1j) ** (ks % 4)) * bessel_j(ks, alpha)
+        if np.max(np.abs(c[-8:])) < precision * 1e-6 or kmax > 4 * alpha + 4000:
+            break
+        kmax = int(kmax * 1.5) + 8
+    cumtail = np.cumsum(np.abs(c)[::-1])[::-1]
+    K = kmax
+    for k in range(kmax, 0, -1):
+        if cumtail[k] < precision:
+            K = k
+        else:
+            break
+    K = max(K, 2)
+    coeffs = c[:K + 1].astype(np.complex128)
+    if t < 0:
+        coeffs = np.conj(coeffs)
+    tail = float(cumtail[K + 1]) if K + 1 < len(cumtail) else 0.0
+    return coeffs, K, tail
+
+
+def chebyshev_evolve_batch(states, hamiltonian_csr, t, precision=1e-10):
+    """Evolve R states at once.  states: complex64 [R, 2^n] (or [2^n]) on the
+    device.  Returns (evolved [R, 2^n] complex64, info)."""
+    single = states.dim() == 1
+    X = states.reshape(1, -1) if single else states
+    R, N = X.shape
+    h = hamiltonian_csr
+    if N != h.n_rows:
+        raise RuntimeError(f"state dimension {N} != Hamiltonian dimension {h.n_rows}")
+    dev = X.device
+    torch.cuda.synchronize(dev)
+    t_start = time.perf_counter()
+    coeffs, K, tail = chebyshev_coefficients(t, h.spectral_bound, precision)
+    scale = 1.0 / h.spectral_bound
+    # RHS-contiguous buffers [N, R]
+    A = X.transpose(0, 1).contiguous()
+    B = torch.empty_like(A)
+    acc = torch.empty_like(A)
+    comp = torch.empty_like(A)
+    s = stream_handle(dev)
+    n_el = N * R
+    c0 = complex(coeffs[0])
+    call("qsim_cheb_init_acc", s, ptr(acc), ptr(comp), ptr(A), n_el, c0.real, c0.imag)
+    args = (ptr(h.row_ptr), ptr(h.col_idx), ptr(h.values), N, R, scale)
+    c1 = complex(coeffs[1])
+    call("qsim_cheb_spmm_step", s, *args, ptr(A), ptr(B), ptr(acc), ptr(comp), c1.real, c1.imag, 1)
+    cur, prv = B, A
+    for k in range(2, K + 1):
+        ck = complex(coeffs[k])
+        call("qsim_cheb_spmm_step", s, *args, ptr(cur), ptr(prv), ptr(acc), ptr(comp),
+             ck.real, ck.imag, 0)
+        cur, prv = prv, cur
+    call("qsim_cheb_finish", s, ptr(prv), ptr(acc), ptr(comp), n_el)
+    out = prv.transpose(0, 1).contiguous()
+    torch.cuda.synchronize(dev)
+    elapsed = time.perf_counter() - t_start
+    info = {
+        "engine": "chebyshev-beta",
+        "experimental": True,
+        "chebyshev_degree": K,
+        "spectral_bound": h.spectral_bound,
+        "truncation_error_bound": tail,
+        "spmv_count": K,
+        "workspace_vectors": 4,
+        "workspace_bytes": 4 * n_el * 8,
+        "elapsed_s": elapsed,
+        "n_rhs": R,
+    }
+    return (out[0] if single else out), info
+
+
+def chebyshev_evolve(state, hamiltonian_csr, t, precision=1e-10, bk=None):
+    """exp(-iHt)|state> for one state [2^n]; returns (result, info)
+    (chebyshev.py:81-172).  `bk` is accepted for signature parity."""
+    return chebyshev_evolve_batch(state, hamiltonian_csr, t, precision)

@@ -1,0 +1,273 @@
+"""Runs compiled circuits on the gfx950 library.
+
+A CompiledCircuit owns the FP64 op list of one circuit and builds, per
+execution mode and device, a list of prepared launches: ctypes argument
+tuples computed once, with every matrix that must live on the device
+uploaded once in a single tensor.  Executing a plan is then a tight loop of
+library calls on torch's current stream (the reference rebuilds small
+device tensors per op: blackwell_backend/simulator.py:348-404).
+
+Modes:
+  plain  - one kernel per op (per-gate baseline);
+  blocks - 5-qubit dense blocks on FP32 MFMA (n >= 10), other entries plain;
+  tiles  - LDS tile programs, many ops per HBM pass;
+  auto   - tiles with the whole state in one tile when n <= 13, else blocks.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from blackwell_kernels import _lib
+from blackwell_kernels._lib import call, OP_DTYPE
+from . import compiler as C
+
+WHOLE_TILE_MAX = 13     # n <= 13: the whole state is one LDS tile (64 KiB)
+
+
+def _d8(m):
+    m = np.asarray(m, dtype=np.complex128).reshape(-1)
+    v = np.empty(2 * m.size, dtype=np.float64)
+    v[0::2] = m.real
+    v[1::2] = m.imag
+    return _lib.f64_array(v)
+
+
+def _local(mask, q):
+    return bin(mask & ((1 << q) - 1)).count("1")
+
+
+def tile_ops(ops, mask):
+    """[Op] -> (OP_DTYPE array with tile-local qubits, complex128 coefs)."""
+    arr = np.zeros(len(ops), dtype=OP_DTYPE)
+    coefs = []
+    off = 0
+    for i, op in enumerate(ops):
+        loc = [_local(mask, q) for q in op.qubits]
+        cm = 0
+        for c in op.controls:
+            cm |= 1 << _local(mask, c)
+        arr[i]["cmask"] = cm
+        arr[i]["nq"] = len(loc)
+        for j, q in enumerate(loc):
+            arr[i]["q"][j] = q
+        k = op.kind
+        if k == "m1":
+            arr[i]["kind"] = _lib.OP_M1
+            c = op.mat.reshape(-1)
+        elif k == "d1":
+            arr[i]["kind"] = _lib.OP_D1
+            c = op.mat.reshape(-1)
+        elif k == "ph":
+            arr[i]["kind"] = _lib.OP_PH
+            c = op.mat.reshape(-1)
+        elif k == "m2":
+            arr[i]["kind"] = _lib.OP_M2
+            c = op.mat.reshape(-1)
+        elif k == "mk":
+            arr[i]["kind"] = _lib.OP_MK
+            c = op.mat.reshape(-1)
+        elif k == "x":
+            arr[i]["kind"] = _lib.OP_X
+            c = np.zeros(0, dtype=np.complex128)
+        elif k == "swap":
+            arr[i]["kind"] = _lib.OP_SWAP
+            c = np.zeros(0, dtype=np.complex128)
+        else:
+            raise ValueError(f"op {k} cannot run inside a tile program")
+        arr[i]["coef"] = off
+        coefs.append(np.asarray(c, dtype=np.complex128))
+        off += c.size
+    cf = np.concatenate(coefs) if coefs else np.zeros(0, dtype=np.complex128)
+    return arr, cf
+
+
+class _Dev:
+    """Device copies of a plan's matrices/op tables (one upload per plan)."""
+
+    def __init__(self, device, mats=None, ops=None, coefs=None):
+        self.mats = None if mats is None else torch.from_numpy(mats).to(device)
+        self.ops = None if ops is None else torch.from_numpy(ops.view(np.uint8).copy()).to(device)
+        self.coefs = None if coefs is None else torch.from_numpy(coefs).to(device)
+
+
+class CompiledCircuit:
+    def __init__(self, n_qubits, n_clbits, ops):
+        self.n_qubits = n_qubits
+        self.n_clbits = n_clbits
+        self.ops = ops
+        self._plans = {}
+        self.has_stochastic = any(o.kind in ("measure", "reset") for o in ops)
+
+    # ---------------------------------------------------------------- plans
+    def plan(self, mode, device):
+        key = (mode, str(device))
+        p = self._plans.get(key)
+        if p is None:
+            p = self._build(mode, device)
+            self._plans[key] = p
+        return p
+
+    def resolve_mode(self, mode):
+        if mode != "auto":
+            return mode
+        return "whole" if self.n_qubits <= WHOLE_TILE_MAX else "blocks"
+
+    def _build(self, mode, device):
+        n = self.n_qubits
+        if mode == "plain":
+            return self._build_plain(self.ops, device)
+        if mode == "whole":
+            mask = (1 << n) - 1
+            unitary = [o for o in self.ops if o.kind not in ("measure", "reset")]
+            if len(unitary) == len(self.ops):
+                return self._build_tiles([C.TilePass(mask, self.ops)] if self.ops else [], device)
+            mode = "tiles"
+        if mode == "tiles":
+            return self._build_tiles(C.plan_tiles(self.ops, n, m=min(WHOLE_TILE_MAX, n)), device)
+        if mode == "blocks":
+            if n < 10:
+                return self._build_tiles(C.plan_tiles(self.ops, n, m=n), device)
+            return self._build_blocks(C.plan_blocks(self.ops, n, k=5), device)
+        raise ValueError(f"unknown execution mode {mode!r}")
+
+    def _build_plain(self, ops, device):
+        n = self.n_qubits
+        mats = []
+        steps = []
+        for op in ops:
+            k, q = op.kind, op.qubits
+            if k in ("measure", "reset"):
+                steps.append((k, op))
+                continue
+            if op.controls and k in ("m1", "d1", "ph"):
+                mats.append(op.target_matrix().astype(np.complex64).reshape(-1))
+                steps.append(("mcgate", (len(mats) - 1, _lib.i64_array(op.controls),
+                                         len(op.controls), q[0])))
+            elif k == "m1":
+                steps.append(("qsim_apply_gate_h", (n, _d8(op.mat), q[0])))
+            elif k == "d1":
+                steps.append(("qsim_apply_diagonal_h", (n, _d8(op.mat), q[0])))
+            elif k == "ph":
+                ph = complex(op.mat[0])
+                steps.append(("qsim_apply_diagonal_phase", (n, ph.real, ph.imag, q[0])))
+            elif k == "m2" and not op.controls:
+                steps.append(("qsim_apply_gate_2q_h", (n, _d8(C.kernel_2q_matrix(op)),
+                                                       q[0], q[1])))
+            elif k == "x":
+                steps.append(("qsim_apply_mcx", (n, _lib.i64_array(op.controls),
+                                                 len(op.controls), q[0])))
+            elif k == "swap":
+                steps.append(("qsim_apply_mcswap", (n, _lib.i64_array(op.controls),
+                                                    len(op.controls), q[0], q[1])))
+            elif k in ("m2", "mk"):
+                if op.controls:
+                    raise ValueError("controlled multi-qubit dense ops are not supported")
+                mats.append(op.mat.astype(np.complex64).reshape(-1))
+                steps.append(("block", (len(mats) - 1, list(q))))
+            else:
+                raise ValueError(f"cannot execute op {op}")
+        return _Plan(self, steps, mats, None, None, device)
+
+    def _build_blocks(self, entries, device):
+        n = self.n_qubits
+        mats = []
+        steps = []
+        for e in entries:
+            if isinstance(e, C.Block):
+                mats.append(e.mat.astype(np.complex64).reshape(-1))
+                if len(e.qubits) == 5:
+                    steps.append(("mfma", (len(mats) - 1, _lib.int_array(e.qubits))))
+                else:
+                    steps.append(("block", (len(mats) - 1, list(e.qubits))))
+            else:
+                steps.append(("sub", self._build_plain([e], device)))
+        return _Plan(self, steps, mats, None, None, device)
+
+    def _build_tiles(self, entries, device):
+        all_ops, all_cf, steps = [], [], []
+        op_off, cf_off = 0, 0
+        for e in entries:
+            if isinstance(e, C.TilePass):
+                arr, cf = tile_ops(e.ops, e.mask)
+                arr["coef"] += cf_off
+                all_ops.append(arr)
+                all_cf.append(cf)
+                steps.append(("tile", (e.mask, op_off, len(arr))))
+                op_off += len(arr)
+                cf_off += cf.size
+            else:
+                steps.append(("sub", self._build_plain([e], device)))
+        ops = np.concatenate(all_ops) if all_ops else np.zeros(0, dtype=OP_DTYPE)
+        cf = np.concatenate(all_cf) if all_cf else np.zeros(0, dtype=np.complex128)
+        if cf.size == 0:
+            cf = np.zeros(1, dtype=np.complex128)
+        return _Plan(self, steps, [], ops, cf.astype(np.complex64), device)
+
+
+class _Plan:
+    def __init__(self, cc, steps, mats, ops, coefs, device):
+        self.cc = cc
+        self.steps = steps
+        self.n_launches = 0
+        mats_arr = None
+        self.mat_offsets = []
+        if mats:
+            off = 0
+            for m in mats:
+                self.mat_offsets.append(off)
+                off += m.size
+            mats_arr = np.concatenate(mats).astype(np.complex64)
+        self.dev = _Dev(device, mats_arr, ops, coefs)
+        for kind, _ in steps:
+            self.n_launches += 1 if kind != "sub" else 0
+        for kind, p in steps:
+            if kind == "sub":
+                self.n_launches += p.n_launches
+
+    def _mat_ptr(self, idx):
+        return ctypes.c_void_p(self.dev.mats.data_ptr() + 8 * self.mat_offsets[idx])
+
+    def run(self, state, rand=None, clbits=None):
+        """Apply the plan to `state` (complex64 [2^n] on the plan's device).
+        rand() supplies uniforms for measure/reset."""
+        n = self.cc.n_qubits
+        s = _lib.stream_handle(state.device)
+        sp = ctypes.c_void_p(state.data_ptr())
+        lib = _lib.load()
+        for kind, a in self.steps:
+            if kind.startswith("qsim_"):
+                rc = getattr(lib, kind)(s, sp, *a)
+                if rc:
+                    call(kind, s, sp, *a)  # re-raise with the message
+            elif kind == "mfma":
+                call("qsim_apply_block_mfma", s, sp, n, self._mat_ptr(a[0]), a[1], 5)
+            elif kind == "tile":
+                mask, off, cnt = a
+                call("qsim_tile_program", s, sp, n, mask,
+                     ctypes.c_void_p(self.dev.ops.data_ptr() + 16 * off), cnt,
+                     ctypes.c_void_p(self.dev.coefs.data_ptr()))
+            elif kind == "mcgate":
+                idx, ctl, nc, t = a
+                call("qsim_apply_mcgate", s, sp, n, self._mat_ptr(idx), ctl, nc, t)
+            elif kind == "block":
+                idx, qs = a
+                k = len(qs)
+                blk = self.dev.mats[self.mat_offsets[idx]:self.mat_offsets[idx] + (1 << (2 * k))]
+                from blackwell_kernels import apply_block
+                apply_block(state, blk.reshape(1 << k, 1 << k), qs)
+            elif kind == "sub":
+                a.run(state, rand, clbits)
+            elif kind in ("measure", "reset"):
+                from blackwell_kernels import measure_qubit, apply_mcx
+                op = a
+                u = rand() if rand is not None else float(torch.rand(1).item())
+                o = measure_qubit(state, op.qubits[0], u)
+                if kind == "measure":
+                    if clbits is not None and op.clbit is not None and op.clbit < len(clbits):
+                        clbits[op.clbit] = o
+                elif o == 1:
+                    apply_mcx(state, [], op.qubits[0])
+            else:
+                raise RuntimeError(f"unknown step {kind}")
+        return state

@@ -1,0 +1,451 @@
+// Reductions, measurement and shot sampling for gfx950.
+//
+// Every sum is FP64 and deterministic: pass 1 gives each workgroup a fixed
+// contiguous range of float4 slots and reduces it in a fixed order (per-lane
+// serial sum, then a wave64 butterfly, then the four waves in order); pass 2
+// is a single workgroup that adds the partials in index order.  No float
+// atomics are used, so the same state always gives the same bits (the
+// reference accumulates FP32 with atomicAdd: bwk/csrc/measure_sm120.cu:46-68,
+// bwk/csrc/renorm_sm120.cu:33-49).
+//
+// Sampling never materialises a CDF of the whole state.  The state is cut
+// into leaves of 256 amplitudes.  One wave computes a leaf's |a|^2 in FP64
+// and takes an inclusive scan across the wave in a fixed order; the last
+// value is the leaf total.  An exclusive FP64 scan of the leaf totals gives
+// the leaf prefix.  For each shot x = u * total: a binary search on the leaf
+// prefix finds the leaf, and the same wave scan, recomputed, finds the
+// smallest index whose cumulative probability exceeds x.  Because the leaf
+// total and the in-leaf search use the identical scan, they agree bit for bit.
+// (Rule: smallest i with cdf(i) > u * total; the reference returns the
+// smallest i with cdf[i] >= u on an FP32 CDF, bwk/csrc/sample_sm120.cu:48-69.)
+#include "qsim_internal.hpp"
+
+namespace qsim {
+
+constexpr int kRedBlocks = 1024;   // pass-1 workgroups for large states
+constexpr int kLeaf = 256;         // amplitudes per sampling leaf
+constexpr int kScanSeg = 4096;     // leaf totals per scan segment
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// Block of 256 threads: fixed-order sum; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) r = ((sh[0] + sh[1]) + (sh[2] + sh[3]));
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double p2(float2 a) {
+  const double x = a.x, y = a.y;
+  return x * x + y * y;
+}
+
+// Pass 1.  mode 0: sum |a|^2.  mode 1: also the part with bit t == 0
+// (out[2*b] = total, out[2*b+1] = part with bit t = 0).
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_red1(const float4* __restrict__ st, uint64_t n4,
+                                                   uint64_t chunk, int t, double* __restrict__ out) {
+  __shared__ double sh[4];
+  const uint64_t beg = (uint64_t)blockIdx.x * chunk;
+  const uint64_t end = beg + chunk < n4 ? beg + chunk : n4;
+  double s = 0.0, s0 = 0.0;
+  for (uint64_t f = beg + threadIdx.x; f < end; f += kThreads) {
+    const float4 v = st[f];
+    const double a = p2(lo2(v)), b = p2(hi2(v));
+    s += a + b;
+    if (MODE == 1) {
+      const uint64_t i0 = 2 * f;
+      if (t == 0) s0 += a;
+      else if (!((i0 >> t) & 1)) s0 += a + b;
+    }
+  }
+  const double r = block_sum(s, sh);
+  if (MODE == 1) {
+    const double r0 = block_sum(s0, sh);
+    if (threadIdx.x == 0) { out[2 * blockIdx.x] = r; out[2 * blockIdx.x + 1] = r0; }
+  } else {
+    if (threadIdx.x == 0) out[blockIdx.x] = r;
+  }
+}
+
+// Pass 2: one workgroup adds nb partials of width w (1 or 2) in order.
+// res[0] = sum of column 0, res[1] = sum of column 1.
+__global__ __launch_bounds__(kThreads) void k_red2(const double* __restrict__ part, int nb, int w,
+                                                   double* __restrict__ res) {
+  __shared__ double sh[4];
+  double a = 0.0, b = 0.0;
+  for (int j = threadIdx.x; j < nb; j += kThreads) {
+    a += part[j * w];
+    if (w == 2) b += part[j * w + 1];
+  }
+  const double ra = block_sum(a, sh);
+  const double rb = block_sum(b, sh);
+  if (threadIdx.x == 0) { res[0] = ra; res[1] = rb; }
+}
+
+struct RedPlan {
+  uint64_t n4, chunk;
+  int blocks;
+};
+static RedPlan red_plan(int n) {
+  RedPlan p;
+  p.n4 = (1ull << n) / 2;
+  if (p.n4 == 0) p.n4 = 1;  // n = 0 is rejected earlier; keep arithmetic sane
+  const uint64_t want = ceil_div(p.n4, 4 * kThreads);
+  p.blocks = (int)(want < (uint64_t)kRedBlocks ? (want ? want : 1) : kRedBlocks);
+  p.chunk = ceil_div(p.n4, p.blocks);
+  return p;
+}
+
+// Workspace layout (doubles): [0, 2*kRedBlocks) partials, then res[4].
+constexpr int64_t kRedWsDoubles = 2 * kRedBlocks + 8;
+
+static int reduce_impl(hipStream_t s, const void* state, int n, int mode, int t, double* ws) {
+  const RedPlan p = red_plan(n);
+  double* res = ws + 2 * kRedBlocks;
+  if (mode == 0)
+    hipLaunchKernelGGL(k_red1<0>, dim3(p.blocks), dim3(kThreads), 0, s, (const float4*)state,
+                       p.n4, p.chunk, t, ws);
+  else
+    hipLaunchKernelGGL(k_red1<1>, dim3(p.blocks), dim3(kThreads), 0, s, (const float4*)state,
+                       p.n4, p.chunk, t, ws);
+  if (int rc = launch_status("reduce pass 1")) return rc;
+  hipLaunchKernelGGL(k_red2, dim3(1), dim3(kThreads), 0, s, ws, p.blocks, mode == 0 ? 1 : 2, res);
+  return launch_status("reduce pass 2");
+}
+
+// ---------------------------------------------------------------- measure
+// res[0] = total, res[1] = p0 (bit t = 0).  The outcome is decided on the
+// device: 0 if random_val * total < p0, else 1; the kept half is scaled by
+// 1/sqrt(p_outcome) in FP64, the other half is zeroed.
+__global__ __launch_bounds__(kThreads) void k_collapse(float4* __restrict__ st, uint64_t n4, int t,
+                                                       double u, const double* __restrict__ res,
+                                                       int* __restrict__ outcome) {
+  const double tot = res[0], p0 = res[1];
+  const int o = (u * tot < p0) ? 0 : 1;
+  const double po = o == 0 ? p0 : tot - p0;
+  const double sc = po > 0.0 ? 1.0 / sqrt(po) : 0.0;
+  const uint64_t f = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (f == 0) *outcome = o;
+  if (f >= n4) return;
+  float4 v = st[f];
+  const uint64_t i0 = 2 * f;
+  const int b0 = (int)((i0 >> t) & 1), b1 = (int)(((i0 + 1) >> t) & 1);
+  const double k0 = b0 == o ? sc : 0.0, k1 = b1 == o ? sc : 0.0;
+  v.x = (float)(v.x * k0); v.y = (float)(v.y * k0);
+  v.z = (float)(v.z * k1); v.w = (float)(v.w * k1);
+  st[f] = v;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scale(float4* __restrict__ st, uint64_t n4,
+                                                    const double* __restrict__ res) {
+  const double tot = res[0];
+  const double sc = tot > 0.0 ? 1.0 / sqrt(tot) : 0.0;
+  const uint64_t f = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (f >= n4) return;
+  float4 v = st[f];
+  v.x = (float)(v.x * sc); v.y = (float)(v.y * sc);
+  v.z = (float)(v.z * sc); v.w = (float)(v.w * sc);
+  st[f] = v;
+}
+
+__global__ __launch_bounds__(kThreads) void k_probs(const float4* __restrict__ st, uint64_t n4,
+                                                    float2* __restrict__ probs) {
+  const uint64_t f = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (f >= n4) return;
+  const float4 v = st[f];
+  probs[f] = make_float2((float)p2(lo2(v)), (float)p2(hi2(v)));
+}
+
+// ---------------------------------------------------------------- sampling
+// Wave-level view of one leaf: lane l holds amplitudes 4l..4l+3 (two float4
+// slots).  c[0..3] receive the inclusive cumulative |a|^2 of those four
+// amplitudes within the leaf: lane-local running sums plus the inclusive
+// Hillis-Steele scan of the lanes below (fixed order, so every caller gets
+// the same bits).  nl = amplitudes in the leaf (256, or 2^n for tiny states).
+__device__ __forceinline__ void leaf_scan(const float4* __restrict__ leaf, int lane, int nl,
+                                          double c[4]) {
+  double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+  if (4 * lane < nl) {
+    const float4 v0 = leaf[2 * lane];
+    q0 = p2(lo2(v0)); q1 = p2(hi2(v0));
+  }
+  if (4 * lane + 2 < nl) {
+    const float4 v1 = leaf[2 * lane + 1];
+    q2 = p2(lo2(v1)); q3 = p2(hi2(v1));
+  }
+  c[0] = q0; c[1] = c[0] + q1; c[2] = c[1] + q2; c[3] = c[2] + q3;
+  double incl = c[3];
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  const double prev = __shfl_up(incl, 1, 64);
+  const double base = lane == 0 ? 0.0 : prev;
+  c[0] += base; c[1] += base; c[2] += base; c[3] += base;
+}
+
+// Leaf totals: one wave per leaf; total = last lane's c[3].
+__global__ __launch_bounds__(kThreads) void k_leaf_totals(const float4* __restrict__ st,
+                                                          uint64_t n_amps, uint64_t n_leaves,
+                                                          double* __restrict__ tot) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t leaf = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (leaf >= n_leaves) return;
+  const int nl = (int)(n_amps < (uint64_t)kLeaf ? n_amps : kLeaf);
+  double c[4];
+  leaf_scan(st + leaf * (kLeaf / 2), lane, nl, c);
+  const double last = __shfl(c[3], 63, 64);
+  if (lane == 0) tot[leaf] = last;
+}
+
+// Exclusive scan of the leaf totals in segments of kScanSeg: pass A writes
+// each segment's sum, pass B scans the segment sums (one workgroup, serial
+// over chunks of 256 with a fixed-order block scan), pass C writes the
+// exclusive prefix of every leaf.
+__device__ __forceinline__ double block_incl_scan(double v, double* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double y = __shfl_up(v, d, 64);
+    if (lane >= d) v += y;
+  }
+  if (lane == 63) sh[wid] = v;
+  __syncthreads();
+  double off = 0.0;
+  for (int k = 0; k < wid; ++k) off += sh[k];
+  __syncthreads();
+  return v + off;
+}
+
+__global__ __launch_bounds__(kThreads) void k_seg_sum(const double* __restrict__ tot, uint64_t nl,
+                                                      double* __restrict__ seg) {
+  __shared__ double sh[4];
+  const uint64_t beg = (uint64_t)blockIdx.x * kScanSeg;
+  double s = 0.0;
+  for (int k = threadIdx.x; k < kScanSeg; k += kThreads) {
+    const uint64_t j = beg + k;
+    if (j < nl) s += tot[j];
+  }
+  const double r = block_sum(s, sh);
+  if (threadIdx.x == 0) seg[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(kThreads) void k_seg_scan(double* __restrict__ seg, uint64_t ns,
+                                                       double* __restrict__ total) {
+  __shared__ double sh[4];
+  double carry = 0.0;
+  for (uint64_t b = 0; b < ns; b += kThreads) {
+    const uint64_t j = b + threadIdx.x;
+    const double v = j < ns ? seg[j] : 0.0;
+    const double incl = block_incl_scan(v, sh);
+    if (j < ns) seg[j] = carry + (incl - v);  // exclusive
+    __shared__ double last;
+    if (threadIdx.x == kThreads - 1) last = incl;
+    __syncthreads();
+    carry += last;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(kThreads) void k_leaf_prefix(const double* __restrict__ tot,
+                                                          uint64_t nl,
+                                                          const double* __restrict__ seg,
+                                                          double* __restrict__ pre) {
+  __shared__ double sh[4];
+  const uint64_t beg = (uint64_t)blockIdx.x * kScanSeg;
+  double carry = seg[blockIdx.x];
+  for (int b = 0; b < kScanSeg; b += kThreads) {
+    const uint64_t j = beg + b + threadIdx.x;
+    const double v = j < nl ? tot[j] : 0.0;
+    const double incl = block_incl_scan(v, sh);
+    if (j < nl) pre[j] = carry + (incl - v);
+    __shared__ double last;
+    if (threadIdx.x == kThreads - 1) last = incl;
+    __syncthreads();
+    carry += last;
+    __syncthreads();
+  }
+}
+
+// Last amplitude of the leaf with nonzero probability (-1 if none), from the
+// cumulative values of leaf_scan.
+__device__ __forceinline__ int64_t leaf_last_nonzero(const double c[4], int lane, int nl) {
+  double prevc = __shfl_up(c[3], 1, 64);
+  if (lane == 0) prevc = 0.0;
+  int lastk = -1;
+  if (4 * lane < nl) {
+    if (c[0] > prevc) lastk = 0;
+    if (c[1] > c[0]) lastk = 1;
+    if (c[2] > c[1]) lastk = 2;
+    if (c[3] > c[2]) lastk = 3;
+  }
+  const uint64_t any = __ballot(lastk >= 0);
+  if (!any) return -1;
+  const int fl = 63 - __clzll((unsigned long long)any);
+  return 4 * fl + __shfl(lastk, fl, 64);
+}
+
+// One wave per shot.  Leaf j = largest j with pre[j] <= x; a leaf with zero
+// total reached by rounding steps back to the previous nonzero leaf, and an
+// x past the leaf's own total (rounding again) takes the leaf's last nonzero
+// amplitude, so a zero-probability index is never returned.
+__global__ __launch_bounds__(kThreads) void k_sample(const float4* __restrict__ st, uint64_t n_amps,
+                                                     uint64_t nl, const double* __restrict__ pre,
+                                                     const double* __restrict__ tot,
+                                                     const double* __restrict__ total,
+                                                     const double* __restrict__ u, uint64_t shots,
+                                                     int64_t* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t shot = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (shot >= shots) return;
+  const double x = u[shot] * total[0];
+  uint64_t lo = 0, hi = nl - 1;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= x) lo = mid; else hi = mid - 1;
+  }
+  uint64_t leaf = lo;
+  while (leaf > 0 && !(tot[leaf] > 0.0)) --leaf;
+  const double y = x - pre[leaf];
+  const int nla = (int)(n_amps < (uint64_t)kLeaf ? n_amps : kLeaf);
+  double c[4];
+  leaf_scan(st + leaf * (kLeaf / 2), lane, nla, c);
+  int k = 4;
+  if (4 * lane < nla && c[3] > y) k = c[0] > y ? 0 : (c[1] > y ? 1 : (c[2] > y ? 2 : 3));
+  const uint64_t hit = __ballot(k < 4);
+  int64_t local;
+  if (hit) {
+    const int fl = __ffsll((unsigned long long)hit) - 1;
+    local = 4 * fl + __shfl(k, fl, 64);
+  } else {
+    local = leaf_last_nonzero(c, lane, nla);
+    if (local < 0) local = 0;  // all-zero state: index 0 (as the reference)
+  }
+  if (lane == 0) out[shot] = (int64_t)(leaf * kLeaf) + local;
+}
+
+struct SamplePlan {
+  uint64_t n_amps, nl, ns;
+};
+static SamplePlan sample_plan(int n) {
+  SamplePlan p;
+  p.n_amps = 1ull << n;
+  p.nl = ceil_div(p.n_amps, kLeaf);
+  p.ns = ceil_div(p.nl, kScanSeg);
+  return p;
+}
+
+}  // namespace qsim
+
+using namespace qsim;
+
+extern "C" {
+
+int64_t qsim_reduce_workspace_bytes(int n_qubits) {
+  (void)n_qubits;
+  return kRedWsDoubles * (int64_t)sizeof(double);
+}
+
+int qsim_norm2(void* stream, const void* state, int n_qubits, void* workspace, void* out_dev) {
+  if (int rc = check_n(n_qubits)) return rc;
+  if (!state || !workspace || !out_dev) return fail(QSIM_EINVAL, "NULL pointer argument");
+  hipStream_t s = as_stream(stream);
+  double* ws = (double*)workspace;
+  if (int rc = reduce_impl(s, state, n_qubits, 0, 0, ws)) return rc;
+  if (hipMemcpyAsync(out_dev, ws + 2 * kRedBlocks, sizeof(double), hipMemcpyDeviceToDevice, s) !=
+      hipSuccess)
+    return fail(QSIM_EHIP, "norm2: copy failed");
+  return QSIM_OK;
+}
+
+int qsim_measure_probs(void* stream, const void* state, int n_qubits, void* probs_dev) {
+  if (int rc = check_n(n_qubits)) return rc;
+  if (!state || !probs_dev) return fail(QSIM_EINVAL, "NULL pointer argument");
+  const uint64_t n4 = (1ull << n_qubits) / 2;
+  hipLaunchKernelGGL(k_probs, dim3(ceil_div(n4, kThreads)), dim3(kThreads), 0, as_stream(stream),
+                     (const float4*)state, n4, (float2*)probs_dev);
+  return launch_status("measure_probs");
+}
+
+int qsim_measure_qubit(void* stream, void* state, int n_qubits, int target, double random_val,
+                       void* workspace, int* outcome) {
+  if (int rc = check_n(n_qubits)) return rc;
+  if (int rc = check_q(target, n_qubits, "target_qubit")) return rc;
+  if (!state || !workspace || !outcome) return fail(QSIM_EINVAL, "NULL pointer argument");
+  hipStream_t s = as_stream(stream);
+  double* ws = (double*)workspace;
+  if (int rc = reduce_impl(s, state, n_qubits, 1, target, ws)) return rc;
+  const uint64_t n4 = (1ull << n_qubits) / 2;
+  int* dev_out = (int*)(ws + 2 * kRedBlocks + 4);
+  hipLaunchKernelGGL(k_collapse, dim3(ceil_div(n4, kThreads)), dim3(kThreads), 0, s, (float4*)state,
+                     n4, target, random_val, ws + 2 * kRedBlocks, dev_out);
+  if (int rc = launch_status("measure_qubit collapse")) return rc;
+  int host = -1;
+  if (hipMemcpyAsync(&host, dev_out, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess)
+    return fail(QSIM_EHIP, "measure_qubit: copy failed");
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(QSIM_EHIP, "measure_qubit: sync failed");
+  *outcome = host;
+  return QSIM_OK;
+}
+
+int qsim_renormalize(void* stream, void* state, int n_qubits, void* workspace) {
+  if (int rc = check_n(n_qubits)) return rc;
+  if (!state || !workspace) return fail(QSIM_EINVAL, "NULL pointer argument");
+  hipStream_t s = as_stream(stream);
+  double* ws = (double*)workspace;
+  if (int rc = reduce_impl(s, state, n_qubits, 0, 0, ws)) return rc;
+  const uint64_t n4 = (1ull << n_qubits) / 2;
+  hipLaunchKernelGGL(k_scale, dim3(ceil_div(n4, kThreads)), dim3(kThreads), 0, s, (float4*)state,
+                     n4, ws + 2 * kRedBlocks);
+  return launch_status("renormalize");
+}
+
+// Layout (doubles): tot[nl] | pre[nl] | seg[ns] | total[1]
+int64_t qsim_sample_workspace_bytes(int n_qubits) {
+  if (n_qubits < 1 || n_qubits > QSIM_MAX_QUBITS) return -1;
+  const SamplePlan p = sample_plan(n_qubits);
+  return (int64_t)(2 * p.nl + p.ns + 2) * (int64_t)sizeof(double);
+}
+
+int qsim_sample(void* stream, const void* state, int n_qubits, const void* uniforms_dev,
+                int64_t n_shots, void* workspace, void* out_dev) {
+  if (int rc = check_n(n_qubits)) return rc;
+  if (n_shots < 0) return fail(QSIM_EINVAL, "n_shots < 0");
+  if (n_shots == 0) return QSIM_OK;
+  if (!state || !uniforms_dev || !workspace || !out_dev)
+    return fail(QSIM_EINVAL, "NULL pointer argument");
+  hipStream_t s = as_stream(stream);
+  const SamplePlan p = sample_plan(n_qubits);
+  double* tot = (double*)workspace;
+  double* pre = tot + p.nl;
+  double* seg = pre + p.nl;
+  double* total = seg + p.ns;
+  hipLaunchKernelGGL(k_leaf_totals, dim3(ceil_div(p.nl, 4)), dim3(kThreads), 0, s,
+                     (const float4*)state, p.n_amps, p.nl, tot);
+  if (int rc = launch_status("sample: leaf totals")) return rc;
+  hipLaunchKernelGGL(k_seg_sum, dim3(p.ns), dim3(kThreads), 0, s, tot, p.nl, seg);
+  if (int rc = launch_status("sample: segment sums")) return rc;
+  hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(kThreads), 0, s, seg, p.ns, total);
+  if (int rc = launch_status("sample: segment scan")) return rc;
+  hipLaunchKernelGGL(k_leaf_prefix, dim3(p.ns), dim3(kThreads), 0, s, tot, p.nl, seg, pre);
+  if (int rc = launch_status("sample: leaf prefix")) return rc;
+  hipLaunchKernelGGL(k_sample, dim3(ceil_div((uint64_t)n_shots, 4)), dim3(kThreads), 0, s,
+                     (const float4*)state, p.n_amps, p.nl, pre, tot, total,
+                     (const double*)uniforms_dev,
+                     (uint64_t)n_shots, (int64_t*)out_dev);
+  return launch_status("sample: search");
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+// Internal helpers shared by the gfx950 kernels of libqsim_gfx950.so.
+//
+// Numerics: every kernel is compiled with -ffp-contract=off, so a*b+c is
+// never silently fused; the only FMAs are the explicit __builtin_fmaf calls
+// below.  That keeps the error-free transforms exact:
+//   TwoProduct (Dekker 1971, FMA form): p = fl(a*b), e = fma(a, b, -p)
+//   gives a*b = p + e exactly.
+// A compensated dot product adds the rounded products and, separately, their
+// exact product errors, then combines the two (the "Dot with TwoProduct"
+// scheme of Ogita, Rump and Oishi, "Accurate sum and dot product", SISC 2005,
+// without the TwoSum stage, which costs 6 more VALU ops per term; MI355X has
+// ~8x less VALU throughput per HBM byte than the FP32 product count of a
+// full Dot2 would need at 8 TB/s).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdarg.h>
+#include "qsim_gfx950.h"
+
+namespace qsim {
+
+// ---------------------------------------------------------------- host side
+int  fail(int code, const char* fmt, ...);     // records message, returns code
+void clear_error();
+int  launch_status(const char* what);          // hipGetLastError after a launch
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Validates the qubit count of a state (1..QSIM_MAX_QUBITS).
+int check_n(int n_qubits);
+// Validates one qubit index against n_qubits.
+int check_q(int q, int n_qubits, const char* what);
+
+constexpr int kThreads = 256;
+
+inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// Sorted affected-qubit list plus control mask for the multi-controlled kernels.
+struct McSpec {
+  int      pos[10];   // ascending positions of all affected qubits
+  int      n_pos;
+  uint64_t cmask;     // OR of 1 << control
+};
+// Builds an McSpec from controls + targets; checks ranges and distinctness.
+int build_mc(McSpec& s, int n_qubits, const int64_t* controls, int n_controls,
+             const int* targets, int n_targets);
+
+// ---------------------------------------------------------------- device side
+// Insert a zero bit at position q of x (bits >= q move up by one).
+template <typename I>
+__device__ __forceinline__ I ins0(I x, int q) {
+  const I lo = x & ((I(1) << q) - 1);
+  return ((x ^ lo) << 1) | lo;
+}
+
+__device__ __forceinline__ void two_prod(float a, float b, float& p, float& e) {
+  p = a * b;
+  e = __builtin_fmaf(a, b, -p);
+}
+
+// Compensated x0*y0 + x1*y1 + x2*y2 + x3*y3.
+__device__ __forceinline__ float cdot4(float x0, float y0, float x1, float y1,
+                                       float x2, float y2, float x3, float y3) {
+  float p0, e0, p1, e1, p2, e2, p3, e3;
+  two_prod(x0, y0, p0, e0);
+  two_prod(x1, y1, p1, e1);
+  two_prod(x2, y2, p2, e2);
+  two_prod(x3, y3, p3, e3);
+  const float s = (p0 + p1) + (p2 + p3);
+  const float e = (e0 + e1) + (e2 + e3);
+  return s + e;
+}
+
+// Compensated complex g0*a0 + g1*a1 (one row of a 2x2 gate).
+__device__ __forceinline__ float2 crow2(float2 g0, float2 a0, float2 g1, float2 a1) {
+  float2 r;
+  r.x = cdot4(g0.x, a0.x, -g0.y, a0.y, g1.x, a1.x, -g1.y, a1.y);
+  r.y = cdot4(g0.x, a0.y, g0.y, a0.x, g1.x, a1.y, g1.y, a1.x);
+  return r;
+}
+
+// Compensated complex sum of four products (one row of a 4x4 gate):
+// eight real products per component, split into two 4-term halves whose
+// rounded sums and exact product errors are combined at the end.
+__device__ __forceinline__ float2 crow4(const float2* g, float2 a0, float2 a1,
+                                        float2 a2, float2 a3) {
+  float p[8], e[8];
+  two_prod(g[0].x, a0.x, p[0], e[0]);  two_prod(-g[0].y, a0.y, p[1], e[1]);
+  two_prod(g[1].x, a1.x, p[2], e[2]);  two_prod(-g[1].y, a1.y, p[3], e[3]);
+  two_prod(g[2].x, a2.x, p[4], e[4]);  two_prod(-g[2].y, a2.y, p[5], e[5]);
+  two_prod(g[3].x, a3.x, p[6], e[6]);  two_prod(-g[3].y, a3.y, p[7], e[7]);
+  float2 r;
+  r.x = (((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]))) +
+        (((e[0] + e[1]) + (e[2] + e[3])) + ((e[4] + e[5]) + (e[6] + e[7])));
+  two_prod(g[0].x, a0.y, p[0], e[0]);  two_prod(g[0].y, a0.x, p[1], e[1]);
+  two_prod(g[1].x, a1.y, p[2], e[2]);  two_prod(g[1].y, a1.x, p[3], e[3]);
+  two_prod(g[2].x, a2.y, p[4], e[4]);  two_prod(g[2].y, a2.x, p[5], e[5]);
+  two_prod(g[3].x, a3.y, p[6], e[6]);  two_prod(g[3].y, a3.x, p[7], e[7]);
+  r.y = (((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]))) +
+        (((e[0] + e[1]) + (e[2] + e[3])) + ((e[4] + e[5]) + (e[6] + e[7])));
+  return r;
+}
+
+// Compensated complex product a*d.
+__device__ __forceinline__ float2 cmulc(float2 a, float2 d) {
+  float p0, e0, p1, e1;
+  float2 r;
+  two_prod(a.x, d.x, p0, e0);
+  two_prod(-a.y, d.y, p1, e1);
+  r.x = (p0 + p1) + (e0 + e1);
+  two_prod(a.x, d.y, p0, e0);
+  two_prod(a.y, d.x, p1, e1);
+  r.y = (p0 + p1) + (e0 + e1);
+  return r;
+}
+
+__device__ __forceinline__ float2 lo2(float4 v) { return make_float2(v.x, v.y); }
+__device__ __forceinline__ float2 hi2(float4 v) { return make_float2(v.z, v.w); }
+__device__ __forceinline__ float4 cat4(float2 a, float2 b) { return make_float4(a.x, a.y, b.x, b.y); }
+
+// Gate coefficients passed by value (kernel arguments live in SGPRs).
+struct M2 { float2 g[4]; };
+struct M4 { float2 g[16]; };
+
+}  // namespace qsim

@@ -1,0 +1,108 @@
+"""The C ABI on the CPU: the library loads, exports every function declared
+in include/qsim_gfx950.h, the ctypes prototypes agree with the header, the
+struct layouts agree with a C compiler's, and argument validation answers
+before any device work (no GPU needed)."""
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "qsim_gfx950.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    out = {}
+    for m in re.finditer(r"^\s*(?:const\s+)?\w+\**\s+\**(qsim_\w+)\s*\(([^;]*?)\)\s*;", src, re.M | re.S):
+        args = m.group(2).strip()
+        n = 0 if args in ("", "void") else args.count(",") + 1
+        out[m.group(1)] = n
+    return out
+
+
+def test_header_parses_and_prototypes_match():
+    from blackwell_kernels import _lib
+    decl = declared_functions()
+    assert len(decl) >= 40
+    assert set(decl) == set(_lib.PROTOTYPES), set(decl) ^ set(_lib.PROTOTYPES)
+    for name, nargs in decl.items():
+        assert len(_lib.PROTOTYPES[name][1]) == nargs, name
+
+
+def test_library_exports_every_symbol():
+    from blackwell_kernels import _lib
+    lib = _lib.load()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    assert set(declared_functions()) <= exported
+
+
+def test_struct_layout_matches_c_compiler():
+    from blackwell_kernels import _lib
+    prog = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "qsim_gfx950.h"
+int main(void) {
+  printf("%zu %zu %zu %zu\n", sizeof(qsim_op), offsetof(qsim_op, cmask), offsetof(qsim_op, coef),
+         offsetof(qsim_op, q));
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(qsim_instance), offsetof(qsim_instance, op_off),
+         offsetof(qsim_instance, shots), offsetof(qsim_instance, n_qubits),
+         offsetof(qsim_instance, want), offsetof(qsim_instance, shot_off));
+  return 0;
+}'''
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "probe.c")
+        exe = os.path.join(d, "probe")
+        open(c, "w").write(prog)
+        subprocess.run(["gcc", "-I", os.path.dirname(HEADER), c, "-o", exe], check=True)
+        l1, l2 = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")[:2]
+    op = [int(x) for x in l1.split()]
+    ins = [int(x) for x in l2.split()]
+    od, idt = _lib.OP_DTYPE, _lib.INSTANCE_DTYPE
+    assert op == [od.itemsize, od.fields["cmask"][1], od.fields["coef"][1], od.fields["q"][1]]
+    assert ins == [idt.itemsize, idt.fields["op_off"][1], idt.fields["shots"][1],
+                   idt.fields["n_qubits"][1], idt.fields["want"][1], idt.fields["shot_off"][1]]
+
+
+def test_meta_and_validation_without_device():
+    from blackwell_kernels import _lib
+    lib = _lib.load()
+    assert lib.qsim_version().startswith(b"qsim-gfx950")
+    assert lib.qsim_abi_version() == 1
+    c = ctypes.c_int(-1)
+    assert lib.qsim_device_count(ctypes.byref(c)) == 0 and c.value >= 0
+    # argument errors are reported before any HIP call
+    assert lib.qsim_state_init(None, None, 0) == 2                      # QSIM_ERANGE
+    assert b"n_qubits" in lib.qsim_last_error()
+    assert lib.qsim_apply_gate(None, ctypes.c_void_p(8), 4, ctypes.c_void_p(8), 4) == 1
+    assert b"target_qubit" in lib.qsim_last_error()
+    assert lib.qsim_apply_gate_2q(None, ctypes.c_void_p(8), 4, ctypes.c_void_p(8), 1, 1) == 1
+    ctl = _lib.i64_array([2])
+    assert lib.qsim_apply_mcx(None, ctypes.c_void_p(8), 4, ctl, 1, 2) == 1
+    assert b"controls" in lib.qsim_last_error()
+    assert lib.qsim_apply_block_mfma(None, ctypes.c_void_p(8), 12, ctypes.c_void_p(8),
+                                     _lib.int_array([0, 1, 2, 3]), 4) == 2
+    assert lib.qsim_sample_workspace_bytes(20) > 0
+    assert lib.qsim_sample_workspace_bytes(0) == -1
+    assert lib.qsim_tile_program(None, ctypes.c_void_p(8), 10, 0b110, ctypes.c_void_p(8), 1,
+                                 ctypes.c_void_p(8)) == 1                 # mask without qubit 0
+    assert lib.qsim_run_instances(None, None, ctypes.c_void_p(8), 1, ctypes.c_void_p(8),
+                                  ctypes.c_void_p(8), None, 15) == 2
+
+
+def test_python_layer_raises_runtime_error():
+    import torch
+    import blackwell_kernels as bk
+    with pytest.raises(RuntimeError):
+        bk.apply_gate(torch.zeros(4, dtype=torch.complex64), bk.pauli_x_gate("cpu"), 0)
+    with pytest.raises(NotImplementedError):
+        bk.expectation_z(None, 1)

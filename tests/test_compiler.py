@@ -1,0 +1,201 @@
+"""Host logic on the CPU: parsing, FP64 fusion, planners, Pauli->CSR,
+Chebyshev coefficients, workloads — each checked against the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import hamiltonian as OH
+from oracle import statevector as O
+
+
+def _instrs(qc):
+    from blackwell_backend.circuit import instructions
+    return [(nm, q, p) for nm, q, _, p in instructions(qc)]
+
+
+def _sim_ops(n, ops):
+    psi = O.zero_state(n)
+    for op in ops:
+        psi = O.apply_matrix(psi, op.full_matrix(), list(op.all_qubits))
+    return psi
+
+
+def random_circuit(n, depth, seed):
+    from blackwell_backend.circuit import QuantumCircuit
+    rng = np.random.default_rng(seed)
+    qc = QuantumCircuit(n)
+    one = ["h", "x", "y", "z", "s", "sdg", "t", "tdg", "sx", "sxdg", "id"]
+    par = {"rx": 1, "ry": 1, "rz": 1, "p": 1, "u1": 1, "u2": 2, "u3": 3, "u": 3}
+    two = ["cx", "cz", "swap", "cp", "crz", "cu"]
+    for _ in range(depth):
+        r = rng.random()
+        if r < 0.45:
+            if rng.random() < 0.5:
+                getattr(qc, str(rng.choice(one)))(int(rng.integers(n)))
+            else:
+                nm = str(rng.choice(list(par)))
+                getattr(qc, nm)(*rng.uniform(-3, 3, par[nm]), int(rng.integers(n)))
+        elif r < 0.85:
+            nm = str(rng.choice(two))
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            k = {"cp": 1, "crz": 1, "cu": 4}.get(nm, 0)
+            getattr(qc, nm)(*rng.uniform(-3, 3, k), a, b)
+        elif r < 0.93:
+            a, b, c = (int(x) for x in rng.choice(n, 3, replace=False))
+            qc.ccx(a, b, c) if rng.random() < 0.5 else qc.cswap(a, b, c)
+        else:
+            qs = [int(x) for x in rng.choice(n, min(n, 5), replace=False)]
+            qc.mcx(qs[:-1], qs[-1])
+    return qc
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_parse_and_fuse_preserve_semantics(seed):
+    from blackwell_backend import compiler as C
+    from blackwell_backend.circuit import instructions
+    n = 6
+    qc = random_circuit(n, 80, seed)
+    ref = O.run_instructions(n, _instrs(qc))
+    raw = C.parse(instructions(qc))
+    fused = C.fuse_1q_runs(raw)
+    assert len(fused) <= len(raw)
+    for ops in (raw, fused):
+        assert O.fidelity(ref, _sim_ops(n, ops)) > 1 - 1e-13
+        assert O.max_abs_diff_phase_aligned(ref, _sim_ops(n, ops)) < 1e-12
+
+
+def test_fusion_merges_runs_and_keeps_permutations():
+    from blackwell_backend import compiler as C
+    from blackwell_backend.circuit import QuantumCircuit, instructions
+    qc = QuantumCircuit(3)
+    qc.rz(0.1, 0).rz(0.2, 0).t(0)       # diagonal run -> one diagonal op
+    qc.h(1).h(1)                        # identity -> dropped
+    qc.x(2)                             # lone X stays a permutation
+    qc.z(1)                             # d0 = 1 -> phase-only op
+    ops = C.compile_ops(instructions(qc))
+    kinds = sorted((o.kind, o.qubits) for o in ops)
+    assert kinds == [("d1", (0,)), ("ph", (1,)), ("x", (2,))]
+
+
+def test_kernel_2q_matrix_convention():
+    from blackwell_backend import compiler as C
+    rng = np.random.default_rng(1)
+    u = np.linalg.qr(rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4)))[0]
+    psi = O.zero_state(4) + 0
+    psi = np.random.default_rng(2).standard_normal(16) + 0j
+    for qs in ([1, 3], [3, 1]):
+        op = C.Op("m2", qs, u)
+        km = C.kernel_2q_matrix(op)
+        a = O.apply_matrix(psi, u, qs)
+        b = O.apply_2q_kernel(psi, km, qs[0], qs[1])
+        assert np.allclose(a, b)
+
+
+@pytest.mark.parametrize("n,seed", [(8, 1), (10, 2), (12, 3)])
+def test_plan_blocks_equals_circuit(n, seed):
+    from blackwell_backend import compiler as C
+    from blackwell_backend.circuit import instructions
+    qc = random_circuit(n, 120, seed)
+    ops = C.compile_ops(instructions(qc))
+    ref = _sim_ops(n, ops)
+    psi = O.zero_state(n)
+    for e in C.plan_blocks(ops, n, k=5):
+        if isinstance(e, C.Block):
+            assert len(e.qubits) == min(5, n) and e.qubits == sorted(e.qubits)
+            psi = O.apply_matrix(psi, e.mat, e.qubits)
+        else:
+            psi = O.apply_matrix(psi, e.full_matrix(), list(e.all_qubits))
+    assert O.max_abs_diff_phase_aligned(ref, psi) < 1e-12
+
+
+@pytest.mark.parametrize("n,m", [(10, 8), (14, 10), (16, 13)])
+def test_plan_tiles_equals_circuit(n, m):
+    from blackwell_backend import compiler as C
+    from blackwell_backend.circuit import instructions
+    qc = random_circuit(n, 150, n)
+    ops = C.compile_ops(instructions(qc))
+    ref = _sim_ops(n, ops)
+    psi = O.zero_state(n)
+    passes = C.plan_tiles(ops, n, m=m, low=4)
+    for p in passes:
+        assert bin(p.mask).count("1") == m and (p.mask & 0xF) == 0xF
+        for op in p.ops:
+            for q in op.all_qubits:
+                assert p.mask >> q & 1
+            psi = O.apply_matrix(psi, op.full_matrix(), list(op.all_qubits))
+    assert sum(len(p.ops) for p in passes) == len(ops)
+    assert O.max_abs_diff_phase_aligned(ref, psi) < 1e-12
+
+
+def test_tile_op_encoding():
+    from blackwell_backend import compiler as C
+    from blackwell_backend.executor import tile_ops
+    from blackwell_kernels import _lib
+    mask = 0b1_0110_0111          # qubits 0,1,2,5,6,8
+    ops = [C.Op("m1", [5], np.eye(2), controls=[8]), C.Op("m2", [8, 0], np.eye(4)),
+           C.Op("x", [6], controls=[1, 2]), C.Op("ph", [2], np.array([1j]))]
+    arr, cf = tile_ops(ops, mask)
+    assert list(arr["kind"]) == [_lib.OP_M1, _lib.OP_M2, _lib.OP_X, _lib.OP_PH]
+    assert arr[0]["q"][0] == 3 and arr[0]["cmask"] == 1 << 5
+    assert list(arr[1]["q"][:2]) == [5, 0]
+    assert arr[2]["q"][0] == 4 and arr[2]["cmask"] == 0b110
+    assert list(arr["coef"]) == [0, 4, 20, 20] and cf.size == 21
+
+
+def test_pauli_to_csr_matches_kronecker():
+    from blackwell_backend.hamiltonian import SparsePauliOp, pauli_to_csr_numpy
+    import scipy.sparse as sp
+    rng = np.random.default_rng(4)
+    for n in (1, 3, 6):
+        labels = ["".join(rng.choice(list("IXYZ"), n)) for _ in range(12)]
+        coeffs = rng.uniform(-1, 1, 12) + 1j * rng.uniform(-1, 1, 12) * (n % 2)
+        rp, ci, v = pauli_to_csr_numpy(labels, coeffs, n)
+        m = sp.csr_matrix((v, ci, rp), shape=(1 << n, 1 << n)).toarray()
+        assert np.allclose(m, OH.pauli_dense(labels, coeffs), atol=1e-14)
+    from blackwell_backend import workloads as W
+    spo = W.heisenberg_chain(6, seed=1)
+    rp, ci, v = pauli_to_csr_numpy(spo.labels, spo.coeffs, 6)
+    m = sp.csr_matrix((v, ci, rp), shape=(64, 64)).toarray()
+    assert np.allclose(m, OH.pauli_dense(spo.labels, spo.coeffs), atol=1e-14)
+    assert np.allclose(m, m.conj().T)
+    # exact zeros removed: XX+YY cancel on rows where the two bits agree
+    assert len(ci) < 64 * (1 + 5 + 6)
+
+
+def test_chebyshev_coefficients_match_oracle():
+    from blackwell_backend.chebyshev import chebyshev_coefficients
+    for t, b, eps in [(0.5, 3.0, 1e-10), (1.0, 42.0, 1e-7), (10.0, 42.0, 1e-7), (-2.0, 5.0, 1e-12)]:
+        c, K, tail = chebyshev_coefficients(t, b, eps)
+        rc, rK, _ = OH.chebyshev_coefficients(t, b, eps)
+        m = min(K, rK)
+        assert np.max(np.abs(c[:m + 1] - rc[:m + 1])) < 1e-13  # (-i)^k rounding in the restatement
+        assert tail < eps
+        # the series reproduces e^{-i alpha x} on [-1, 1] to the precision
+        x = np.linspace(-1, 1, 33)
+        assert np.max(np.abs(np.polynomial.chebyshev.chebval(x, c) - np.exp(-1j * b * t * x))) < 10 * eps
+
+
+def test_counts_hex_keys():
+    from blackwell_backend.simulator import counts_from_samples
+    c = counts_from_samples([0, 15, 15, 3])
+    assert c == {"0x0": 1, "0x3": 1, "0xf": 2}
+
+
+def test_workloads_shapes_and_determinism():
+    from blackwell_backend import workloads as W
+    qc = W.c1_rotation_ladder()
+    assert qc.size() == 564
+    a = W.su4_volume(28, 28, 2024)
+    b = W.su4_volume(28, 28, 2024)
+    assert a.size() == 392
+    assert all(np.array_equal(x.operation.params[0], y.operation.params[0]) for x, y in zip(a.data, b.data))
+    u = a.data[5].operation.params[0]
+    assert np.allclose(u.conj().T @ u, np.eye(4), atol=1e-13)
+    c3 = W.c3_rotation_mcx(32, 8)
+    assert c3.size() == 8 * (64 + 31 + 4)
+    ns = W.serving_qubit_counts(20000, 1)
+    assert ns.min() >= 10 and ns.max() <= 24
+    assert abs(np.mean(ns == 10) - 0.25 / (1 - 0.75 ** 15)) < 0.02
+    spo = W.heisenberg_chain(22)
+    assert spo.num_qubits == 22 and len(spo.labels) == 3 * 21 + 22

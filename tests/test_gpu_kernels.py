@@ -1,0 +1,425 @@
+"""Kernel-level parity of libqsim_gfx950.so against the FP64 oracle.
+
+Every test feeds the same complex64 input to the GPU kernel and to the
+oracle (up-cast to complex128) and bounds the difference:
+  * permutations (X, CNOT, Toffoli, MCX, SWAP, Fredkin, state init): bit-exact;
+  * compensated dense/diagonal gates: max |delta| <= 1e-6 * max |a|
+    (coefficients are rounded to FP32 on the way in, ~6e-8 each);
+  * MFMA blocks (FP32 accumulation over 64 terms): <= 8e-6 * max |a|;
+  * sampling: identical indices for identical uniforms except at CDF ties
+    (>= 99.9 %), plus a chi-square test.
+"""
+import itertools
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import statevector as O
+from helpers import rand_state, rand_unitary, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel_err(ref, got):
+    ref = np.asarray(ref, dtype=np.complex128)
+    got = np.asarray(got, dtype=np.complex128)
+    return float(np.max(np.abs(ref - got)) / max(np.max(np.abs(ref)), 1e-30))
+
+
+def _bits(x):
+    return np.ascontiguousarray(x).view(np.uint32)
+
+
+def test_state_init(bk, dev):
+    for n in (1, 2, 3, 7, 12, 16):
+        st = bk.state_init(n, device=dev).cpu().numpy()
+        ref = np.zeros(1 << n, dtype=np.complex64)
+        ref[0] = 1
+        assert np.array_equal(_bits(st), _bits(ref)), n
+    sb = bk.state_init_batched(5, 4, device=dev).cpu().numpy()
+    ref = np.zeros((5, 16), dtype=np.complex64)
+    ref[:, 0] = 1
+    assert np.array_equal(_bits(sb), _bits(ref))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 10])
+def test_gate_1q_all_targets(bk, dev, n):
+    for t in range(n):
+        psi = rand_state(n, 100 * n + t)
+        u = rand_unitary(2, t + 7)
+        st = to_dev(psi, dev)
+        bk.apply_gate(st, torch.from_numpy(u.astype(np.complex64)).to(dev), t)
+        ref = O.apply_1q(psi.astype(np.complex128), u.astype(np.complex64).astype(np.complex128), t)
+        assert _rel_err(ref, st.cpu().numpy()) < 1e-6, (n, t)
+
+
+def test_gate_1q_by_value_matches_device_matrix(bk, dev):
+    from blackwell_kernels import _lib
+    n = 11
+    for t in (0, 1, 4, 10):
+        psi = rand_state(n, t)
+        u = rand_unitary(2, 3 + t).astype(np.complex64)
+        a = to_dev(psi, dev)
+        b = to_dev(psi, dev)
+        bk.apply_gate(a, torch.from_numpy(u).to(dev), t)
+        v = np.empty(8)
+        v[0::2] = u.reshape(-1).real
+        v[1::2] = u.reshape(-1).imag
+        _lib.call("qsim_apply_gate_h", _lib.stream_handle(dev), _lib.ptr(b), n, _lib.f64_array(v), t)
+        assert np.array_equal(_bits(a.cpu().numpy()), _bits(b.cpu().numpy()))
+
+
+@pytest.mark.parametrize("n", [2, 3, 6])
+def test_gate_2q_all_pairs(bk, dev, n):
+    for q0, q1 in itertools.permutations(range(n), 2):
+        psi = rand_state(n, 10 * q0 + q1)
+        u = rand_unitary(4, q0 * 31 + q1).astype(np.complex64)
+        st = to_dev(psi, dev)
+        bk.apply_gate_2q(st, torch.from_numpy(u).to(dev), q0, q1)
+        ref = O.apply_2q_kernel(psi.astype(np.complex128), u.astype(np.complex128), q0, q1)
+        assert _rel_err(ref, st.cpu().numpy()) < 1e-6, (n, q0, q1)
+
+
+def test_gate_2q_wide(bk, dev):
+    n = 14
+    for q0, q1 in [(0, 13), (13, 0), (1, 2), (7, 3), (12, 13), (0, 1)]:
+        psi = rand_state(n, q0 + 17 * q1)
+        u = rand_unitary(4, q0 + q1).astype(np.complex64)
+        st = to_dev(psi, dev)
+        bk.apply_gate_2q(st, torch.from_numpy(u).to(dev), q0, q1)
+        ref = O.apply_2q_kernel(psi.astype(np.complex128), u.astype(np.complex128), q0, q1)
+        assert _rel_err(ref, st.cpu().numpy()) < 1e-6
+
+
+def test_diagonals(bk, dev):
+    n = 9
+    rng = np.random.default_rng(3)
+    for t in range(n):
+        psi = rand_state(n, t)
+        d = np.exp(1j * rng.uniform(0, 6.28, 2)).astype(np.complex64)
+        st = to_dev(psi, dev)
+        bk.apply_diagonal(st, torch.from_numpy(d).to(dev), t)
+        assert _rel_err(O.apply_diag(psi.astype(np.complex128), d.astype(np.complex128), t),
+                        st.cpu().numpy()) < 1e-6
+        ph = complex(np.exp(1j * rng.uniform(0, 6.28)))
+        st = to_dev(psi, dev)
+        bk.apply_diagonal_phase(st, ph, t)
+        ref = O.apply_diag_phase(psi.astype(np.complex128), complex(np.complex64(ph)), t)
+        got = st.cpu().numpy()
+        assert _rel_err(ref, got) < 1e-6
+        # the bit=0 half is untouched, bit for bit
+        idx = np.arange(1 << n)
+        zero = ((idx >> t) & 1) == 0
+        assert np.array_equal(_bits(got[zero]), _bits(psi[zero]))
+    for q0, q1 in [(0, 1), (1, 0), (3, 8), (8, 2)]:
+        psi = rand_state(n, q0 * 9 + q1)
+        d = np.exp(1j * rng.uniform(0, 6.28, 4)).astype(np.complex64)
+        st = to_dev(psi, dev)
+        bk.apply_diagonal_2q(st, torch.from_numpy(d).to(dev), q0, q1)
+        ref = O.apply_diag_2q(psi.astype(np.complex128), d.astype(np.complex128), q0, q1)
+        assert _rel_err(ref, st.cpu().numpy()) < 1e-6
+
+
+def test_mcx_mcswap_bit_exact(bk, dev):
+    n = 6
+    psi = rand_state(n, 42)
+    for k in range(0, 4):
+        for combo in itertools.permutations(range(n), k + 1):
+            ctl, t = list(combo[:-1]), combo[-1]
+            st = to_dev(psi, dev)
+            bk.apply_mcx(st, ctl, t)
+            ref = psi[O.mcx_perm(n, ctl, t)]
+            assert np.array_equal(_bits(st.cpu().numpy()), _bits(ref)), (ctl, t)
+    for k in range(0, 3):
+        for combo in itertools.permutations(range(n), k + 2):
+            ctl, a, b = list(combo[:-2]), combo[-2], combo[-1]
+            st = to_dev(psi, dev)
+            bk.apply_mcswap(st, ctl, a, b)
+            ref = psi[O.mcswap_perm(n, ctl, a, b)]
+            assert np.array_equal(_bits(st.cpu().numpy()), _bits(ref)), (ctl, a, b)
+
+
+def test_mcx_seven_controls_wide(bk, dev):
+    n = 16
+    psi = rand_state(n, 5)
+    for ctl, t in [([1, 3, 5, 7, 9, 11, 13], 0), ([0, 2, 4, 6, 8, 10, 12], 15), ([15, 14], 13)]:
+        st = to_dev(psi, dev)
+        bk.apply_mcx(st, ctl, t)
+        assert np.array_equal(_bits(st.cpu().numpy()), _bits(psi[O.mcx_perm(n, ctl, t)]))
+
+
+def test_mcgate_and_mc2qgate(bk, dev):
+    n = 7
+    for ctl, t in [([1], 0), ([0], 3), ([2, 5], 6), ([6, 0, 3], 1), ([], 4)]:
+        psi = rand_state(n, t)
+        u = rand_unitary(2, 11 + t).astype(np.complex64)
+        st = to_dev(psi, dev)
+        bk.apply_mcgate(st, torch.from_numpy(u).to(dev), ctl, t)
+        ref = O.apply_mc(psi.astype(np.complex128), u.astype(np.complex128), ctl, [t])
+        assert _rel_err(ref, st.cpu().numpy()) < 1e-6, (ctl, t)
+    for ctl, a, b in [([0], 1, 2), ([3], 6, 0), ([1, 2], 0, 5), ([], 3, 4)]:
+        psi = rand_state(n, a + b)
+        u = rand_unitary(4, 5 + a).astype(np.complex64)
+        st = to_dev(psi, dev)
+        bk.apply_mc2qgate(st, torch.from_numpy(u).to(dev), ctl, a, b)
+        ref = O.apply_mc2q_kernel(psi.astype(np.complex128), u.astype(np.complex128), ctl, a, b)
+        assert _rel_err(ref, st.cpu().numpy()) < 1e-6, (ctl, a, b)
+
+
+def test_argument_errors(bk, dev):
+    st = bk.state_init(4, device=dev)
+    x = bk.pauli_x_gate(dev)
+    with pytest.raises(RuntimeError):
+        bk.apply_gate(st, x, 4)
+    with pytest.raises(RuntimeError):
+        bk.apply_gate_2q(st, bk.swap_gate(dev), 2, 2)
+    with pytest.raises(RuntimeError, match="controls"):
+        bk.apply_mcgate(st, x, [0], 0)
+    with pytest.raises(RuntimeError):
+        bk.apply_mcx(st, list(range(8)), 9)
+    with pytest.raises(RuntimeError):
+        bk.apply_gate(st.to(torch.complex128), x, 0)
+
+
+def test_gates_fused_and_reg_tiled(bk, dev):
+    n = 13
+    rng = np.random.default_rng(8)
+    targets = [int(t) for t in rng.integers(0, n, 40)]
+    gates = np.stack([rand_unitary(2, 50 + i) for i in range(len(targets))]).astype(np.complex64)
+    psi = rand_state(n, 1)
+    ref = psi.astype(np.complex128)
+    for g, t in zip(gates, targets):
+        ref = O.apply_1q(ref, g.astype(np.complex128), t)
+    st = to_dev(psi, dev)
+    bk.apply_gates_fused(st, torch.from_numpy(gates).to(dev), torch.tensor(targets))
+    assert _rel_err(ref, st.cpu().numpy()) < 1e-5
+    # reg-tiled: targets < 8, device targets like the reference
+    t8 = [int(t) for t in rng.integers(0, 8, 30)]
+    g8 = np.stack([rand_unitary(2, 90 + i) for i in range(len(t8))]).astype(np.complex64)
+    ref = psi.astype(np.complex128)
+    for g, t in zip(g8, t8):
+        ref = O.apply_1q(ref, g.astype(np.complex128), t)
+    st = to_dev(psi, dev)
+    bk.apply_gates_reg_tiled(st, torch.from_numpy(g8).to(dev),
+                             torch.tensor(t8, dtype=torch.int32, device=dev))
+    assert _rel_err(ref, st.cpu().numpy()) < 1e-5
+    # batched: same list on 3 states of 9 qubits
+    ps = np.stack([rand_state(9, s) for s in range(3)])
+    sts = to_dev(ps, dev)
+    t7 = [t % 8 for t in t8]
+    bk.apply_gates_batched_fused(sts, torch.from_numpy(g8).to(dev),
+                                 torch.tensor(t7, dtype=torch.int32, device=dev))
+    for i in range(3):
+        ref = ps[i].astype(np.complex128)
+        for g, t in zip(g8, t7):
+            ref = O.apply_1q(ref, g.astype(np.complex128), t)
+        assert _rel_err(ref, sts[i].cpu().numpy()) < 1e-5
+    u = rand_unitary(2, 1).astype(np.complex64)
+    sts = to_dev(ps, dev)
+    bk.apply_gate_batched(sts, torch.from_numpy(u).to(dev), 5)
+    for i in range(3):
+        assert _rel_err(O.apply_1q(ps[i].astype(np.complex128), u.astype(np.complex128), 5),
+                        sts[i].cpu().numpy()) < 1e-6
+
+
+def test_reductions_and_measure(bk, dev):
+    for n in (1, 3, 10, 15):
+        psi = rand_state(n, n) * np.float32(1.7)
+        st = to_dev(psi, dev)
+        ref = np.sum(np.abs(psi.astype(np.complex128)) ** 2)
+        assert abs(bk.norm2(st) - ref) <= 1e-12 * ref
+        pr = bk.measure_probs(st).cpu().numpy()
+        assert np.allclose(pr, np.abs(psi.astype(np.complex128)) ** 2, rtol=1e-6, atol=0)
+        bk.renormalize(st)
+        got = st.cpu().numpy()
+        assert abs(np.sum(np.abs(got.astype(np.complex128)) ** 2) - 1) < 1e-6
+        for t in range(n):
+            for u in (0.1, 0.77):
+                s2 = to_dev(psi, dev)
+                o = bk.measure_qubit(s2, t, u)
+                ro, rs = O.measure(psi.astype(np.complex128), t, u)
+                assert o == ro
+                assert _rel_err(rs, s2.cpu().numpy()) < 1e-6
+    # determinism: same state, same bits
+    psi = rand_state(18, 3)
+    a = bk.norm2(to_dev(psi, dev))
+    b = bk.norm2(to_dev(psi, dev))
+    assert a == b
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 8, 9, 12, 16])
+def test_sample_matches_oracle(bk, dev, n):
+    rng = np.random.default_rng(n)
+    psi = rand_state(n, 77 + n)
+    if n >= 5:
+        psi[rng.choice(1 << n, (1 << n) // 3, replace=False)] = 0  # zero-probability outcomes
+    u = rng.random(20000)
+    got = bk.sample_with_uniforms(to_dev(psi, dev), torch.from_numpy(u)).cpu().numpy()
+    ref = O.sample(psi.astype(np.complex128), u)
+    assert np.mean(got == ref) >= 0.999
+    p = np.abs(psi.astype(np.complex128)) ** 2
+    assert np.all(p[got] > 0)              # never a zero-probability index
+    pval, _, _ = O.chi_square_p(got, p, len(u))
+    assert pval > 1e-3
+
+
+def test_tile_program_random(bk, dev):
+    from blackwell_backend import compiler as C
+    from blackwell_backend.executor import tile_ops
+    rng = np.random.default_rng(12)
+    n = 14
+    for trial in range(4):
+        m = [10, 12, 13, 14][trial]
+        extra = rng.choice(np.arange(3, n), m - 3, replace=False)
+        mask = 0b111
+        for q in extra:
+            mask |= 1 << int(q)
+        tq = [q for q in range(n) if mask >> q & 1]
+        ops = []
+        for i in range(25):
+            kind = rng.choice(["m1", "d1", "ph", "m2", "x", "swap", "mk"])
+            qs = [int(q) for q in rng.choice(tq, 5, replace=False)]
+            if kind == "m1":
+                ops.append(C.Op("m1", qs[:1], rand_unitary(2, i), controls=qs[1:1 + i % 3]))
+            elif kind == "d1":
+                ops.append(C.Op("d1", qs[:1], np.exp(1j * rng.uniform(0, 6, 2))))
+            elif kind == "ph":
+                ops.append(C.Op("ph", qs[:1], np.exp(1j * rng.uniform(0, 6, 1)), controls=qs[1:2]))
+            elif kind == "m2":
+                ops.append(C.Op("m2", qs[:2], rand_unitary(4, i)))
+            elif kind == "x":
+                ops.append(C.Op("x", qs[:1], controls=qs[1:1 + i % 4]))
+            elif kind == "swap":
+                ops.append(C.Op("swap", qs[:2], controls=qs[2:2 + i % 2]))
+            else:
+                k = 3 + i % 3
+                ops.append(C.Op("mk", qs[:k], rand_unitary(1 << k, i)))
+        psi = rand_state(n, trial)
+        ref = psi.astype(np.complex128)
+        for op in ops:
+            m64 = op.full_matrix().astype(np.complex64).astype(np.complex128)
+            ref = O.apply_matrix(ref, m64, list(op.all_qubits))
+        arr, cf = tile_ops(ops, mask)
+        st = to_dev(psi, dev)
+        bk.run_tile_program(st, mask, arr, cf.astype(np.complex64))
+        assert _rel_err(ref, st.cpu().numpy()) < 2e-5, trial
+
+
+@pytest.mark.parametrize("n,qubits", [(10, [0, 1, 2, 3, 4]), (10, [9, 5, 0, 7, 2]),
+                                      (12, [11, 10, 9, 8, 7]), (16, [3, 15, 8, 0, 11])])
+def test_block_mfma(bk, dev, n, qubits):
+    psi = rand_state(n, n + qubits[0])
+    u = rand_unitary(32, sum(qubits)).astype(np.complex64)
+    st = to_dev(psi, dev)
+    bk.apply_block(st, torch.from_numpy(u).to(dev), qubits)
+    ref = O.apply_matrix(psi.astype(np.complex128), u.astype(np.complex128), qubits)
+    assert _rel_err(ref, st.cpu().numpy()) < 8e-6
+
+
+def test_qv8_qv4(bk, dev):
+    m, q, g = bk.generate_qv8_circuits(37, seed=3)
+    got = bk.qv8_simulate(m.to(dev), q.to(dev), 37).cpu().numpy()
+    for c in range(0, 37, 6):
+        psi = O.zero_state(8)
+        for k in range(g):
+            u = (m[c, k, ..., 0].numpy() + 1j * m[c, k, ..., 1].numpy()).astype(np.complex128)
+            psi = O.apply_matrix(psi, u, [int(q[c, k, 0]), int(q[c, k, 1])])
+        assert np.allclose(got[c], O.probs(psi), atol=2e-6)
+    gd, pid = bk.generate_qv4_circuits(40, rng=np.random.default_rng(2))
+    got = bk.qv4_simulate_cuda(gd, pid).cpu().numpy()
+    ref = bk.qv4_simulate_numpy(gd, pid)
+    assert np.allclose(got, ref, atol=2e-6)
+
+
+def test_run_instances_with_sampling(bk, dev):
+    from blackwell_kernels import _lib
+    from blackwell_backend import compiler as C
+    from blackwell_backend.executor import tile_ops
+    rng = np.random.default_rng(4)
+    ns = [1, 3, 8, 10, 12, 14, 9, 13]
+    all_ops, all_cf, inst, refs = [], [], [], []
+    op_off = cf_off = st_off = shot_off = 0
+    shots = 3000
+    for i, n in enumerate(ns):
+        ops = []
+        for j in range(12):
+            if n >= 2 and j % 2:
+                a, b = rng.choice(n, 2, replace=False)
+                ops.append(C.Op("m2", [int(a), int(b)], rand_unitary(4, 10 * i + j)))
+            else:
+                ops.append(C.Op("m1", [int(rng.integers(n))], rand_unitary(2, 10 * i + j)))
+        arr, cf = tile_ops(ops, (1 << n) - 1)
+        arr["coef"] += cf_off
+        rec = np.zeros(1, dtype=_lib.INSTANCE_DTYPE)
+        rec["state_off"] = st_off
+        rec["seed"] = 1000 + i
+        rec["shot_off"] = shot_off
+        rec["op_off"] = op_off
+        rec["n_ops"] = len(arr)
+        rec["shots"] = shots
+        rec["n_qubits"] = n
+        rec["init"] = 1
+        rec["want"] = 1
+        inst.append(rec)
+        all_ops.append(arr)
+        all_cf.append(cf)
+        psi = O.zero_state(n)
+        for op in ops:
+            psi = O.apply_matrix(psi, op.full_matrix().astype(np.complex64).astype(np.complex128),
+                                 list(op.all_qubits))
+        refs.append(psi)
+        op_off += len(arr)
+        cf_off += cf.size
+        st_off += 1 << n
+        shot_off += shots
+    states = torch.zeros(st_off, dtype=torch.complex64, device=dev)
+    out = torch.full((shot_off,), -1, dtype=torch.int64, device=dev)
+    inst_d = torch.from_numpy(np.concatenate(inst).view(np.uint8).copy()).to(dev)
+    ops_d = torch.from_numpy(np.concatenate(all_ops).view(np.uint8).copy()).to(dev)
+    cf_d = torch.from_numpy(np.concatenate(all_cf).astype(np.complex64)).to(dev)
+    _lib.call("qsim_run_instances", _lib.stream_handle(dev), _lib.ptr(states), _lib.ptr(inst_d),
+              len(ns), _lib.ptr(ops_d), _lib.ptr(cf_d), _lib.ptr(out), max(ns))
+    st = states.cpu().numpy()
+    sh = out.cpu().numpy()
+    off = 0
+    for i, n in enumerate(ns):
+        got = st[off:off + (1 << n)]
+        assert _rel_err(refs[i], got) < 2e-5, n
+        u = O.splitmix_uniforms(1000 + i, shots)
+        ref_idx = O.sample(got.astype(np.complex128), u)
+        g = sh[i * shots:(i + 1) * shots]
+        assert np.mean(g == ref_idx) >= 0.999, n
+        off += 1 << n
+
+
+def test_spmv_and_chebyshev_kernels(bk, dev):
+    import scipy.sparse as sp
+    rng = np.random.default_rng(0)
+    N = 700
+    A = sp.random(N, N, density=0.02, random_state=1, format="csr")
+    A = (A + 1j * sp.random(N, N, density=0.02, random_state=2, format="csr")).tocsr()
+    A.sort_indices()
+    x = (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(np.complex64)
+    y = (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(np.complex64)
+    rp = torch.from_numpy(A.indptr.astype(np.int32)).to(dev)
+    ci = torch.from_numpy(A.indices.astype(np.int32)).to(dev)
+    va = torch.from_numpy(A.data.astype(np.complex64)).to(dev)
+    yd = to_dev(y, dev)
+    bk.spmv_csr(rp, ci, va, to_dev(x, dev), yd, 0.5, -0.25, 0.1, 0.3)
+    Av = A.data.astype(np.complex64).astype(np.complex128)
+    A64 = sp.csr_matrix((Av, A.indices, A.indptr), shape=A.shape)
+    ref = (0.5 - 0.25j) * (A64 @ x.astype(np.complex128)) + (0.1 + 0.3j) * y
+    assert np.max(np.abs(ref - yd.cpu().numpy())) < 1e-5 * np.max(np.abs(ref))
+    s, tp = to_dev(x, dev), to_dev(y, dev)
+    tn = torch.empty_like(s)
+    bk.chebyshev_step(s, tp, tn)
+    assert np.allclose(tn.cpu().numpy(), 2 * x - y, atol=1e-6)
+    acc = to_dev(y, dev)
+    bk.chebyshev_accum(acc, s, 0.3, -0.7)
+    assert np.allclose(acc.cpu().numpy(), y + (0.3 - 0.7j) * x, atol=1e-6)
+    acc = to_dev(y, dev)
+    bk.chebyshev_step_accum(s, tp, tn, acc, 0.2, 0.1)
+    t2 = 2 * x - y
+    assert np.allclose(tn.cpu().numpy(), t2, atol=1e-6)
+    assert np.allclose(acc.cpu().numpy(), y + (0.2 + 0.1j) * t2, atol=1e-5)
