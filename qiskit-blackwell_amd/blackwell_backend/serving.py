@@ -1,0 +1,399 @@
+"""Continuous-batching circuit server on one MI355X.
+
+Requests (circuit, shots, seed) arrive over time.  The server admits them as
+capacity frees up, keeps every state resident on the device from the first
+gate to the last shot, and returns seeded samples.  Two lanes run
+concurrently on separate HIP streams:
+
+  * LDS lane (n <= 14): many circuits per launch of qsim_run_instances, one
+    workgroup per circuit with its whole state in LDS, its own gate table,
+    and in-kernel sampling.  Nothing but the samples touches HBM.
+  * HBM lane (n > 14): each circuit in its own slab of HBM, run as fused
+    5-qubit blocks on FP32 MFMA, then sampled with qsim_sample.
+
+Admission is by statevector bytes against an HBM budget (default 80 % of
+the free device memory; tests inject a small one).  A request that arrives
+while others run is admitted at the next scheduling point if its bytes fit;
+LDS-lane requests are grouped into a launch as soon as the previous LDS
+launch has been handed to the device, so their work overlaps with the
+running batches and the HBM lane.
+
+Sampling in both lanes uses the same rule and the same uniforms for a given
+seed: u_k = SplitMix64(seed + (k+1) * golden) >> 11 scaled by 2^-53 and
+index = smallest i with cdf(i) > u_k * ||psi||^2.  Each instance's arithmetic
+is independent of the batch it runs in (one workgroup per instance, fixed
+thread count), so a request's samples are bitwise the same alone and in a
+mix.
+
+The reference has no serving loop; its batched precedent is fixed-n and
+single-launch (bwk/csrc/qv8_sim_sm120.cu:58-151).
+"""
+import collections
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from blackwell_kernels import _lib
+from blackwell_kernels._lib import call, OP_DTYPE, INSTANCE_DTYPE
+from . import compiler as C
+from .circuit import instructions
+from .executor import CompiledCircuit, tile_ops
+from . import workloads as W
+
+LDS_MAX_QUBITS = 14
+GOLDEN = 0x9E3779B97F4A7C15
+
+
+def splitmix_uniforms(seed, count):
+    """Host copy of the in-kernel uniforms (the HBM lane's sampler input)."""
+    with np.errstate(over="ignore"):
+        k = np.arange(1, count + 1, dtype=np.uint64)
+        z = np.uint64(seed) + k * np.uint64(GOLDEN)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+
+
+class Request:
+    """One circuit to serve.  Either `circuit` (any circuit with Qiskit's
+    data layout) or a (`program`, `n_qubits`) pair from a program builder."""
+    __slots__ = ("rid", "arrival", "n_qubits", "shots", "seed", "circuit", "program",
+                 "t_admit", "t_done", "samples", "lane")
+
+    def __init__(self, rid, arrival, n_qubits, shots, seed, circuit=None, program=None):
+        self.rid = rid
+        self.arrival = float(arrival)
+        self.n_qubits = int(n_qubits)
+        self.shots = int(shots)
+        self.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        self.circuit = circuit
+        self.program = program
+        self.t_admit = None
+        self.t_done = None
+        self.samples = None
+        self.lane = None
+
+    def state_bytes(self):
+        return 8 << self.n_qubits
+
+    def counts(self):
+        v, c = np.unique(self.samples, return_counts=True)
+        return {hex(int(a)): int(b) for a, b in zip(v, c)}
+
+
+# ---------------------------------------------------------------- programs
+def su4_program(n, depth, seed):
+    """Whole-state LDS program (OP_DTYPE ops, complex128 coefs) of
+    workloads.su4_volume(n, depth, seed), built without per-gate objects."""
+    layers = W.su4_layers(n, depth, seed)
+    pairs = np.array([p for ps, _ in layers for p in ps], dtype=np.int64).reshape(-1, 2)
+    us = np.concatenate([u for _, u in layers]).reshape(-1, 16)
+    g = len(pairs)
+    ops = np.zeros(g, dtype=OP_DTYPE)
+    ops["kind"] = _lib.OP_M2
+    ops["nq"] = 2
+    ops["q"][:, 0] = pairs[:, 0]
+    ops["q"][:, 1] = pairs[:, 1]
+    ops["coef"] = np.arange(g, dtype=np.uint32) * 16
+    return ops, us.reshape(-1)
+
+
+def circuit_program(circuit):
+    """Whole-state LDS program of an arbitrary circuit (via the compiler)."""
+    ops = C.compile_ops(instructions(circuit))
+    if any(o.kind in ("measure", "reset") for o in ops):
+        raise ValueError("served circuits may not contain mid-circuit measure/reset")
+    n = circuit.num_qubits
+    return tile_ops(ops, (1 << n) - 1)
+
+
+def su4_request(rid, arrival, n, shots, seed):
+    """Request for the random-SU(4) volume circuit of depth n."""
+    return Request(rid, arrival, n, shots, seed, program=("su4", n, n, seed))
+
+
+def _materialise(req):
+    """(ops, coefs) for the LDS lane."""
+    p = req.program
+    if p is not None and isinstance(p, tuple) and p and p[0] == "su4":
+        return su4_program(p[1], p[2], p[3])
+    if p is not None:
+        return p
+    return circuit_program(req.circuit)
+
+
+def _circuit_of(req):
+    if req.circuit is not None:
+        return req.circuit
+    p = req.program
+    if isinstance(p, tuple) and p and p[0] == "su4":
+        return W.su4_volume(p[1], p[2], p[3])
+    raise ValueError("request has neither a circuit nor a rebuildable program")
+
+
+# ---------------------------------------------------------------- LDS lane
+class _LdsBatch:
+    def __init__(self, reqs, device, stream):
+        self.reqs = reqs
+        self.device = device
+        self.stream = stream
+        self.event = None
+
+    def launch(self):
+        inst = np.zeros(len(self.reqs), dtype=INSTANCE_DTYPE)
+        all_ops, all_cf = [], []
+        op_off = cf_off = shot_off = 0
+        max_n = 1
+        for i, r in enumerate(self.reqs):
+            ops, cf = _materialise(r)
+            ops = ops.copy()
+            ops["coef"] += cf_off
+            inst[i]["seed"] = r.seed
+            inst[i]["shot_off"] = shot_off
+            inst[i]["op_off"] = op_off
+            inst[i]["n_ops"] = len(ops)
+            inst[i]["shots"] = r.shots
+            inst[i]["n_qubits"] = r.n_qubits
+            inst[i]["init"] = 1
+            inst[i]["want"] = 0
+            all_ops.append(ops)
+            all_cf.append(np.asarray(cf, dtype=np.complex64))
+            op_off += len(ops)
+            cf_off += len(cf)
+            shot_off += r.shots
+            max_n = max(max_n, r.n_qubits)
+        blob_ops = np.concatenate(all_ops).view(np.uint8)
+        blob_cf = np.concatenate(all_cf) if cf_off else np.zeros(1, np.complex64)
+        with torch.cuda.stream(self.stream):
+            self.d_inst = torch.from_numpy(inst.view(np.uint8).copy()).to(self.device)
+            self.d_ops = torch.from_numpy(blob_ops.copy()).to(self.device)
+            self.d_cf = torch.from_numpy(blob_cf).to(self.device)
+            self.d_out = torch.empty(max(shot_off, 1), dtype=torch.int64, device=self.device)
+            dummy = self.d_out  # no HBM states in this lane (want = 0)
+            call("qsim_run_instances", ctypes.c_void_p(self.stream.cuda_stream),
+                 ctypes.c_void_p(dummy.data_ptr()), ctypes.c_void_p(self.d_inst.data_ptr()),
+                 len(self.reqs), ctypes.c_void_p(self.d_ops.data_ptr()),
+                 ctypes.c_void_p(self.d_cf.data_ptr()), ctypes.c_void_p(self.d_out.data_ptr()),
+                 max_n)
+            self.h_out = torch.empty(self.d_out.numel(), dtype=torch.int64, pin_memory=True)
+            self.h_out.copy_(self.d_out, non_blocking=True)
+            self.event = torch.cuda.Event()
+            self.event.record(self.stream)
+
+    def done(self):
+        return self.event is not None and self.event.query()
+
+    def finish(self, now):
+        self.event.synchronize()
+        out = self.h_out.numpy()
+        off = 0
+        for r in self.reqs:
+            r.samples = out[off:off + r.shots].copy()
+            r.t_done = now
+            off += r.shots
+
+
+# ---------------------------------------------------------------- HBM lane
+class _HbmJob:
+    def __init__(self, req, sim, stream):
+        self.req = req
+        self.reqs = [req]
+        self.sim = sim
+        self.stream = stream
+        self.event = None
+
+    def launch(self):
+        r = self.req
+        dev = self.sim.device
+        cc = self.sim.compile(_circuit_of(r))
+        plan = cc.plan(cc.resolve_mode("blocks"), dev)
+        with torch.cuda.stream(self.stream):
+            self.state = torch.empty(1 << r.n_qubits, dtype=torch.complex64, device=dev)
+            call("qsim_state_init", ctypes.c_void_p(self.stream.cuda_stream),
+                 ctypes.c_void_p(self.state.data_ptr()), r.n_qubits)
+            plan.run(self.state)
+            u = torch.from_numpy(splitmix_uniforms(r.seed, r.shots)).to(dev)
+            from blackwell_kernels import sample_with_uniforms
+            self.d_out = sample_with_uniforms(self.state, u)
+            self.h_out = torch.empty(r.shots, dtype=torch.int64, pin_memory=True)
+            self.h_out.copy_(self.d_out, non_blocking=True)
+            self.event = torch.cuda.Event()
+            self.event.record(self.stream)
+
+    def done(self):
+        return self.event is not None and self.event.query()
+
+    def finish(self, now):
+        self.event.synchronize()
+        self.req.samples = self.h_out.numpy().copy()
+        self.req.t_done = now
+        self.state = None
+
+
+# ---------------------------------------------------------------- server
+class Server:
+    """Continuous-batching server.
+
+    budget_bytes   HBM budget for resident states (None: 80 % of free memory)
+    max_batch      most requests per LDS launch
+    max_hbm_jobs   HBM-lane circuits in flight at once
+    """
+
+    def __init__(self, sim=None, budget_bytes=None, max_batch=2048, max_hbm_jobs=4,
+                 device="cuda:0", lds_factory=None, hbm_factory=None):
+        """lds_factory(reqs) / hbm_factory(req) build lane jobs with launch(),
+        done() and finish(now); the defaults run on the GPU, tests inject
+        host-only fakes to drive the scheduling logic."""
+        fake = lds_factory is not None and hbm_factory is not None
+        if sim is None and not fake:
+            from .simulator import BlackwellSimulator
+            sim = BlackwellSimulator(device)
+        self.sim = sim
+        self.device = sim.device if sim is not None else None
+        if budget_bytes is None:
+            free, _ = torch.cuda.mem_get_info(self.device)
+            budget_bytes = int(0.8 * free)
+        self.budget = int(budget_bytes)
+        self.max_batch = int(max_batch)
+        self.max_hbm_jobs = int(max_hbm_jobs)
+        self.in_use = 0
+        self.peak_in_use = 0
+        if fake:
+            self.lds_factory, self.hbm_factory = lds_factory, hbm_factory
+        else:
+            self.lds_stream = torch.cuda.Stream(self.device)
+            self.hbm_stream = torch.cuda.Stream(self.device)
+            self.lds_factory = lambda reqs: _LdsBatch(reqs, self.device, self.lds_stream)
+            self.hbm_factory = lambda req: _HbmJob(req, self.sim, self.hbm_stream)
+        self.rejected = []
+        self.launches = 0
+
+    def lane_of(self, req):
+        return "lds" if req.n_qubits <= LDS_MAX_QUBITS else "hbm"
+
+    def admit_bytes(self, req):
+        # LDS-lane circuits never hold an HBM state; their sample buffer is tiny
+        return 8 * req.shots if self.lane_of(req) == "lds" else req.state_bytes() + 8 * req.shots + (1 << 20)
+
+    def serve(self, requests, realtime=True, timeout_s=600.0):
+        """Serve requests (arrival times relative to the start).  Returns
+        stats; every request gets .samples, .t_admit, .t_done (seconds from
+        the start)."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        self.rejected = [r for r in reqs if self.admit_bytes(r) > self.budget]
+        rej = {id(r) for r in self.rejected}
+        pending = collections.deque(r for r in reqs if id(r) not in rej)
+        queue_lds, queue_hbm = collections.deque(), collections.deque()
+        lds_inflight, hbm_inflight = [], []
+        t0 = time.perf_counter()
+        clock = (lambda: time.perf_counter() - t0) if realtime else None
+        done = 0
+        total = len(pending) + 0
+        while done < total:
+            now = clock() if realtime else float("inf")
+            while pending and pending[0].arrival <= now:
+                r = pending.popleft()
+                r.lane = self.lane_of(r)
+                (queue_lds if r.lane == "lds" else queue_hbm).append(r)
+            # retire finished work, freeing budget
+            for b in [b for b in lds_inflight if b.done()]:
+                b.finish(clock() if realtime else time.perf_counter() - t0)
+                lds_inflight.remove(b)
+                for r in b.reqs:
+                    self.in_use -= self.admit_bytes(r)
+                done += len(b.reqs)
+            for j in [j for j in hbm_inflight if j.done()]:
+                j.finish(clock() if realtime else time.perf_counter() - t0)
+                hbm_inflight.remove(j)
+                for r in j.reqs:
+                    self.in_use -= self.admit_bytes(r)
+                done += len(j.reqs)
+            launched = False
+            # LDS lane: one launch in flight at a time, everything queued goes in
+            if queue_lds and len(lds_inflight) < 2:
+                batch = []
+                while queue_lds and len(batch) < self.max_batch:
+                    r = queue_lds[0]
+                    need = self.admit_bytes(r)
+                    if self.in_use + need > self.budget:
+                        break
+                    queue_lds.popleft()
+                    self.in_use += need
+                    r.t_admit = time.perf_counter() - t0
+                    batch.append(r)
+                if batch:
+                    b = self.lds_factory(batch)
+                    b.launch()
+                    self.launches += 1
+                    self.peak_in_use = max(self.peak_in_use, self.in_use)
+                    lds_inflight.append(b)
+                    launched = True
+            # HBM lane
+            while queue_hbm and len(hbm_inflight) < self.max_hbm_jobs:
+                r = queue_hbm[0]
+                need = self.admit_bytes(r)
+                if self.in_use + need > self.budget:
+                    break
+                queue_hbm.popleft()
+                self.in_use += need
+                r.t_admit = time.perf_counter() - t0
+                j = self.hbm_factory(r)
+                j.launch()
+                self.launches += 1
+                self.peak_in_use = max(self.peak_in_use, self.in_use)
+                hbm_inflight.append(j)
+                launched = True
+            if time.perf_counter() - t0 > timeout_s:
+                raise TimeoutError(f"serving did not finish within {timeout_s} s ({done}/{total} done)")
+            if not launched:
+                if lds_inflight or hbm_inflight:
+                    time.sleep(0.0002)
+                elif pending and realtime:
+                    time.sleep(max(0.0, min(0.001, pending[0].arrival - clock())))
+        elapsed = time.perf_counter() - t0
+        return summarize(reqs, elapsed, self.rejected)
+
+
+def summarize(reqs, elapsed, rejected=()):
+    served = [r for r in reqs if r.t_done is not None]
+    lat = np.array([r.t_done - r.arrival for r in served]) if served else np.zeros(1)
+    first = min((r.arrival for r in served), default=0.0)
+    last = max((r.t_done for r in served), default=elapsed)
+    span = max(last - first, 1e-9)
+    return {
+        "served": len(served),
+        "rejected": len(rejected),
+        "circuits_per_s": len(served) / span,
+        "span_s": span,
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "mean_latency_s": float(np.mean(lat)),
+    }
+
+
+def serve_plain(requests, sim, realtime=True):
+    """The plain baseline: one circuit at a time, in arrival order, through
+    the per-gate kernels, then qsim_sample with the same uniforms."""
+    from blackwell_kernels import sample_with_uniforms
+    reqs = sorted(requests, key=lambda r: r.arrival)
+    dev = sim.device
+    t0 = time.perf_counter()
+    for r in reqs:
+        if realtime:
+            wait = r.arrival - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+        r.t_admit = time.perf_counter() - t0
+        cc = sim.compile(_circuit_of(r))
+        plan = cc.plan("plain", dev)
+        st = torch.empty(1 << r.n_qubits, dtype=torch.complex64, device=dev)
+        call("qsim_state_init", _lib.stream_handle(dev), ctypes.c_void_p(st.data_ptr()), r.n_qubits)
+        plan.run(st)
+        u = torch.from_numpy(splitmix_uniforms(r.seed, r.shots)).to(dev)
+        r.samples = sample_with_uniforms(st, u).cpu().numpy()
+        r.t_done = time.perf_counter() - t0
+        r.lane = "plain"
+    return summarize(reqs, time.perf_counter() - t0)

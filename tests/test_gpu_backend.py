@@ -1,0 +1,203 @@
+"""BlackwellSimulator on the GPU against the FP64 oracle.
+
+Bars: fidelity >= 1 - 1e-7 (complex128 up-cast before normalising) and the
+phase-aligned max |delta amp| reported and bounded; permutation circuits
+bit-exact; shot counts pass chi-square (p > 1e-3) against the FP64
+distribution; Chebyshev evolution within 1e-5 of scipy's expm_multiply.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import hamiltonian as OH
+from oracle import statevector as O
+
+pytestmark = pytest.mark.gpu
+
+MODES = ("auto", "plain", "tiles", "blocks")
+
+
+@pytest.fixture(scope="module")
+def sim():
+    from blackwell_backend import BlackwellSimulator
+    return BlackwellSimulator("cuda:0")
+
+
+def oracle_sv(qc):
+    from blackwell_backend.circuit import instructions
+    return O.run_instructions(qc.num_qubits, [(nm, q, p) for nm, q, _, p in instructions(qc)])
+
+
+def check(ref, sv, f_min=1 - 1e-7, d_max=2e-6):
+    f = O.fidelity(ref, sv)
+    d = O.max_abs_diff_phase_aligned(ref, sv)
+    assert f >= f_min and d <= d_max, (f, d)
+    return f, d
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_config1_rotation_ladder_12q(sim, mode):
+    from blackwell_backend import workloads as W
+    qc = W.c1_rotation_ladder(12, 12, seed=1234)
+    ref = oracle_sv(qc)
+    sv, _ = sim.run_statevector(qc, mode=mode)
+    f, d = check(ref, sv)
+    print(f"C1 {mode}: fidelity {f:.12f} max|d| {d:.3e}")
+
+
+@pytest.mark.parametrize("mode", ["plain", "blocks", "tiles"])
+def test_su4_volume_20q(sim, mode):
+    from blackwell_backend import workloads as W
+    qc = W.su4_volume(20, 20, seed=2024)
+    ref = oracle_sv(qc)
+    sv, _ = sim.run_statevector(qc, mode=mode)
+    f, d = check(ref, sv, d_max=5e-6)
+    print(f"SU4-20 {mode}: fidelity {f:.12f} max|d| {d:.3e}")
+
+
+@pytest.mark.parametrize("mode", ["plain", "blocks", "tiles"])
+def test_rotation_mcx_16q(sim, mode):
+    from blackwell_backend import workloads as W
+    qc = W.c3_rotation_mcx(16, 3, seed=77)
+    ref = oracle_sv(qc)
+    sv, _ = sim.run_statevector(qc, mode=mode)
+    f, d = check(ref, sv)
+    print(f"rot-MCX-16 {mode}: fidelity {f:.12f} max|d| {d:.3e}")
+
+
+def test_random_circuits_all_modes(sim):
+    from test_compiler import random_circuit
+    for seed, n in [(1, 6), (2, 11), (3, 14)]:
+        qc = random_circuit(n, 200, seed)
+        ref = oracle_sv(qc)
+        for mode in MODES:
+            sv, _ = sim.run_statevector(qc, mode=mode)
+            check(ref, sv)
+
+
+def test_permutation_circuit_bit_exact(sim):
+    from blackwell_backend.circuit import QuantumCircuit
+    rng = np.random.default_rng(0)
+    n = 14
+    qc = QuantumCircuit(n)
+    qc.x(3).x(0).x(13)
+    for _ in range(60):
+        qs = [int(q) for q in rng.choice(n, 4, replace=False)]
+        r = rng.integers(4)
+        if r == 0:
+            qc.cx(qs[0], qs[1])
+        elif r == 1:
+            qc.ccx(qs[0], qs[1], qs[2])
+        elif r == 2:
+            qc.swap(qs[0], qs[1])
+        else:
+            qc.cswap(qs[0], qs[1], qs[2])
+    ref = oracle_sv(qc).astype(np.complex64)
+    for mode in ("plain", "tiles", "auto"):
+        sv, _ = sim.run_statevector(qc, mode=mode)
+        assert np.array_equal(sv.view(np.uint32), ref.view(np.uint32)), mode
+
+
+def test_stress_repeated_gates_through_the_abi(bk, dev):
+    n = 8
+    psi = np.zeros(1 << n, dtype=np.complex64)
+    psi[0] = 1
+    st = torch.from_numpy(psi.copy()).to(dev)
+    h = bk.hadamard_gate(dev)
+    for _ in range(1000):
+        bk.apply_gate(st, h, 3)
+    got = st.cpu().numpy()
+    assert abs(got[0] - 1) < 1e-5 and np.sum(np.abs(got[1:])) < 1e-5
+    st = torch.from_numpy(psi.copy()).to(dev)
+    for _ in range(999):
+        bk.apply_mcx(st, [], 5)
+    ref = np.zeros_like(psi)
+    ref[1 << 5] = 1
+    assert np.array_equal(st.cpu().numpy(), ref)
+    st = torch.from_numpy(psi.copy()).to(dev)
+    bk.apply_mcx(st, [], 1)
+    for _ in range(1000):
+        bk.apply_mcx(st, [1], 4)
+    ref = np.zeros_like(psi)
+    ref[2] = 1
+    assert np.array_equal(st.cpu().numpy(), ref)
+    st = torch.from_numpy((np.ones(1 << n) / 16).astype(np.complex64)).to(dev)
+    d = bk.rz_gate_diag(0.001, dev)
+    for _ in range(10000):
+        bk.apply_diagonal(st, d, 2)
+    ref = O.apply_diag(np.ones(1 << n) / 16, [np.exp(-5j), np.exp(5j)], 2)
+    assert O.fidelity(ref, st.cpu().numpy()) > 0.99999
+
+
+def test_run_counts_and_chi_square(sim):
+    from blackwell_backend.circuit import QuantumCircuit
+    qc = QuantumCircuit(2, 2, name="bell")
+    qc.h(0).cx(0, 1).measure_all()
+    res = sim.run(qc, shots=50000, seed=7)
+    counts = res.get_counts()
+    assert set(counts) <= {"0x0", "0x3"}
+    assert sum(counts.values()) == 50000
+    assert res.results[0].header["n_qubits"] == 2
+    again = sim.run(qc, shots=50000, seed=7).get_counts()
+    assert again == counts
+    from blackwell_backend import workloads as W
+    qc = W.su4_volume(10, 10, seed=5)
+    ref = oracle_sv(qc)
+    res = sim.run(qc, shots=200000, seed=3)
+    idx = np.concatenate([[int(k, 16)] * v for k, v in res.get_counts().items()])
+    p, _, _ = O.chi_square_p(idx, O.probs(ref), 200000)
+    assert p > 1e-3
+
+
+def test_midcircuit_measure_and_reset(sim):
+    from blackwell_backend.circuit import QuantumCircuit
+    qc = QuantumCircuit(3, 1)
+    qc.h(0).cx(0, 1).measure(0, 0).reset(2).h(2)
+    for seed in range(4):
+        torch.manual_seed(seed)
+        sv, _ = sim.run_statevector(qc)
+        p = np.abs(sv) ** 2
+        # qubits 0 and 1 collapsed together; qubit 2 in |+>
+        assert np.isclose(p.sum(), 1, atol=1e-6)
+        on = {i for i in range(8) if p[i] > 1e-6}
+        assert on in ({0, 4}, {3, 7})
+
+
+def test_hamiltonian_evolution_matches_expm(sim):
+    from blackwell_backend import workloads as W
+    from blackwell_backend.hamiltonian import HamiltonianCSR
+    from blackwell_backend.chebyshev import chebyshev_evolve_batch
+    spo = W.heisenberg_chain(10, seed=3)
+    h = OH.pauli_sparse(spo.labels, spo.coeffs)
+    psi0 = np.zeros(1 << 10, dtype=np.complex128)
+    psi0[0] = 1
+    for t in (0.5, 2.0):
+        sv, info = sim.run_hamiltonian(spo, t, precision=1e-8)
+        ref = OH.expm_reference(h, psi0, t)
+        assert np.max(np.abs(sv - ref)) < 1e-5, (t, info)
+        assert info["engine"] == "chebyshev-beta" and info["chebyshev_degree"] >= 2
+    hc = HamiltonianCSR.from_sparse_pauli_op(spo, device="cuda:0")
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal((4, 1 << 10)) + 1j * rng.standard_normal((4, 1 << 10))
+    X /= np.linalg.norm(X, axis=1, keepdims=True)
+    out, info = chebyshev_evolve_batch(torch.from_numpy(X.astype(np.complex64)).cuda(), hc, 1.5, 1e-8)
+    out = out.cpu().numpy()
+    for r in range(4):
+        ref = OH.expm_reference(h, X[r].astype(np.complex64).astype(np.complex128), 1.5)
+        assert np.max(np.abs(out[r] - ref)) < 1e-5
+    # H = Z: exact phases (the reference's analytic case)
+    from blackwell_backend.hamiltonian import SparsePauliOp
+    sv, _ = sim.run_hamiltonian(SparsePauliOp.from_list([("Z", 1.0)]), 1.3,
+                                initial_state=np.array([0.6, 0.8j], dtype=np.complex64))
+    assert np.allclose(sv, [0.6 * np.exp(-1.3j), 0.8j * np.exp(1.3j)], atol=1e-6)
+    res, info = sim.run_hamiltonian(spo, 0.7, shots=4096)
+    assert sum(res.get_counts().values()) == 4096
+
+
+def test_run_and_time_stats(sim):
+    from blackwell_backend import workloads as W
+    res, stats = sim.run_and_time(W.c1_rotation_ladder(10, 2), shots=64, warmup=1, repeats=3)
+    assert len(stats["times_s"]) == 3 and stats["median_s"] > 0
+    assert sum(res.get_counts().values()) == 64

@@ -1,0 +1,96 @@
+"""Serving scheduler on the CPU with a fake device executor: admission by
+statevector bytes against an injected budget, rejection of requests that
+can never fit, lane routing, overlap of the two lanes, continuous admission
+of late arrivals, and the timeout."""
+import time
+
+import numpy as np
+import pytest
+
+
+class FakeJob:
+    """Completes `dur` seconds after launch; samples = rid repeated."""
+    log = []
+
+    def __init__(self, reqs, dur):
+        self.reqs = reqs if isinstance(reqs, list) else [reqs]
+        self.dur = dur
+        self.t = None
+
+    def launch(self):
+        self.t = time.perf_counter()
+        FakeJob.log.append(("launch", [r.rid for r in self.reqs], self.t))
+
+    def done(self):
+        return time.perf_counter() - self.t >= self.dur
+
+    def finish(self, now):
+        for r in self.reqs:
+            r.samples = np.full(r.shots, r.rid)
+            r.t_done = now
+
+
+def make_server(budget, dur_lds=0.002, dur_hbm=0.01, **kw):
+    from blackwell_backend.serving import Server
+    FakeJob.log = []
+    return Server(budget_bytes=budget, lds_factory=lambda rs: FakeJob(rs, dur_lds),
+                  hbm_factory=lambda r: FakeJob(r, dur_hbm), **kw)
+
+
+def reqs(ns, gap=0.0, shots=16):
+    from blackwell_backend.serving import Request
+    return [Request(i, i * gap, n, shots, 1000 + i, program=("su4", n, n, i)) for i, n in enumerate(ns)]
+
+
+def test_lane_routing_and_all_served():
+    s = make_server(1 << 40)
+    rs = reqs([10, 14, 15, 24, 12, 20])
+    st = s.serve(rs)
+    assert st["served"] == 6 and st["rejected"] == 0
+    assert [r.lane for r in rs] == ["lds", "lds", "hbm", "hbm", "lds", "hbm"]
+    assert all(np.all(r.samples == r.rid) for r in rs)
+    assert st["p99_latency_s"] >= st["p50_latency_s"] > 0
+
+
+def test_budget_rejects_what_never_fits_and_bounds_residency():
+    # budget fits one 20-qubit state (8 MiB + shots + 1 MiB slack) but not a 24-qubit one
+    budget = (8 << 20) + (2 << 20)
+    s = make_server(budget, dur_hbm=0.005)
+    rs = reqs([20, 24, 20, 20, 11])
+    st = s.serve(rs)
+    assert st["rejected"] == 1 and rs[1].t_done is None
+    assert st["served"] == 4
+    assert s.peak_in_use <= budget
+    # the three 20-qubit jobs could not overlap under this budget
+    hbm = sorted((r.t_admit, r.t_done) for r in rs if r.lane == "hbm")
+    for (a0, d0), (a1, d1) in zip(hbm, hbm[1:]):
+        assert a1 >= d0 - 1e-3
+
+
+def test_late_arrivals_join_while_others_run():
+    s = make_server(1 << 40, dur_lds=0.02)
+    rs = reqs([10] * 6, gap=0.005)
+    s.serve(rs)
+    launches = [e for e in FakeJob.log if e[0] == "launch"]
+    assert len(launches) >= 2                       # not one batch after the last arrival
+    assert launches[0][1][0] == 0
+    assert all(r.t_admit >= r.arrival - 1e-4 for r in rs)
+
+
+def test_timeout():
+    s = make_server(1 << 40, dur_lds=10.0)
+    with pytest.raises(TimeoutError):
+        s.serve(reqs([10]), timeout_s=0.05)
+
+
+def test_splitmix_matches_oracle():
+    from blackwell_backend.serving import splitmix_uniforms
+    from oracle import statevector as O
+    assert np.array_equal(splitmix_uniforms(12345, 1000), O.splitmix_uniforms(12345, 1000))
+
+
+def test_su4_program_matches_circuit_program():
+    from blackwell_backend import serving as S, workloads as W
+    ops, cf = S.su4_program(9, 9, 4)
+    ops2, cf2 = S.circuit_program(W.su4_volume(9, 9, 4))
+    assert np.array_equal(ops, ops2) and np.allclose(cf, cf2)
