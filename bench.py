@@ -140,7 +140,9 @@ def per_gate_sweep(n=30, reps=4):
         ("2q_dense_28_29", lambda: bk.apply_gate_2q(st, g4, n - 2, n - 1), N, 16 * N),
         ("diag_t5", lambda: bk.apply_diagonal(st, d2, 5), N, 16 * N),
         ("phase_t9", lambda: bk.apply_diagonal_phase(st, 1j, 9), N // 2, 8 * N),
+        ("cnot_17_3", lambda: bk.apply_mcx(st, [17], 3), N // 2, 8 * N),
         ("cnot_3_17", lambda: bk.apply_mcx(st, [3], 17), N // 2, 8 * N),
+        ("mcx3_11_19_25_to_2", lambda: bk.apply_mcx(st, [11, 19, 25], 2), N // 8, 2 * N),
         ("mcx3_2_11_25_to_19", lambda: bk.apply_mcx(st, [2, 11, 25], 19), N // 8, 2 * N),
         ("swap_4_23", lambda: bk.apply_mcswap(st, [], 4, 23), N // 2, 8 * N),
         ("norm2", lambda: bk.norm2(st), N, 8 * N),
@@ -198,6 +200,42 @@ def cpu_baseline(n, depth, seed, budget_s=15.0):
             "kind": "port",
             "sample": f"first {done} of {n * depth // 2} gates of the {n}-qubit depth-{depth} "
                       f"SU(4) circuit, FP64 C/OpenMP statevector (oracle/sv_fp64.c), {el:.1f} s"}
+
+
+def serving_bench(n_requests=512, seed=99, shots=1024):
+    """Mixed 10..24-qubit random-SU(4) request stream (depth n, 1024 shots):
+    the continuous-batching server vs the plain one-circuit-at-a-time path.
+    Capacity with all requests queued at t = 0, then Poisson arrivals at
+    1.2x the plain capacity (latency p50/p99)."""
+    from blackwell_backend import BlackwellSimulator, serving as S, workloads as W
+    sim = BlackwellSimulator("cuda:0")
+    ns = W.serving_qubit_counts(n_requests, seed)
+
+    def reqs(rate):
+        rng = np.random.default_rng(seed + 1)
+        arr = np.cumsum(rng.exponential(1.0 / rate, n_requests)) if rate else np.zeros(n_requests)
+        return [S.su4_request(i, float(arr[i]), int(ns[i]), shots, seed * 100003 + i)
+                for i in range(n_requests)]
+
+    # warm both paths (compiles kernels, allocator)
+    S.serve_plain(reqs(None)[:8], sim, realtime=False)
+    S.Server(sim).serve(reqs(None)[:8], realtime=False)
+    plain_cap = S.serve_plain(reqs(None), sim, realtime=False)
+    served_cap = S.Server(sim).serve(reqs(None), realtime=False)
+    rate = 1.2 * plain_cap["circuits_per_s"]
+    plain_rt = S.serve_plain(reqs(rate), sim, realtime=True)
+    served_rt = S.Server(sim).serve(reqs(rate), realtime=True)
+    return {
+        "workload": f"{n_requests} requests, n = 10 + k, P(k) ~ 0.25*0.75^k (k <= 14), random-SU(4) "
+                    f"depth n, {shots} shots each",
+        "qubit_histogram": {int(k): int(v) for k, v in zip(*np.unique(ns, return_counts=True))},
+        "capacity": {"served_circuits_per_s": served_cap["circuits_per_s"],
+                     "plain_circuits_per_s": plain_cap["circuits_per_s"],
+                     "speedup": served_cap["circuits_per_s"] / plain_cap["circuits_per_s"]},
+        "poisson_rate_per_s": rate,
+        "poisson_served": {k: served_rt[k] for k in ("circuits_per_s", "p50_latency_s", "p99_latency_s")},
+        "poisson_plain": {k: plain_rt[k] for k in ("circuits_per_s", "p50_latency_s", "p99_latency_s")},
+    }
 
 
 def main():
@@ -275,6 +313,11 @@ def main():
             line["per_gate_30q"] = per_gate_sweep(30)
         except Exception as e:
             line["per_gate_30q"] = {"error": repr(e)}
+        try:
+            log("[bench] serving stream")
+            line["serving"] = serving_bench()
+        except Exception as e:
+            line["serving"] = {"error": repr(e)}
         try:
             log("[bench] CPU baseline")
             line["cpu_baseline"] = cpu_baseline(n, depth, seed)

@@ -286,6 +286,13 @@ class Block:
         self.mat = mat
 
 
+def _mask(qs):
+    m = 0
+    for q in qs:
+        m |= 1 << q
+    return m
+
+
 def plan_blocks(ops, n, k=5, window=None):
     """Greedy dependency-aware packing of unitary ops into <=k-qubit blocks.
 
@@ -293,52 +300,65 @@ def plan_blocks(ops, n, k=5, window=None):
     of its qubits is touched by an earlier op that was left out (so every
     op still sees its predecessors first) and the union of qubits stays
     <= k.  measure/reset stay as their own entries (returned as Op).
-    Returns a list of Block | Op in execution order."""
-    remaining = list(ops)
+    Qubit sets are integer bitmasks.  Returns a list of Block | Op in
+    execution order."""
+    remaining = [(op, _mask(op.all_qubits)) for op in ops]
     out = []
     while remaining:
-        if remaining[0].kind in ("measure", "reset"):
-            out.append(remaining.pop(0))
+        if remaining[0][0].kind in ("measure", "reset"):
+            out.append(remaining.pop(0)[0])
             continue
-        bq = set()
-        chosen = []
-        blocked = set()
-        keep = []
-        scanned = 0
-        for idx, op in enumerate(remaining):
-            if window is not None and scanned >= window:
+        bq = 0
+        nb = 0
+        chosen, keep = [], []
+        blocked = 0
+        for idx, (op, m) in enumerate(remaining):
+            if window is not None and idx >= window:
                 keep.extend(remaining[idx:])
                 break
-            scanned += 1
-            qs = set(op.all_qubits)
-            if op.kind in ("measure", "reset") or (qs & blocked):
-                blocked |= qs
-                keep.append(op)
+            if op.kind in ("measure", "reset") or (m & blocked):
+                blocked |= m
+                keep.append((op, m))
                 continue
-            if len(bq | qs) <= k:
-                bq |= qs
+            u = bq | m
+            if bin(u).count("1") <= k:
+                bq = u
                 chosen.append(op)
             else:
-                blocked |= qs
-                keep.append(op)
+                blocked |= m
+                keep.append((op, m))
         remaining = keep
         if not chosen:
             # the first op is wider than k qubits: it runs on its own kernel
-            out.append(remaining.pop(0))
+            out.append(remaining.pop(0)[0])
             continue
-        bqs = sorted(bq)
+        bqs = [q for q in range(n) if bq >> q & 1]
         # pad to k qubits with the lowest unused qubits so the MFMA shape fits
         q = 0
         while len(bqs) < min(k, n) and q < n:
-            if q not in bq:
+            if not (bq >> q & 1):
                 bqs.append(q)
             q += 1
         bqs.sort()
         mat = np.eye(1 << len(bqs), dtype=np.complex128)
         for op in chosen:
-            mat = embed(op.full_matrix(), list(op.all_qubits), bqs) @ mat
+            mat = apply_to_block(mat, op.full_matrix(), list(op.all_qubits), bqs)
         out.append(Block(bqs, chosen, mat))
     return out
+
+
+def apply_to_block(mat, u, qubits, block_qubits):
+    """u (row bit j <-> qubits[j]) applied on the left of the block matrix
+    mat (row bit b <-> block_qubits[b]), by contracting the matching axes."""
+    k = len(block_qubits)
+    kq = len(qubits)
+    d = 1 << k
+    t = mat.reshape([2] * k + [d])            # axis a <-> block bit k-1-a
+    axes = [k - 1 - block_qubits.index(q) for q in qubits]
+    ut = np.asarray(u).reshape([2] * (2 * kq))
+    res = np.tensordot(ut, t, axes=([kq + (kq - 1 - j) for j in range(kq)], axes))
+    res = np.moveaxis(res, [kq - 1 - j for j in range(kq)], axes)
+    return res.reshape(d, d)
 
 
 class TilePass:

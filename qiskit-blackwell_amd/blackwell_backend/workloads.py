@@ -30,7 +30,9 @@ def haar_unitary(rng, dim):
 
 
 def haar_batch(rng, count, dim=4):
-    z = (rng.standard_normal((count, dim, dim)) + 1j * rng.standard_normal((count, dim, dim))) / math.sqrt(2)
+    re = rng.standard_normal((count, dim, dim))
+    im = rng.standard_normal((count, dim, dim))
+    z = (re + 1j * im) / math.sqrt(2)
     q, r = np.linalg.qr(z)
     d = np.diagonal(r, axis1=-2, axis2=-1)
     return q * (d / np.abs(d))[:, np.newaxis, :]
@@ -53,14 +55,26 @@ def c1_rotation_ladder(n=12, layers=12, seed=1234):
 
 
 def su4_layers(n, depth, seed):
-    """[(pairs [(q0, q1)], unitaries [P, 4, 4] complex128)] per layer."""
+    """[(pairs [(q0, q1)], unitaries [P, 4, 4] complex128)] per layer.
+
+    Draw order per layer: the permutation, then the layer's complex Gaussian
+    matrices (real parts, then imaginary parts); all QR factorisations run
+    in one batched call afterwards (same matrices as one call per layer)."""
     rng = np.random.default_rng(seed)
-    out = []
+    P = n // 2
+    perms, zs = [], []
     for _ in range(depth):
-        perm = rng.permutation(n)
-        pairs = [(int(perm[2 * j]), int(perm[2 * j + 1])) for j in range(n // 2)]
-        out.append((pairs, haar_batch(rng, len(pairs))))
-    return out
+        perms.append(rng.permutation(n))
+        re = rng.standard_normal((P, 4, 4))
+        im = rng.standard_normal((P, 4, 4))
+        zs.append((re + 1j * im) / math.sqrt(2))
+    if depth == 0 or P == 0:
+        return [([], np.zeros((0, 4, 4), dtype=np.complex128)) for _ in range(depth)]
+    q, r = np.linalg.qr(np.concatenate(zs))
+    d = np.diagonal(r, axis1=-2, axis2=-1)
+    us = (q * (d / np.abs(d))[:, np.newaxis, :]).reshape(depth, P, 4, 4)
+    return [([(int(pm[2 * j]), int(pm[2 * j + 1])) for j in range(P)], us[k])
+            for k, pm in enumerate(perms)]
 
 
 def su4_volume(n=28, depth=28, seed=2024):

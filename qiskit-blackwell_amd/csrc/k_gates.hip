@@ -2,20 +2,20 @@
 // multi-controlled, permutation and batched gates, plus the library's error
 // plumbing.
 //
-// Streaming layout.  Every kernel moves amplitudes as float4 (two adjacent
-// complex64, 16 B per lane), so one wave64 load instruction covers 1 KiB of
-// contiguous HBM.  A "work item" is a fixed set of float4 slots; each thread
-// owns U items spaced by the block size, all loads are issued before any
-// arithmetic so that 2*U (1Q) or 4*U (2Q) 16-byte loads are in flight per
-// lane.  Index math uses 32-bit integers for states up to 2^32 amplitudes and
-// 64-bit above (template parameter I).
-//
-// Pair addressing for a 2x2 gate on target t (amplitude pairs differing in
-// bit t): in float4 units the two slots of item w are
-//     f0 = ins0(w, s),  f1 = f0 + 2^s,  s = max(t - 1, 0).
-// For t >= 1 the pairs are (f0.lo, f1.lo) and (f0.hi, f1.hi); for t = 0 they
-// are the two halves of each slot.  The index math follows the reference's
-// insert-zero rule (bwk/csrc/apply_gate_sm120.cu:100-104) in float4 units.
+// Streaming layout.  Amplitudes move as float4 slots (two adjacent complex64,
+// 16 B per lane) and every load/store instruction of a wave covers 64
+// consecutive slots (1 KiB), with nontemporal hints: the state is touched once
+// per gate and must not evict the L2/MALL working set.  How a gate finds the
+// partner of an amplitude depends on the target bit t:
+//   t = 0      the pair sits inside one float4;
+//   t = 1..6   the pair sits in lane ^ 2^(t-1) of the same wave (cross-lane
+//              shuffle), so the wave still reads one contiguous KiB;
+//   t >= 7     two lane-contiguous streams 2^(t-1) slots apart:
+//              f0 = ins0(w, t-1), f1 = f0 + 2^(t-1).
+// The insert-zero rule is the reference's pair enumeration
+// (bwk/csrc/apply_gate_sm120.cu:100-104) expressed in float4 slots.  Grids
+// of the stream modes are renumbered XCD-major (xcd_block).  Index math is
+// 32-bit for states up to 2^32 amplitudes and 64-bit above (template I).
 #include "qsim_internal.hpp"
 #include <stdio.h>
 #include <string.h>
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kThreads) void k_init(float4* __restrict__ st, I n4
   const I f = (I)blockIdx.x * kThreads + threadIdx.x;
   if (f >= n4) return;
   const bool first = (f & (per_state4 - 1)) == 0;
-  st[f] = make_float4(first ? 1.f : 0.f, 0.f, 0.f, 0.f);
+  st_nt(st + f, make_float4(first ? 1.f : 0.f, 0.f, 0.f, 0.f));
 }
 
 static int init_impl(void* stream, void* states, uint64_t n_states, int n) {
@@ -122,42 +122,40 @@ __device__ __forceinline__ void load_m2(const float2* gdev, const M2& gv, float2
   }
 }
 
-// Batched states need no extra index math: with s <= n-2 the insert-zero of
-// the flat item index w = state*2^(n-2) + local shifts the state number up by
-// one bit, which is exactly the state stride 2^(n-1) in float4 units.
-template <typename I, int U>
-__global__ __launch_bounds__(kThreads) void k_gate1q(float4* __restrict__ st, I n_items, int t,
+// MODE 0: t = 0, MODE 1: t = 1..6 (cross-lane), MODE 2: t >= 7 (streams).
+// n_work = float4 slots (modes 0, 1) or slot pairs (mode 2).  Batched states
+// need no extra index math: the flat slot index has the state number above
+// bit n-2 and t < n keeps every pair inside one state.
+template <typename I, int MODE>
+__global__ __launch_bounds__(kThreads) void k_gate1q(float4* __restrict__ st, I n_work, int t,
                                                      const float2* __restrict__ gdev, M2 gv) {
   float2 g[4];
   load_m2(gdev, gv, g);
-  const int s = t > 0 ? t - 1 : 0;
-  const I step = I(1) << s;
-  const I base = (I)blockIdx.x * (kThreads * U) + threadIdx.x;
-  I f0[U];
-  float4 v0[U], v1[U];
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    I w = base + (I)u * kThreads;
-    if (w >= n_items) w = n_items - 1;  // duplicate the last item; store is masked below
-    f0[u] = ins0(w, s);
-    v0[u] = st[f0[u]];
-    v1[u] = st[f0[u] + step];
-  }
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const I w = base + (I)u * kThreads;
-    float4 r0, r1;
-    if (t == 0) {
-      r0 = cat4(crow2(g[0], lo2(v0[u]), g[1], hi2(v0[u])), crow2(g[2], lo2(v0[u]), g[3], hi2(v0[u])));
-      r1 = cat4(crow2(g[0], lo2(v1[u]), g[1], hi2(v1[u])), crow2(g[2], lo2(v1[u]), g[3], hi2(v1[u])));
-    } else {
-      r0 = cat4(crow2(g[0], lo2(v0[u]), g[1], lo2(v1[u])), crow2(g[0], hi2(v0[u]), g[1], hi2(v1[u])));
-      r1 = cat4(crow2(g[2], lo2(v0[u]), g[3], lo2(v1[u])), crow2(g[2], hi2(v0[u]), g[3], hi2(v1[u])));
-    }
-    if (w < n_items) {
-      st[f0[u]] = r0;
-      st[f0[u] + step] = r1;
-    }
+  const I w0 = (MODE == 2 ? xcd_block<I>() : (I)blockIdx.x) * kThreads + threadIdx.x;
+  const bool live = w0 < n_work;
+  const I w = live ? w0 : n_work - 1;
+  if (MODE == 0) {
+    const float4 v = ld_nt(st + w);
+    const float4 r = cat4(crow2(g[0], lo2(v), g[1], hi2(v)), crow2(g[2], lo2(v), g[3], hi2(v)));
+    if (live) st_nt(st + w, r);
+  } else if (MODE == 1) {
+    // lane l holds slot w with w & 63 == l; bit (t-1) of the slot index is
+    // bit t of the amplitude index.  Partner lanes of live lanes are live
+    // (n_work is a power of two larger than 2^(t-1)).
+    const float4 v = ld_nt(st + w);
+    const int m = 1 << (t - 1);
+    const float4 p = shfl_xor4(v, m);
+    const bool low = (threadIdx.x & m) == 0;
+    const float2 ga = low ? g[0] : g[3], gb = low ? g[1] : g[2];
+    const float4 r = cat4(crow2(ga, lo2(v), gb, lo2(p)), crow2(ga, hi2(v), gb, hi2(p)));
+    if (live) st_nt(st + w, r);
+  } else {
+    const int s = t - 1;
+    const I f0 = ins0(w, s), f1 = f0 + (I(1) << s);
+    const float4 v0 = ld_nt(st + f0), v1 = ld_nt(st + f1);
+    const float4 r0 = cat4(crow2(g[0], lo2(v0), g[1], lo2(v1)), crow2(g[0], hi2(v0), g[1], hi2(v1)));
+    const float4 r1 = cat4(crow2(g[2], lo2(v0), g[3], lo2(v1)), crow2(g[2], hi2(v0), g[3], hi2(v1)));
+    if (live) { st_nt(st + f0, r0); st_nt(st + f1, r1); }
   }
 }
 
@@ -177,8 +175,6 @@ __global__ void k_gate1q_tiny(float2* __restrict__ st, uint64_t n_states, int n,
   s[i1] = crow2(g[2], a0, g[3], a1);
 }
 
-constexpr int kU1 = 2;
-
 int gate1q_impl(void* stream, void* states, uint64_t n_states, int n, const float2* gdev,
                 const M2& gv, int t) {
   if (int rc = check_n(n)) return rc;
@@ -192,25 +188,29 @@ int gate1q_impl(void* stream, void* states, uint64_t n_states, int n, const floa
                        n_states, n, t, gdev, gv);
     return launch_status("apply_gate");
   }
-  const uint64_t items = (1ull << n) / 4 * n_states;
-  const uint64_t blocks = ceil_div(items, (uint64_t)kThreads * kU1);
-  if (items * 2 <= 0xffffffffull)   // largest float4 index fits 32 bits
-    hipLaunchKernelGGL((k_gate1q<uint32_t, kU1>), dim3(blocks), dim3(kThreads), 0, s,
-                       (float4*)states, (uint32_t)items, t, gdev, gv);
-  else
-    hipLaunchKernelGGL((k_gate1q<uint64_t, kU1>), dim3(blocks), dim3(kThreads), 0, s,
-                       (float4*)states, items, t, gdev, gv);
+  const uint64_t slots = (1ull << n) / 2 * n_states;
+  const int mode = t == 0 ? 0 : (t < 7 ? 1 : 2);
+  const uint64_t work = mode == 2 ? slots / 2 : slots;
+  const dim3 grid(ceil_div(work, kThreads)), block(kThreads);
+  if (slots <= 0xffffffffull) {
+    const uint32_t w = (uint32_t)work;
+    if (mode == 0) hipLaunchKernelGGL((k_gate1q<uint32_t, 0>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
+    if (mode == 1) hipLaunchKernelGGL((k_gate1q<uint32_t, 1>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
+    if (mode == 2) hipLaunchKernelGGL((k_gate1q<uint32_t, 2>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
+  } else {
+    if (mode == 0) hipLaunchKernelGGL((k_gate1q<uint64_t, 0>), grid, block, 0, s, (float4*)states, work, t, gdev, gv);
+    if (mode == 1) hipLaunchKernelGGL((k_gate1q<uint64_t, 1>), grid, block, 0, s, (float4*)states, work, t, gdev, gv);
+    if (mode == 2) hipLaunchKernelGGL((k_gate1q<uint64_t, 2>), grid, block, 0, s, (float4*)states, work, t, gdev, gv);
+  }
   return launch_status("apply_gate");
 }
 
 // ============================================================ 2Q dense
 // Gate row/column index r = 2*bit(hi) + bit(lo), lo < hi
-// (bwk/csrc/apply_gate2q_sm120.cu:11-20, 245-246).  Item w covers two groups
-// through four float4 slots F[0..3]:
-//   lo >= 1: F = f, f+2^(lo-1), f+2^(hi-1), f+both, f = ins0(ins0(w,lo-1),hi-1);
-//            group A = the .lo halves, group B = the .hi halves.
-//   lo == 0: F[0]=ins0(2w,hi-1), F[1]=F[0]+2^(hi-1), F[2]=ins0(2w+1,hi-1),
-//            F[3]=F[2]+2^(hi-1); group A = (F0.lo,F0.hi,F1.lo,F1.hi), B likewise.
+// (bwk/csrc/apply_gate2q_sm120.cu:11-20, 245-246).  S = number of targets
+// >= 7, each of which becomes a lane-contiguous stream; a target in 1..6 is
+// reached by a cross-lane shuffle and target 0 inside the float4.  Each lane
+// owns 2^S slots and computes the rows of the amplitudes it holds.
 __device__ __forceinline__ void load_m4(const float2* gdev, const M4& gv, float2 g[16]) {
   if (gdev) {
 #pragma unroll
@@ -230,53 +230,97 @@ __device__ __forceinline__ void mat4(const float2 g[16], float2& a0, float2& a1,
   a0 = b0; a1 = b1; a2 = b2; a3 = b3;
 }
 
-template <typename I>
-__global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I n_items, int lo,
+// Row r of g selected without dynamic register indexing.
+__device__ __forceinline__ void pick_row(const float2 g[16], bool b1, bool b0, float2 out[4]) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const float2 r0 = b0 ? g[4 + c] : g[c];
+    const float2 r1 = b0 ? g[12 + c] : g[8 + c];
+    out[c] = b1 ? r1 : r0;
+  }
+}
+
+template <typename I, int S>
+__global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I n_work, int lo,
                                                      int hi, const float2* __restrict__ gdev,
                                                      M4 gv) {
   float2 g[16];
   load_m4(gdev, gv, g);
-  I w = (I)blockIdx.x * kThreads + threadIdx.x;
-  if (w >= n_items) return;
-  I F[4];
-  const I mh = I(1) << (hi - 1);
-  if (lo > 0) {
-    const I ml = I(1) << (lo - 1);
-    F[0] = ins0(ins0(w, lo - 1), hi - 1);
-    F[1] = F[0] + ml; F[2] = F[0] + mh; F[3] = F[0] + ml + mh;
-  } else {
-    F[0] = ins0(2 * w, hi - 1);     F[1] = F[0] + mh;
-    F[2] = ins0(2 * w + 1, hi - 1); F[3] = F[2] + mh;
-  }
-  float4 v[4];
+  const I w0 = (S > 0 ? xcd_block<I>() : (I)blockIdx.x) * kThreads + threadIdx.x;
+  const bool live = w0 < n_work;
+  const I w = live ? w0 : n_work - 1;
+  const int lane = threadIdx.x & 63;
+  if (S == 2) {
+    const I ml = I(1) << (lo - 1), mh = I(1) << (hi - 1);
+    const I f0 = ins0(ins0(w, lo - 1), hi - 1), f1 = f0 + ml, f2 = f0 + mh, f3 = f1 + mh;
+    const float4 v0 = ld_nt(st + f0), v1 = ld_nt(st + f1), v2 = ld_nt(st + f2), v3 = ld_nt(st + f3);
+    float2 a0 = lo2(v0), a1 = lo2(v1), a2 = lo2(v2), a3 = lo2(v3);
+    float2 b0 = hi2(v0), b1 = hi2(v1), b2 = hi2(v2), b3 = hi2(v3);
+    mat4(g, a0, a1, a2, a3);
+    mat4(g, b0, b1, b2, b3);
+    if (live) {
+      st_nt(st + f0, cat4(a0, b0)); st_nt(st + f1, cat4(a1, b1));
+      st_nt(st + f2, cat4(a2, b2)); st_nt(st + f3, cat4(a3, b3));
+    }
+  } else if (S == 1) {
+    // hi is a stream: slot f0 has bit(hi) = 0, slot f1 bit(hi) = 1
+    const I f0 = ins0(w, hi - 1), f1 = f0 + (I(1) << (hi - 1));
+    const float4 v0 = ld_nt(st + f0), v1 = ld_nt(st + f1);
+    float4 o0, o1;
+    if (lo == 0) {
+      float2 a0 = lo2(v0), a1 = hi2(v0), a2 = lo2(v1), a3 = hi2(v1);
+      mat4(g, a0, a1, a2, a3);
+      o0 = cat4(a0, a1);
+      o1 = cat4(a2, a3);
+    } else {
+      const int m = 1 << (lo - 1);
+      const bool bl = (lane & m) != 0;
+      const float4 p0 = shfl_xor4(v0, m), p1 = shfl_xor4(v1, m);
+      float2 ga[4], gb[4];
+      pick_row(g, false, bl, ga);   // rows of the amplitudes held here: r = bl, 2 + bl
+      pick_row(g, true, bl, gb);
+      float2 y[2][2];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) v[k] = st[F[k]];
-  float2 a[8];
-  if (lo > 0) {
-    a[0] = lo2(v[0]); a[1] = lo2(v[1]); a[2] = lo2(v[2]); a[3] = lo2(v[3]);
-    a[4] = hi2(v[0]); a[5] = hi2(v[1]); a[6] = hi2(v[2]); a[7] = hi2(v[3]);
+      for (int h = 0; h < 2; ++h) {
+        const float2 s0 = h ? hi2(v0) : lo2(v0), s1 = h ? hi2(v1) : lo2(v1);
+        const float2 q0 = h ? hi2(p0) : lo2(p0), q1 = h ? hi2(p1) : lo2(p1);
+        const float2 a0 = bl ? q0 : s0, a1 = bl ? s0 : q0, a2 = bl ? q1 : s1, a3 = bl ? s1 : q1;
+        y[h][0] = crow4(ga, a0, a1, a2, a3);
+        y[h][1] = crow4(gb, a0, a1, a2, a3);
+      }
+      o0 = cat4(y[0][0], y[1][0]);
+      o1 = cat4(y[0][1], y[1][1]);
+    }
+    if (live) { st_nt(st + f0, o0); st_nt(st + f1, o1); }
   } else {
-    a[0] = lo2(v[0]); a[1] = hi2(v[0]); a[2] = lo2(v[1]); a[3] = hi2(v[1]);
-    a[4] = lo2(v[2]); a[5] = hi2(v[2]); a[6] = lo2(v[3]); a[7] = hi2(v[3]);
+    const float4 v = ld_nt(st + w);
+    const int mh = 1 << (hi - 1);
+    const bool bh = (lane & mh) != 0;
+    float4 o;
+    if (lo == 0) {
+      const float4 p = shfl_xor4(v, mh);
+      const float2 a0 = bh ? lo2(p) : lo2(v), a1 = bh ? hi2(p) : hi2(v);
+      const float2 a2 = bh ? lo2(v) : lo2(p), a3 = bh ? hi2(v) : hi2(p);
+      float2 ga[4], gb[4];
+      pick_row(g, bh, false, ga);
+      pick_row(g, bh, true, gb);
+      o = cat4(crow4(ga, a0, a1, a2, a3), crow4(gb, a0, a1, a2, a3));
+    } else {
+      const int ml = 1 << (lo - 1);
+      const bool bl = (lane & ml) != 0;
+      const float4 pl = shfl_xor4(v, ml), ph = shfl_xor4(v, mh), pb = shfl_xor4(v, ml | mh);
+      // the slot with (bit lo, bit hi) = (x, y) lives in the lane with those bits
+#define QSIM_SRC(x, y) ((x) == bl ? ((y) == bh ? v : ph) : ((y) == bh ? pl : pb))
+      const float4 A0 = QSIM_SRC(false, false), A1 = QSIM_SRC(true, false);
+      const float4 A2 = QSIM_SRC(false, true), A3 = QSIM_SRC(true, true);
+#undef QSIM_SRC
+      float2 gr[4];
+      pick_row(g, bh, bl, gr);
+      o = cat4(crow4(gr, lo2(A0), lo2(A1), lo2(A2), lo2(A3)),
+               crow4(gr, hi2(A0), hi2(A1), hi2(A2), hi2(A3)));
+    }
+    if (live) st_nt(st + w, o);
   }
-  mat4(g, a[0], a[1], a[2], a[3]);
-  mat4(g, a[4], a[5], a[6], a[7]);
-  if (lo > 0) {
-    v[0] = cat4(a[0], a[4]); v[1] = cat4(a[1], a[5]); v[2] = cat4(a[2], a[6]); v[3] = cat4(a[3], a[7]);
-  } else {
-    v[0] = cat4(a[0], a[1]); v[1] = cat4(a[2], a[3]); v[2] = cat4(a[4], a[5]); v[3] = cat4(a[6], a[7]);
-  }
-#pragma unroll
-  for (int k = 0; k < 4; ++k) st[F[k]] = v[k];
-}
-
-__global__ void k_gate2q_tiny(float2* __restrict__ st, int lo, int hi,
-                              const float2* __restrict__ gdev, M4 gv) {
-  float2 g[16];
-  load_m4(gdev, gv, g);
-  float2 a0 = st[0], a1 = st[1u << lo], a2 = st[1u << hi], a3 = st[(1u << lo) | (1u << hi)];
-  mat4(g, a0, a1, a2, a3);
-  st[0] = a0; st[1u << lo] = a1; st[1u << hi] = a2; st[(1u << lo) | (1u << hi)] = a3;
 }
 
 int gate2q_impl(void* stream, void* state, int n, const float2* gdev, const M4& gv, int q0,
@@ -288,18 +332,19 @@ int gate2q_impl(void* stream, void* state, int n, const float2* gdev, const M4& 
   if (!state) return fail(QSIM_EINVAL, "state is NULL");
   const int lo = std::min(q0, q1), hi = std::max(q0, q1);
   hipStream_t s = as_stream(stream);
-  if (n == 2) {
-    hipLaunchKernelGGL(k_gate2q_tiny, dim3(1), dim3(1), 0, s, (float2*)state, lo, hi, gdev, gv);
-    return launch_status("apply_gate_2q");
+  const int S = (lo >= 7) + (hi >= 7);
+  const uint64_t work = ((1ull << n) / 2) >> S;
+  const dim3 grid(ceil_div(work, kThreads)), block(kThreads);
+  if (n <= 32) {
+    const uint32_t w = (uint32_t)work;
+    if (S == 0) hipLaunchKernelGGL((k_gate2q<uint32_t, 0>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
+    if (S == 1) hipLaunchKernelGGL((k_gate2q<uint32_t, 1>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
+    if (S == 2) hipLaunchKernelGGL((k_gate2q<uint32_t, 2>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
+  } else {
+    if (S == 0) hipLaunchKernelGGL((k_gate2q<uint64_t, 0>), grid, block, 0, s, (float4*)state, work, lo, hi, gdev, gv);
+    if (S == 1) hipLaunchKernelGGL((k_gate2q<uint64_t, 1>), grid, block, 0, s, (float4*)state, work, lo, hi, gdev, gv);
+    if (S == 2) hipLaunchKernelGGL((k_gate2q<uint64_t, 2>), grid, block, 0, s, (float4*)state, work, lo, hi, gdev, gv);
   }
-  const uint64_t items = (1ull << n) / 8;
-  const uint64_t blocks = ceil_div(items, kThreads);
-  if (n <= 32)
-    hipLaunchKernelGGL(k_gate2q<uint32_t>, dim3(blocks), dim3(kThreads), 0, s, (float4*)state,
-                       (uint32_t)items, lo, hi, gdev, gv);
-  else
-    hipLaunchKernelGGL(k_gate2q<uint64_t>, dim3(blocks), dim3(kThreads), 0, s, (float4*)state,
-                       items, lo, hi, gdev, gv);
   return launch_status("apply_gate_2q");
 }
 
@@ -321,7 +366,7 @@ __global__ __launch_bounds__(kThreads) void k_diag(float4* __restrict__ st, I n4
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     I f = base + (I)u * kThreads;
-    v[u] = st[f < n4 ? f : n4 - 1];
+    v[u] = ld_nt(st + (f < n4 ? f : n4 - 1));
   }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
@@ -333,7 +378,7 @@ __global__ __launch_bounds__(kThreads) void k_diag(float4* __restrict__ st, I n4
     const float2 e0 = h0 ? (b0 ? d3 : d2) : (b0 ? d1 : d0);
     const float2 e1 = h1 ? (b1 ? d3 : d2) : (b1 ? d1 : d0);
     const float4 r = cat4(cmulc(lo2(v[u]), e0), cmulc(hi2(v[u]), e1));
-    if (f < n4) st[f] = r;
+    if (f < n4) st_nt(st + f, r);
   }
 }
 
@@ -363,7 +408,7 @@ __global__ __launch_bounds__(kThreads) void k_phase(float4* __restrict__ st, I n
   }
 }
 
-constexpr int kUd = 4;
+constexpr int kUd = 1;
 
 static int diag_impl(void* stream, void* state, int n, int kind, int q0, int q1,
                      const float2* ddev, const M2& dv) {

@@ -24,6 +24,7 @@
 // far inside the fidelity target, so no split-precision correction is run.
 #include "qsim_internal.hpp"
 #include <string.h>
+#include <algorithm>
 
 namespace qsim {
 
@@ -51,13 +52,15 @@ __device__ __forceinline__ uint64_t row_off(int r, const BlockArgs& a) {
   return o;
 }
 
-template <int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void k_block5(float2* __restrict__ st, uint64_t n_waves,
-                                                       BlockArgs a, const float2* __restrict__ U) {
+// Direct variant: operands straight from HBM into the MFMA lane layout,
+// persistent waves with one set of 32 groups prefetched into registers while
+// the MFMA chains of the current set run (load, MFMA and store phases of one
+// wave overlap instead of alternating).  Row offsets are 32-bit (n <= 32).
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_block5(float2* __restrict__ st, uint64_t n_sets,
+                                                   BlockArgs a, const float2* __restrict__ U) {
   const int lane = threadIdx.x & 63;
   const int h = lane >> 5, c = lane & 31;
-  const uint64_t wave = (uint64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
-  // A fragments: U[c][2s+h], s = 0..15
   float ur[16], ui[16];
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
@@ -65,12 +68,127 @@ __global__ __launch_bounds__(64 * WAVES) void k_block5(float2* __restrict__ st, 
     ur[s] = u.x;
     ui[s] = u.y;
   }
-  if (wave >= n_waves) return;
-  const uint64_t base = group_base(wave * 32 + c, a);
+  uint32_t rof[16];
+#pragma unroll
+  for (int s = 0; s < 16; ++s) rof[s] = (uint32_t)row_off(2 * s + h, a);
+  const uint64_t stride = (uint64_t)gridDim.x * W;
+  uint64_t set = (uint64_t)blockIdx.x * W + (threadIdx.x >> 6);
+  if (set >= n_sets) return;
+  uint64_t base = group_base(set * 32 + c, a);
+  float2 x[16];
+#pragma unroll
+  for (int s = 0; s < 16; ++s) x[s] = st[base | rof[s]];
+  while (true) {
+    const uint64_t nxt = set + stride;
+    const bool more = nxt < n_sets;
+    uint64_t nbase = 0;
+    float2 y[16];
+    if (more) {
+      nbase = group_base(nxt * 32 + c, a);
+#pragma unroll
+      for (int s = 0; s < 16; ++s) y[s] = st[nbase | rof[s]];
+    }
+    f32x16 accr = {0.f}, acci = {0.f};
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      accr = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[s], x[s].x, accr, 0, 0, 0);
+      accr = __builtin_amdgcn_mfma_f32_32x32x2f32(-ui[s], x[s].y, accr, 0, 0, 0);
+      acci = __builtin_amdgcn_mfma_f32_32x32x2f32(ui[s], x[s].x, acci, 0, 0, 0);
+      acci = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[s], x[s].y, acci, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int i = 8 * (r >> 2) + 4 * h + (r & 3);
+      st[base | row_off(i, a)] = make_float2(accr[r], acci[r]);
+    }
+    if (!more) break;
+    set = nxt;
+    base = nbase;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) x[s] = y[s];
+  }
+}
+
+// LDS-staged variant (n >= 12).  A workgroup of 4 waves owns 128 groups:
+// the tile is the 4096 amplitudes spanned by the 5 block qubits and the 7
+// lowest non-block qubits.  Those 12 qubits always include qubits 0..6, so
+// the tile is read and written as runs of >= 128 contiguous amplitudes with
+// lane-contiguous 16-byte nontemporal accesses, whatever the block qubits.
+// On the way in each amplitude is transposed to X[row][group] in LDS (row
+// stride padded by one element so the scattered writes spread over banks);
+// wave w then feeds its 32 groups to the MFMA chains from LDS, writes the
+// results back in place (its own columns only), and after a barrier the tile
+// is stored with the same coalesced mapping.
+constexpr int kTileGroups = 128;
+constexpr int kRowStride = kTileGroups + 1;
+
+struct Block5Tile {
+  int n;
+  uint8_t tq[12];     // tile qubits, ascending (tq[0] = 0)
+  uint8_t tcode[12];  // tile bit k: < 8 -> row bit tcode[k]; >= 8 -> group bit tcode[k] - 8
+  int nh;             // non-block qubits above the tile
+  uint8_t hp[QSIM_MAX_QUBITS];
+};
+
+__device__ __forceinline__ void tile_rc(uint32_t j, const Block5Tile& a, uint32_t& r, uint32_t& c,
+                                        uint64_t& off) {
+  r = 0; c = 0; off = 0;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    const uint32_t bit = (j >> k) & 1u;
+    off |= (uint64_t)bit << a.tq[k];
+    const uint32_t code = a.tcode[k];
+    if (code < 8) r |= bit << code;
+    else c |= bit << (code - 8);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_block5_lds(float2* __restrict__ st, Block5Tile a,
+                                                    const float2* __restrict__ U) {
+  __shared__ float2 X[32 * kRowStride];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int h = lane >> 5, c = lane & 31;
+  // tile base: workgroup index deposited on the non-block qubits above the tile
+  uint64_t base = 0;
+  const uint64_t b = blockIdx.x;
+#pragma unroll
+  for (int k = 0; k < QSIM_MAX_QUBITS; ++k)
+    if (k < a.nh) base |= ((b >> k) & 1ull) << a.hp[k];
+  float4* st4 = reinterpret_cast<float4*>(st);
+  // bit 0 of the tile index is qubit 0: its code tells where amplitude 2s+1 goes
+  const uint32_t c0 = a.tcode[0];
+  // all eight 16-byte loads of a thread are issued before the first LDS write
+  float4 v[8];
+  uint32_t lds0[8], lds1[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    uint32_t r, cc;
+    uint64_t off;
+    tile_rc(2u * (threadIdx.x + 256 * k), a, r, cc, off);
+    v[k] = ld_nt(st4 + ((base | off) >> 1));
+    const uint32_t r1 = c0 < 8 ? (r | (1u << c0)) : r;
+    const uint32_t cc1 = c0 < 8 ? cc : (cc | (1u << (c0 - 8)));
+    lds0[k] = r * kRowStride + cc;
+    lds1[k] = r1 * kRowStride + cc1;
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    X[lds0[k]] = lo2(v[k]);
+    X[lds1[k]] = hi2(v[k]);
+  }
+  float ur[16], ui[16];
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const float2 u = U[c * 32 + 2 * s + h];
+    ur[s] = u.x;
+    ui[s] = u.y;
+  }
+  __syncthreads();
+  const int col = wv * 32 + c;
   float xr[16], xi[16];
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
-    const float2 x = st[base | row_off(2 * s + h, a)];
+    const float2 x = X[(2 * s + h) * kRowStride + col];
     xr[s] = x.x;
     xi[s] = x.y;
   }
@@ -85,7 +203,17 @@ __global__ __launch_bounds__(64 * WAVES) void k_block5(float2* __restrict__ st, 
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int i = 8 * (r >> 2) + 4 * h + (r & 3);
-    st[base | row_off(i, a)] = make_float2(accr[r], acci[r]);
+    X[i * kRowStride + col] = make_float2(accr[r], acci[r]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    uint32_t r, cc;
+    uint64_t off;
+    tile_rc(2u * (threadIdx.x + 256 * k), a, r, cc, off);
+    const uint32_t r1 = c0 < 8 ? (r | (1u << c0)) : r;
+    const uint32_t cc1 = c0 < 8 ? cc : (cc | (1u << (c0 - 8)));
+    st_nt(st4 + ((base | off) >> 1), cat4(X[r * kRowStride + cc], X[r1 * kRowStride + cc1]));
   }
 }
 
@@ -114,10 +242,41 @@ int qsim_apply_block_mfma(void* stream, void* state, int n_qubits, const void* b
   a.nc = 0;
   for (int q = 0; q < n_qubits; ++q)
     if (!(seen >> q & 1)) a.cp[a.nc++] = (uint8_t)q;
-  const uint64_t n_waves = (1ull << (n_qubits - 5)) / 32;
+  // Blocks with three or more qubits among 0..4 make the direct loads stride
+  // across many cache lines per instruction; they go through the LDS tile.
+  int low = 0;
+  for (int j = 0; j < 5; ++j) low += a.q[j] < 5;
+  if (n_qubits >= 12 && (low >= 3 || n_qubits > 32)) {
+    Block5Tile tl;
+    memset(&tl, 0, sizeof tl);
+    tl.n = n_qubits;
+    uint64_t tmask = seen;
+    for (int k = 0; k < 7; ++k) tmask |= 1ull << a.cp[k];   // 7 lowest non-block qubits
+    int nt = 0;
+    for (int q = 0; q < n_qubits; ++q) {
+      if (!(tmask >> q & 1)) continue;
+      tl.tq[nt] = (uint8_t)q;
+      int code = -1;
+      for (int j = 0; j < 5; ++j)
+        if (a.q[j] == q) code = j;
+      if (code < 0)
+        for (int k = 0; k < 7; ++k)
+          if (a.cp[k] == q) code = 8 + k;
+      tl.tcode[nt++] = (uint8_t)code;
+    }
+    tl.nh = 0;
+    for (int k = 7; k < a.nc; ++k) tl.hp[tl.nh++] = a.cp[k];
+    hipLaunchKernelGGL(k_block5_lds, dim3(1ull << tl.nh), dim3(256), 0, as_stream(stream),
+                       (float2*)state, tl, (const float2*)block_dev);
+    return launch_status("apply_block_mfma");
+  }
+  const uint64_t n_sets = (1ull << (n_qubits - 5)) / 32;
   constexpr int W = 4;
-  hipLaunchKernelGGL((k_block5<W>), dim3(ceil_div(n_waves, W)), dim3(64 * W), 0, as_stream(stream),
-                     (float2*)state, n_waves, a, (const float2*)block_dev);
+  // persistent grid: two workgroups of 4 waves per CU (the kernel holds ~180
+  // registers per lane, two waves per SIMD), never more than there are sets
+  const uint64_t blocks = std::min<uint64_t>(ceil_div(n_sets, W), 256 * 2);
+  hipLaunchKernelGGL((k_block5<W>), dim3(blocks), dim3(64 * W), 0, as_stream(stream),
+                     (float2*)state, n_sets, a, (const float2*)block_dev);
   return launch_status("apply_block_mfma");
 }
 

@@ -59,7 +59,7 @@ __global__ __launch_bounds__(kThreads) void k_red1(const float4* __restrict__ st
   const uint64_t end = beg + chunk < n4 ? beg + chunk : n4;
   double s = 0.0, s0 = 0.0;
   for (uint64_t f = beg + threadIdx.x; f < end; f += kThreads) {
-    const float4 v = st[f];
+    const float4 v = ld_nt(st + f);
     const double a = p2(lo2(v)), b = p2(hi2(v));
     s += a + b;
     if (MODE == 1) {
@@ -137,13 +137,13 @@ __global__ __launch_bounds__(kThreads) void k_collapse(float4* __restrict__ st, 
   const uint64_t f = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
   if (f == 0) *outcome = o;
   if (f >= n4) return;
-  float4 v = st[f];
+  float4 v = ld_nt(st + f);
   const uint64_t i0 = 2 * f;
   const int b0 = (int)((i0 >> t) & 1), b1 = (int)(((i0 + 1) >> t) & 1);
   const double k0 = b0 == o ? sc : 0.0, k1 = b1 == o ? sc : 0.0;
   v.x = (float)(v.x * k0); v.y = (float)(v.y * k0);
   v.z = (float)(v.z * k1); v.w = (float)(v.w * k1);
-  st[f] = v;
+  st_nt(st + f, v);
 }
 
 __global__ __launch_bounds__(kThreads) void k_scale(float4* __restrict__ st, uint64_t n4,
@@ -152,17 +152,17 @@ __global__ __launch_bounds__(kThreads) void k_scale(float4* __restrict__ st, uin
   const double sc = tot > 0.0 ? 1.0 / sqrt(tot) : 0.0;
   const uint64_t f = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
   if (f >= n4) return;
-  float4 v = st[f];
+  float4 v = ld_nt(st + f);
   v.x = (float)(v.x * sc); v.y = (float)(v.y * sc);
   v.z = (float)(v.z * sc); v.w = (float)(v.w * sc);
-  st[f] = v;
+  st_nt(st + f, v);
 }
 
 __global__ __launch_bounds__(kThreads) void k_probs(const float4* __restrict__ st, uint64_t n4,
                                                     float2* __restrict__ probs) {
   const uint64_t f = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
   if (f >= n4) return;
-  const float4 v = st[f];
+  const float4 v = ld_nt(st + f);
   probs[f] = make_float2((float)p2(lo2(v)), (float)p2(hi2(v)));
 }
 

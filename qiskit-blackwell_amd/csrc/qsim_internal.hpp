@@ -118,6 +118,41 @@ __device__ __forceinline__ float2 lo2(float4 v) { return make_float2(v.x, v.y); 
 __device__ __forceinline__ float2 hi2(float4 v) { return make_float2(v.z, v.w); }
 __device__ __forceinline__ float4 cat4(float2 a, float2 b) { return make_float4(a.x, a.y, b.x, b.y); }
 
+// Nontemporal 16-byte global accesses (global_load/store_dwordx4 ... nt):
+// streaming data that is touched once per kernel does not displace the L2/MALL
+// working set.  Every caller accesses whole cache lines from one instruction
+// (lane-contiguous float4 slots), so no line is fetched twice.
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld_nt(const float4* p) {
+  const f32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_nt(float4* p, float4 v) {
+  const f32x4_t w = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(w, reinterpret_cast<f32x4_t*>(p));
+}
+
+// Cross-lane exchange of a float4 with lane ^ m (m < 64).
+__device__ __forceinline__ float4 shfl_xor4(float4 v, int m) {
+  float4 r;
+  r.x = __shfl_xor(v.x, m, 64);
+  r.y = __shfl_xor(v.y, m, 64);
+  r.z = __shfl_xor(v.z, m, 64);
+  r.w = __shfl_xor(v.w, m, 64);
+  return r;
+}
+
+// XCD-aware block order.  Workgroups are dispatched round-robin over the 8
+// XCDs (block b -> XCD b % 8); renumbering so that XCD x walks the x-th
+// eighth of the grid keeps each XCD's L2 and DRAM pages on one contiguous
+// region instead of 4 KiB stripes shared by all eight.
+template <typename I>
+__device__ __forceinline__ I xcd_block() {
+  const I nb = gridDim.x, b = blockIdx.x;
+  if (nb & 7) return b;
+  return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
 // Gate coefficients passed by value (kernel arguments live in SGPRs).
 struct M2 { float2 g[4]; };
 struct M4 { float2 g[16]; };

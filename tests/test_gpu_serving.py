@@ -1,0 +1,65 @@
+"""Continuous-batching server on the GPU: every request is answered, its
+samples are bitwise the same when it runs alone and inside a mixed batch,
+and the sampled distribution matches the FP64 statevector (chi-square)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import statevector as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sim():
+    from blackwell_backend import BlackwellSimulator
+    return BlackwellSimulator("cuda:0")
+
+
+def _reqs(ns, shots=2048, base_seed=17):
+    from blackwell_backend import serving as S
+    return [S.su4_request(i, 0.0, n, shots, base_seed * 1000 + i) for i, n in enumerate(ns)]
+
+
+def test_alone_equals_mixed_and_matches_fp64(sim):
+    from blackwell_backend import serving as S, workloads as W
+    from blackwell_backend.circuit import instructions
+    ns = [10, 12, 14, 11, 16, 13, 10, 18, 12, 15]
+    mixed = _reqs(ns)
+    st = S.Server(sim).serve(mixed, realtime=False)
+    assert st["served"] == len(ns)
+    for i, r in enumerate(mixed):
+        alone = _reqs([n for n in ns])[i]
+        S.Server(sim).serve([alone], realtime=False)
+        assert np.array_equal(alone.samples, r.samples), (i, r.n_qubits)
+        assert r.lane == ("lds" if r.n_qubits <= 14 else "hbm")
+    # distributions against FP64 for a few requests
+    for r in mixed[:3] + [mixed[4]]:
+        qc = W.su4_volume(r.n_qubits, r.n_qubits, r.program[3])
+        psi = O.run_instructions(r.n_qubits, [(nm, q, p) for nm, q, _, p in instructions(qc)])
+        p, _, _ = O.chi_square_p(r.samples, O.probs(psi), r.shots)
+        assert p > 1e-3, (r.n_qubits, p)
+        # the samples are the inverse CDF of the seeded uniforms on the FP64 state,
+        # except where the FP32 state moves a CDF boundary past a uniform
+        u = O.splitmix_uniforms(r.seed, r.shots)
+        assert np.mean(O.sample(psi, u) == r.samples) > 0.99
+
+
+def test_plain_path_matches_distribution(sim):
+    from blackwell_backend import serving as S
+    reqs = _reqs([10, 13, 15])
+    S.serve_plain(reqs, sim, realtime=False)
+    served = _reqs([10, 13, 15])
+    S.Server(sim).serve(served, realtime=False)
+    for a, b in zip(reqs, served):
+        # same uniforms, different (but both FP32-compensated) arithmetic
+        assert np.mean(a.samples == b.samples) > 0.99
+
+
+def test_budget_and_lanes_on_device(sim):
+    from blackwell_backend import serving as S
+    reqs = _reqs([20, 10, 21, 11])
+    srv = S.Server(sim, budget_bytes=(8 << 21) + (4 << 20))
+    st = srv.serve(reqs, realtime=False)
+    assert st["rejected"] == 0 and st["served"] == 4
+    assert srv.peak_in_use <= srv.budget
