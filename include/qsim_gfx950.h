@@ -165,6 +165,25 @@ int qsim_tile_program(void* stream, void* state, int n_qubits, uint64_t tile_mas
 int qsim_apply_block_mfma(void* stream, void* state, int n_qubits, const void* block_dev,
                           const int* qubits, int k);
 
+/* ---- host runtime (new) --------------------------------------------------- */
+/* Dense-block planner (host only, FP64).  Ops in circuit order: op i has
+ * op_nq[i] (1..8) qubits op_qubits[8*i .. 8*i+op_nq[i]) and a row-major
+ * complex matrix of 2^nq x 2^nq (re, im doubles) in op_mats, concatenated.
+ * Packs them greedily into blocks of exactly min(k, n) qubits (k <= 6) such
+ * that every op still follows the earlier ops it shares a qubit with.
+ * Outputs, in execution order, *n_out entries out_ref[e]: b >= 0 names block
+ * b (qubits block_qubits[8*b ..], ascending, -1 padded; matrix block_mats +
+ * b*2^(2k)*2 doubles, row bit j <-> j-th block qubit) and -1-i passes op i
+ * through (wider than k).  Capacity: n_ops entries and blocks.            */
+int qsim_plan_blocks(int n_qubits, int k, int n_ops, const int32_t* op_nq, const int32_t* op_qubits,
+                     const double* op_mats, int32_t* out_ref, int32_t* block_qubits,
+                     double* block_mats, int* n_out);
+/* Apply n_blocks 5-qubit blocks in order with qsim_apply_block_mfma:
+ * blocks_dev complex64 [n_blocks][32][32], block_qubits host int32
+ * [n_blocks][8] (first 5 used).  One call issues the whole plan.         */
+int qsim_apply_blocks(void* stream, void* state, int n_qubits, const void* blocks_dev,
+                      const int32_t* block_qubits, int n_blocks);
+
 /* ---- reductions, measurement, sampling ---------------------------------- */
 /* Size in bytes of the workspace the reduction entry points need for a
  * state of n_qubits (new; the reference keeps process-global scratch,

@@ -293,58 +293,58 @@ def _mask(qs):
     return m
 
 
-def plan_blocks(ops, n, k=5, window=None):
-    """Greedy dependency-aware packing of unitary ops into <=k-qubit blocks.
-
-    Scans the remaining ops in order; an op joins the current block if none
-    of its qubits is touched by an earlier op that was left out (so every
-    op still sees its predecessors first) and the union of qubits stays
-    <= k.  measure/reset stay as their own entries (returned as Op).
-    Qubit sets are integer bitmasks.  Returns a list of Block | Op in
-    execution order."""
-    remaining = [(op, _mask(op.all_qubits)) for op in ops]
+def plan_blocks(ops, n, k=5):
+    """Pack unitary ops into dense blocks of min(k, n) qubits with the
+    library's host planner (qsim_plan_blocks, FP64): a greedy scan in
+    circuit order where an op joins the current block if none of its qubits
+    was touched by an earlier op left out, and the union stays <= k.
+    measure/reset split the list into independently planned segments.
+    Returns a list of Block | Op (ops wider than k pass through)."""
     out = []
-    while remaining:
-        if remaining[0][0].kind in ("measure", "reset"):
-            out.append(remaining.pop(0)[0])
-            continue
-        bq = 0
-        nb = 0
-        chosen, keep = [], []
-        blocked = 0
-        for idx, (op, m) in enumerate(remaining):
-            if window is not None and idx >= window:
-                keep.extend(remaining[idx:])
-                break
-            if op.kind in ("measure", "reset") or (m & blocked):
-                blocked |= m
-                keep.append((op, m))
-                continue
-            u = bq | m
-            if bin(u).count("1") <= k:
-                bq = u
-                chosen.append(op)
-            else:
-                blocked |= m
-                keep.append((op, m))
-        remaining = keep
-        if not chosen:
-            # the first op is wider than k qubits: it runs on its own kernel
-            out.append(remaining.pop(0)[0])
-            continue
-        bqs = [q for q in range(n) if bq >> q & 1]
-        # pad to k qubits with the lowest unused qubits so the MFMA shape fits
-        q = 0
-        while len(bqs) < min(k, n) and q < n:
-            if not (bq >> q & 1):
-                bqs.append(q)
-            q += 1
-        bqs.sort()
-        mat = np.eye(1 << len(bqs), dtype=np.complex128)
-        for op in chosen:
-            mat = apply_to_block(mat, op.full_matrix(), list(op.all_qubits), bqs)
-        out.append(Block(bqs, chosen, mat))
+    seg = []
+    for op in ops:
+        if op.kind in ("measure", "reset"):
+            out.extend(_plan_segment(seg, n, k))
+            seg = []
+            out.append(op)
+        else:
+            seg.append(op)
+    out.extend(_plan_segment(seg, n, k))
     return out
+
+
+def _plan_segment(ops, n, k):
+    if not ops:
+        return []
+    import ctypes
+    from blackwell_kernels import _lib
+    m = len(ops)
+    nq = np.zeros(m, dtype=np.int32)
+    qb = np.full((m, 8), -1, dtype=np.int32)
+    mats = []
+    for i, op in enumerate(ops):
+        qs = op.all_qubits
+        nq[i] = len(qs)
+        qb[i, :len(qs)] = qs
+        mats.append(np.asarray(op.full_matrix(), dtype=np.complex128).reshape(-1))
+    mv = np.ascontiguousarray(np.concatenate(mats)).view(np.float64)
+    kk = min(k, n)
+    D = 1 << kk
+    out_ref = np.zeros(m, dtype=np.int32)
+    bqs = np.zeros((m, 8), dtype=np.int32)
+    bm = np.zeros((m, D * D), dtype=np.complex128)
+    n_out = ctypes.c_int(0)
+    vp = lambda a: ctypes.c_void_p(a.ctypes.data)
+    _lib.call("qsim_plan_blocks", n, k, m, vp(nq), vp(qb), vp(mv), vp(out_ref), vp(bqs),
+              vp(bm), ctypes.byref(n_out))
+    res = []
+    for e in out_ref[:n_out.value]:
+        if e >= 0:
+            qs = [int(q) for q in bqs[e] if q >= 0]
+            res.append(Block(qs, None, bm[e].reshape(D, D).copy()))
+        else:
+            res.append(ops[-1 - int(e)])
+    return res
 
 
 def apply_to_block(mat, u, qubits, block_qubits):

@@ -88,9 +88,8 @@ class Request:
 def su4_program(n, depth, seed):
     """Whole-state LDS program (OP_DTYPE ops, complex128 coefs) of
     workloads.su4_volume(n, depth, seed), built without per-gate objects."""
-    layers = W.su4_layers(n, depth, seed)
-    pairs = np.array([p for ps, _ in layers for p in ps], dtype=np.int64).reshape(-1, 2)
-    us = np.concatenate([u for _, u in layers]).reshape(-1, 16)
+    pairs, us = W.su4_arrays(n, depth, seed)
+    us = us.reshape(-1, 16)
     g = len(pairs)
     ops = np.zeros(g, dtype=OP_DTYPE)
     ops["kind"] = _lib.OP_M2

@@ -32,10 +32,7 @@ def haar_unitary(rng, dim):
 def haar_batch(rng, count, dim=4):
     re = rng.standard_normal((count, dim, dim))
     im = rng.standard_normal((count, dim, dim))
-    z = (re + 1j * im) / math.sqrt(2)
-    q, r = np.linalg.qr(z)
-    d = np.diagonal(r, axis1=-2, axis2=-1)
-    return q * (d / np.abs(d))[:, np.newaxis, :]
+    return qr_phase_fixed((re + 1j * im) / math.sqrt(2))
 
 
 def c1_rotation_ladder(n=12, layers=12, seed=1234):
@@ -54,27 +51,49 @@ def c1_rotation_ladder(n=12, layers=12, seed=1234):
     return qc
 
 
-def su4_layers(n, depth, seed):
-    """[(pairs [(q0, q1)], unitaries [P, 4, 4] complex128)] per layer.
+def su4_arrays(n, depth, seed):
+    """(pairs int64 [depth*P, 2], unitaries complex128 [depth*P, 4, 4]),
+    P = floor(n/2), gates in layer order.
 
-    Draw order per layer: the permutation, then the layer's complex Gaussian
-    matrices (real parts, then imaginary parts); all QR factorisations run
-    in one batched call afterwards (same matrices as one call per layer)."""
+    Draws, in this order from np.random.default_rng(seed): the depth
+    permutations (one call, row-wise), then the real parts and then the
+    imaginary parts of all depth*P complex Gaussian 4x4 matrices; the QR
+    factorisations run as one batched call."""
     rng = np.random.default_rng(seed)
     P = n // 2
-    perms, zs = [], []
-    for _ in range(depth):
-        perms.append(rng.permutation(n))
-        re = rng.standard_normal((P, 4, 4))
-        im = rng.standard_normal((P, 4, 4))
-        zs.append((re + 1j * im) / math.sqrt(2))
-    if depth == 0 or P == 0:
-        return [([], np.zeros((0, 4, 4), dtype=np.complex128)) for _ in range(depth)]
-    q, r = np.linalg.qr(np.concatenate(zs))
-    d = np.diagonal(r, axis1=-2, axis2=-1)
-    us = (q * (d / np.abs(d))[:, np.newaxis, :]).reshape(depth, P, 4, 4)
-    return [([(int(pm[2 * j]), int(pm[2 * j + 1])) for j in range(P)], us[k])
-            for k, pm in enumerate(perms)]
+    G = depth * P
+    perms = rng.permuted(np.tile(np.arange(n, dtype=np.int64), (depth, 1)), axis=1)
+    pairs = perms[:, :2 * P].reshape(G, 2)
+    if G == 0:
+        return pairs, np.zeros((0, 4, 4), dtype=np.complex128)
+    re = rng.standard_normal((G, 4, 4))
+    im = rng.standard_normal((G, 4, 4))
+    return pairs, qr_phase_fixed((re + 1j * im) / math.sqrt(2))
+
+
+def qr_phase_fixed(z):
+    """Q of the QR factorisation with a positive real R diagonal, for a batch
+    [G, d, d] — i.e. Q * diag(R_ii / |R_ii|) of any QR — by modified
+    Gram-Schmidt on the columns (vectorised over the batch)."""
+    z = np.array(z, dtype=np.complex128)
+    d = z.shape[-1]
+    q = np.empty_like(z)
+    for j in range(d):
+        v = z[:, :, j]
+        for k in range(j):
+            qk = q[:, :, k]
+            v = v - np.einsum("gi,gi->g", qk.conj(), v)[:, None] * qk
+        q[:, :, j] = v / np.linalg.norm(v, axis=1)[:, None]
+    return q
+
+
+def su4_layers(n, depth, seed):
+    """[(pairs [(q0, q1)], unitaries [P, 4, 4] complex128)] per layer (the
+    same gates as su4_arrays)."""
+    pairs, us = su4_arrays(n, depth, seed)
+    P = n // 2
+    return [([(int(a), int(b)) for a, b in pairs[k * P:(k + 1) * P]], us[k * P:(k + 1) * P])
+            for k in range(depth)]
 
 
 def su4_volume(n=28, depth=28, seed=2024):

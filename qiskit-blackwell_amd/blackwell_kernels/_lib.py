@@ -50,6 +50,8 @@ PROTOTYPES = {
     "qsim_apply_gates_reg_tiled": (_i, [_vp, _vp, _i, _vp, _vp, _i]),
     "qsim_tile_program": (_i, [_vp, _vp, _i, _u64, _vp, _i, _vp]),
     "qsim_apply_block_mfma": (_i, [_vp, _vp, _i, _vp, _pint, _i]),
+    "qsim_plan_blocks": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _pint]),
+    "qsim_apply_blocks": (_i, [_vp, _vp, _i, _vp, _vp, _i]),
     "qsim_reduce_workspace_bytes": (_i64, [_i]),
     "qsim_norm2": (_i, [_vp, _vp, _i, _vp, _vp]),
     "qsim_measure_probs": (_i, [_vp, _vp, _i, _vp]),

@@ -1,0 +1,185 @@
+// Host-side runtime of libqsim_gfx950.so: the dense-block planner and the
+// block-plan launcher.  Pure host code (no kernels): the planner runs on any
+// machine, the launcher issues one MFMA block launch per entry without a
+// round trip through the caller's language per block.
+//
+// Planner.  Ops arrive in circuit order, each with up to 8 qubits and a
+// dense FP64 matrix on them (row bit j <-> qubit j of the op).  A block is
+// grown by scanning the remaining ops in order: an op joins when none of its
+// qubits was touched by an earlier op left out of this block (so every op
+// still follows its predecessors) and the union of qubits stays <= k.  The
+// block matrix is the FP64 product of the members, each applied to the rows
+// of the running 2^k x 2^k matrix.  Blocks are padded with the lowest unused
+// qubits to exactly min(k, n) qubits.  An op wider than k becomes a
+// passthrough entry for the caller to run on its own kernel.
+#include "qsim_internal.hpp"
+#include <complex>
+#include <string.h>
+#include <vector>
+
+namespace qsim {
+
+typedef std::complex<double> cd;
+
+// rows of m (D x D, row bit b <-> block bit b) <- u (on block bits pos[0..kq)) * rows.
+// Plain real arithmetic: std::complex products would go through the
+// IEEE-special-case helper on every multiply.
+static void apply_rows(cd* m, int D, const cd* u, const int* pos, int kq) {
+  const int d = 1 << kq;
+  uint32_t pm = 0;
+  for (int j = 0; j < kq; ++j) pm |= 1u << pos[j];
+  int idx[256];
+  double cr[256], ci[256];
+  const double* U = reinterpret_cast<const double*>(u);
+  double* M = reinterpret_cast<double*>(m);
+  for (int base = 0; base < D; ++base) {
+    if (base & pm) continue;
+    for (int s = 0; s < d; ++s) {
+      int r = base;
+      for (int j = 0; j < kq; ++j)
+        if (s >> j & 1) r |= 1 << pos[j];
+      idx[s] = r;
+    }
+    for (int c = 0; c < D; ++c) {
+      for (int s = 0; s < d; ++s) {
+        cr[s] = M[2 * (idx[s] * D + c)];
+        ci[s] = M[2 * (idx[s] * D + c) + 1];
+      }
+      for (int s = 0; s < d; ++s) {
+        double ar = 0.0, ai = 0.0;
+        const double* ur = U + 2 * (size_t)s * d;
+        for (int t = 0; t < d; ++t) {
+          ar += ur[2 * t] * cr[t] - ur[2 * t + 1] * ci[t];
+          ai += ur[2 * t] * ci[t] + ur[2 * t + 1] * cr[t];
+        }
+        M[2 * (idx[s] * D + c)] = ar;
+        M[2 * (idx[s] * D + c) + 1] = ai;
+      }
+    }
+  }
+}
+
+}  // namespace qsim
+
+using namespace qsim;
+
+extern "C" {
+
+int qsim_plan_blocks(int n_qubits, int k, int n_ops, const int32_t* op_nq, const int32_t* op_qubits,
+                     const double* op_mats, int32_t* out_ref, int32_t* block_qubits,
+                     double* block_mats, int* n_out) {
+  if (n_qubits < 1 || n_qubits > QSIM_MAX_QUBITS) return fail(QSIM_ERANGE, "n_qubits=%d", n_qubits);
+  if (k < 1 || k > QSIM_MAX_BLOCK_QUBITS) return fail(QSIM_ERANGE, "block size k=%d outside 1..%d", k,
+                                                      QSIM_MAX_BLOCK_QUBITS);
+  if (n_ops < 0) return fail(QSIM_EINVAL, "n_ops < 0");
+  if (!n_out || (n_ops > 0 && (!op_nq || !op_qubits || !op_mats || !out_ref || !block_qubits ||
+                               !block_mats)))
+    return fail(QSIM_EINVAL, "NULL pointer argument");
+  const int kk = k < n_qubits ? k : n_qubits;
+  const int D = 1 << kk;
+  std::vector<uint64_t> mask(n_ops);
+  std::vector<size_t> moff(n_ops);
+  size_t off = 0;
+  for (int i = 0; i < n_ops; ++i) {
+    const int nq = op_nq[i];
+    if (nq < 1 || nq > 8) return fail(QSIM_EINVAL, "op %d has %d qubits (1..8)", i, nq);
+    uint64_t m = 0;
+    for (int j = 0; j < nq; ++j) {
+      const int q = op_qubits[i * 8 + j];
+      if (q < 0 || q >= n_qubits) return fail(QSIM_EINVAL, "op %d: qubit %d out of range", i, q);
+      if (m >> q & 1) return fail(QSIM_EINVAL, "op %d: repeated qubit %d", i, q);
+      m |= 1ull << q;
+    }
+    mask[i] = m;
+    moff[i] = off;
+    off += (size_t)1 << (2 * nq);
+  }
+  std::vector<int> rem(n_ops), keep;
+  for (int i = 0; i < n_ops; ++i) rem[i] = i;
+  std::vector<int> chosen;
+  int n_entries = 0, n_blocks = 0;
+  std::vector<cd> mat((size_t)D * D), u;
+  while (!rem.empty()) {
+    uint64_t bq = 0, blocked = 0;
+    chosen.clear();
+    keep.clear();
+    for (size_t t = 0; t < rem.size(); ++t) {
+      const int i = rem[t];
+      const uint64_t m = mask[i];
+      if (m & blocked) {
+        blocked |= m;
+        keep.push_back(i);
+        continue;
+      }
+      const uint64_t un = bq | m;
+      if (__builtin_popcountll(un) <= kk) {
+        bq = un;
+        chosen.push_back(i);
+      } else {
+        blocked |= m;
+        keep.push_back(i);
+      }
+      // a full block whose qubits are all blocked can take nothing more
+      if (__builtin_popcountll(bq) == kk && (blocked & bq) == bq) {
+        for (size_t r = t + 1; r < rem.size(); ++r) keep.push_back(rem[r]);
+        break;
+      }
+    }
+    if (chosen.empty()) {  // the first op is wider than k: passthrough
+      out_ref[n_entries++] = -1 - rem[0];
+      rem.erase(rem.begin());
+      continue;
+    }
+    rem.swap(keep);
+    int bqs[8], nb = 0;
+    for (int q = 0; q < n_qubits && nb < 8; ++q)
+      if (bq >> q & 1) bqs[nb++] = q;
+    for (int q = 0; q < n_qubits && nb < kk; ++q)
+      if (!(bq >> q & 1)) { bqs[nb++] = q; bq |= 1ull << q; }
+    // ascending order
+    for (int a = 0; a < nb; ++a)
+      for (int b = a + 1; b < nb; ++b)
+        if (bqs[b] < bqs[a]) { int t = bqs[a]; bqs[a] = bqs[b]; bqs[b] = t; }
+    std::fill(mat.begin(), mat.end(), cd(0));
+    for (int r = 0; r < D; ++r) mat[(size_t)r * D + r] = 1;
+    for (int i : chosen) {
+      const int nq = op_nq[i];
+      const int d = 1 << nq;
+      u.resize((size_t)d * d);
+      const double* src = op_mats + 2 * moff[i];
+      for (int e = 0; e < d * d; ++e) u[e] = cd(src[2 * e], src[2 * e + 1]);
+      int pos[8];
+      for (int j = 0; j < nq; ++j) {
+        const int q = op_qubits[i * 8 + j];
+        for (int b = 0; b < nb; ++b)
+          if (bqs[b] == q) pos[j] = b;
+      }
+      apply_rows(mat.data(), D, u.data(), pos, nq);
+    }
+    for (int b = 0; b < 8; ++b) block_qubits[n_blocks * 8 + b] = b < nb ? bqs[b] : -1;
+    double* dst = block_mats + (size_t)n_blocks * D * D * 2;
+    for (size_t e = 0; e < (size_t)D * D; ++e) {
+      dst[2 * e] = mat[e].real();
+      dst[2 * e + 1] = mat[e].imag();
+    }
+    out_ref[n_entries++] = n_blocks++;
+  }
+  *n_out = n_entries;
+  return QSIM_OK;
+}
+
+int qsim_apply_blocks(void* stream, void* state, int n_qubits, const void* blocks_dev,
+                      const int32_t* block_qubits, int n_blocks) {
+  if (n_blocks < 0) return fail(QSIM_EINVAL, "n_blocks < 0");
+  if (n_blocks > 0 && (!blocks_dev || !block_qubits)) return fail(QSIM_EINVAL, "NULL pointer argument");
+  const float2* b = (const float2*)blocks_dev;
+  for (int i = 0; i < n_blocks; ++i) {
+    int qs[5];
+    for (int j = 0; j < 5; ++j) qs[j] = block_qubits[i * 8 + j];
+    if (int rc = qsim_apply_block_mfma(stream, state, n_qubits, b + (size_t)i * 1024, qs, 5))
+      return rc;
+  }
+  return QSIM_OK;
+}
+
+}  // extern "C"

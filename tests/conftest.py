@@ -12,6 +12,15 @@ for p in (ROOT, PKG):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built library")
+    # the C-ABI library and the C oracle are built in-tree; build them once if
+    # this checkout has not (hipcc cross-compiles without a GPU)
+    lib = os.path.join(PKG, "blackwell_kernels", "libqsim_gfx950.so")
+    if not os.path.exists(lib):
+        import importlib
+        importlib.import_module("build").build()
+    from oracle import build_ref
+    if not os.path.exists(build_ref.OUT):
+        build_ref.build()
 
 
 def _gpu_ok():
