@@ -170,6 +170,152 @@ def per_gate_sweep(n=30, reps=4):
     return out
 
 
+class StepEvents:
+    """step_timer for _Plan.run: a pair of HIP events around each step on
+    the launch stream, kept for aggregation after one synchronize."""
+
+    def __init__(self, stream):
+        self.stream = stream
+        self.rows = []
+
+    def __call__(self, kind, args):
+        timer = self
+
+        class _Ctx:
+            def __enter__(self_):
+                self_.a = torch.cuda.Event(enable_timing=True)
+                self_.a.record(timer.stream)
+
+            def __exit__(self_, *exc):
+                b = torch.cuda.Event(enable_timing=True)
+                b.record(timer.stream)
+                timer.rows.append((kind, args, self_.a, b))
+        return _Ctx()
+
+
+def _c3_class(kind, a, n):
+    """(class name, algorithmic bytes) of one plain-plan step."""
+    N = 1 << n
+    if kind == "qsim_apply_gate_h":
+        return "1q_dense", 16 * N
+    if kind == "qsim_apply_diagonal_h":
+        return "1q_diagonal", 16 * N
+    if kind == "qsim_apply_diagonal_phase":
+        return "phase", 8 * N
+    if kind == "qsim_apply_mcx":
+        nc = a[2]
+        name = {0: "x", 1: "cnot", 2: "toffoli"}.get(nc, f"mcx{nc}")
+        return name, 16 * (N >> max(nc, 1))
+    return kind, 16 * N
+
+
+def c3_bench(n=32, reps=8, seed=77, shots=1 << 20):
+    """Layered rotation ansatz on 32 qubits (32 GiB complex64): 8 reps of
+    Ry, Rz on every qubit, CNOT q -> q+1, 4 MCX with 3 random controls, run
+    through the plain per-gate kernels (Ry.Rz fused per qubit), then 2^20
+    shots from seeded uniforms.  Per-class GB/s from events per launch."""
+    import blackwell_kernels as bk
+    from blackwell_backend import BlackwellSimulator, workloads as W
+    from blackwell_backend.serving import splitmix_uniforms
+    sim = BlackwellSimulator("cuda:0")
+    qc = W.c3_rotation_mcx(n, reps, seed)
+    cc = sim.compile(qc)
+    plan = cc.plan("plain", sim.device)
+    stream = torch.cuda.current_stream(sim.device)
+    st = bk.state_init(n, device=sim.device)
+    torch.cuda.synchronize()
+    timer = StepEvents(stream)
+    t0 = time.perf_counter()
+    plan.run(st, step_timer=timer)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    classes = {}
+    for kind, a, ea, eb in timer.rows:
+        name, nbytes = _c3_class(kind, a, n)
+        c = classes.setdefault(name, {"launches": 0, "ms": 0.0, "bytes": 0})
+        c["launches"] += 1
+        c["ms"] += ea.elapsed_time(eb)
+        c["bytes"] += nbytes
+    for c in classes.values():
+        gbs = c["bytes"] / (c["ms"] * 1e-3) / 1e9
+        c["ms"] = round(c["ms"], 3)
+        c["GB_per_s"] = round(gbs, 1)
+        c["frac_hbm_peak"] = round(gbs / HBM_PEAK_GBS, 4)
+        del c["bytes"]
+    norm = float(bk.norm2(st))
+    u = torch.from_numpy(splitmix_uniforms(seed, shots)).to(sim.device)
+    bk.sample_with_uniforms(st, u[:16])
+    torch.cuda.synchronize()
+    with EventTimer(stream) as ev:
+        bk.sample_with_uniforms(st, u)
+    samp_ms = ev.ms()
+    del st
+    torch.cuda.empty_cache()
+    return {"workload": f"{n}-qubit rotation + CNOT ladder + 4 MCX(3 controls) ansatz, {reps} reps, "
+                        f"seed {seed}, plain per-gate kernels, then {shots} shots",
+            "launches": len(timer.rows), "circuit_s": round(wall, 3), "norm2_after": norm,
+            "classes": classes,
+            "sample": {"shots": shots, "ms": round(samp_ms, 3),
+                       "state_read_GB_per_s": round((8 << n) / (samp_ms * 1e-3) / 1e9, 1)}}
+
+
+def c5_bench(n=22, n_rhs=32, times=(1.0, 5.0, 10.0), seed=5, precision=1e-7):
+    """Chebyshev evolution exp(-iHt) of 32 normalised complex-Gaussian
+    vectors under the 22-qubit open Heisenberg-type chain, one fused SpMM
+    launch per Chebyshev term.  Algorithmic bytes per term:
+    12 nnz + 4 (N+1) + 5 * 8 N R (CSR once; T_k, T_{k-1} read, T_{k+1}
+    written, accumulator read and written).  Checks that need no FP64
+    solve at this size: norm and energy conservation, and
+    exp(-iH 5) exp(-iH 5) = exp(-iH 10)."""
+    from blackwell_backend import workloads as W
+    from blackwell_backend.hamiltonian import HamiltonianCSR
+    from blackwell_backend.chebyshev import chebyshev_evolve_batch
+    import blackwell_kernels as bk
+    dev = torch.device("cuda", 0)
+    t0 = time.perf_counter()
+    h = HamiltonianCSR.from_sparse_pauli_op(W.heisenberg_chain(n, seed), device=dev)
+    build_s = time.perf_counter() - t0
+    N = 1 << n
+    rng = np.random.default_rng(seed + 1)
+    X = (rng.standard_normal((n_rhs, N)) + 1j * rng.standard_normal((n_rhs, N))).astype(np.complex64)
+    X /= np.linalg.norm(X, axis=1, keepdims=True).astype(np.float32)
+    X = torch.from_numpy(X).to(dev)
+
+    def energy(Y):
+        """<y|H|y> per row (FP64 host reduction of the device SpMV)."""
+        out = []
+        for r in range(min(4, Y.shape[0])):
+            hy = torch.empty_like(Y[r])
+            bk.spmv_csr(h.row_ptr, h.col_idx, h.values, Y[r].contiguous(), hy)
+            out.append(complex(torch.vdot(Y[r].to(torch.complex128), hy.to(torch.complex128))).real)
+        return np.array(out)
+
+    e0 = energy(X)
+    chebyshev_evolve_batch(X[:2], h, 0.1, precision)          # warm
+    res = {}
+    outs = {}
+    for t in times:
+        torch.cuda.synchronize()
+        Y, info = chebyshev_evolve_batch(X, h, t, precision)
+        K = info["chebyshev_degree"]
+        el = info["elapsed_s"]
+        per = 12.0 * h.nnz + 4.0 * (N + 1) + 5.0 * 8.0 * N * n_rhs
+        gbs = per * K / el / 1e9
+        nrm = torch.linalg.vector_norm(Y.to(torch.complex128), dim=1).cpu().numpy()
+        res[f"t={t:g}"] = {"K": K, "s": round(el, 4), "ms_per_term": round(el / K * 1e3, 4),
+                           "GB_per_s": round(gbs, 1), "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+                           "max_norm_dev": float(np.max(np.abs(nrm - 1.0))),
+                           "max_energy_drift": float(np.max(np.abs(energy(Y) - e0)))}
+        outs[t] = Y
+    if 5.0 in outs and 10.0 in outs:
+        Y55, _ = chebyshev_evolve_batch(outs[5.0][:4].contiguous(), h, 5.0, precision)
+        d = (Y55.to(torch.complex128) - outs[10.0][:4].to(torch.complex128)).abs().max().item()
+        res["composition_5+5_vs_10_max_abs"] = d
+    return {"workload": f"{n}-qubit open chain ZZ + (XX+YY) + X, seed {seed}, CSR complex64 "
+                        f"nnz {h.nnz}, {n_rhs} RHS, Bessel tail < {precision:g}",
+            "csr_build_s": round(build_s, 2), "results": res}
+
+
 def cpu_baseline(n, depth, seed, budget_s=15.0):
     """FP64 C/OpenMP statevector on host cores: amp-updates/s on a bounded
     prefix of the same circuit (gates applied until budget_s elapses)."""
@@ -202,7 +348,7 @@ def cpu_baseline(n, depth, seed, budget_s=15.0):
                       f"SU(4) circuit, FP64 C/OpenMP statevector (oracle/sv_fp64.c), {el:.1f} s"}
 
 
-def serving_bench(n_requests=512, seed=99, shots=1024):
+def serving_bench(n_requests=4096, seed=99, shots=1024):
     """Mixed 10..24-qubit random-SU(4) request stream (depth n, 1024 shots):
     the continuous-batching server vs the plain one-circuit-at-a-time path.
     Capacity with all requests queued at t = 0, then Poisson arrivals at
@@ -313,6 +459,16 @@ def main():
             line["per_gate_30q"] = per_gate_sweep(30)
         except Exception as e:
             line["per_gate_30q"] = {"error": repr(e)}
+        try:
+            log("[bench] 32-qubit rotation + MCX ansatz, per-class GB/s and 2^20 shots")
+            line["c3_rotation_mcx_32q"] = c3_bench()
+        except Exception as e:
+            line["c3_rotation_mcx_32q"] = {"error": repr(e)}
+        try:
+            log("[bench] 22-qubit Chebyshev evolution, 32 RHS")
+            line["c5_chebyshev_22q"] = c5_bench()
+        except Exception as e:
+            line["c5_chebyshev_22q"] = {"error": repr(e)}
         try:
             log("[bench] serving stream")
             line["serving"] = serving_bench()

@@ -257,6 +257,26 @@ int qsim_qv8_simulate(void* stream, const void* gate_matrices, const void* gate_
 int qsim_qv4_simulate(void* stream, const void* gate_data, const void* pair_ids,
                       int64_t n_circuits, int n_gates, void* probs_out);
 
+/* ---- Pauli sum -> CSR on the device --------------------------------------- */
+/* Replaces SparsePauliOp.to_matrix(sparse=True).tocsr() plus the H2D copy in
+ * HamiltonianCSR.from_sparse_pauli_op (blackwell_backend/hamiltonian.py:44-69).
+ * Terms grouped by X mask: group g (n_groups of them, X masks xm[g] uint64
+ * ascending) owns the terms t in [toff[g], toff[g+1]) (int32 toff[n_groups+1])
+ * in input order; term t has the Y|Z mask sm[t] (uint64) and an FP64 complex
+ * coefficient coef[t] (2 doubles) with (-i)^popcount(Y) folded in.  Row i of
+ * group g holds sum_t coef[t] (-1)^popcount(i & sm[t]) (FP64, term order) at
+ * column i ^ xm[g]; entries that are exactly zero are dropped; values are
+ * rounded to complex64.  All inputs are device pointers.
+ * Two calls with the same workspace: _count (synchronises `stream`, returns
+ * the number of nonzeros, QSIM_ERANGE above INT32_MAX), then _fill into
+ * int32 row_ptr[2^n+1], int32 col_idx[nnz], complex64 values[nnz].     (new) */
+int64_t qsim_pauli_csr_workspace_bytes(int n_qubits);
+int qsim_pauli_csr_count(void* stream, int n_qubits, int n_groups, const void* xm, const void* toff,
+                         const void* sm, const void* coef, void* ws, int64_t* nnz_out);
+int qsim_pauli_csr_fill(void* stream, int n_qubits, int n_groups, const void* xm, const void* toff,
+                        const void* sm, const void* coef, const void* ws, int64_t nnz,
+                        void* row_ptr, void* col_idx, void* values);
+
 /* ---- Chebyshev evolution ------------------------------------------------- */
 /* y = alpha*A*x + beta*y, CSR int32 row_ptr[n_rows+1], int32 col_idx[nnz],
  * complex64 values[nnz]; per-row sums TwoProduct + Kahan compensated.

@@ -4,6 +4,13 @@ TEST INFRASTRUCTURE ONLY (tests/, smoke() and bench.py's cpu_baseline leg).
 * pauli_dense: brute-force Kronecker products of 2x2 Paulis in Qiskit label
   order (rightmost character = qubit 0), the definition behind
   SparsePauliOp.to_matrix used at blackwell_backend/hamiltonian.py:58.
+* pauli_csr_grouped: the CSR layout the device builder writes (terms
+  grouped by X mask, each group summed per row in FP64 in term order, exact
+  zeros dropped, groups in ascending X-mask order within a row), computed
+  with numpy; pinned against pauli_dense.  The reference gets its CSR from
+  Qiskit's to_matrix(sparse=True) (hamiltonian.py:58), whose exact layout
+  (explicit zeros, column order) only an import could pin; the operator it
+  represents is what pauli_dense fixes.
 * chebyshev_coefficients: restates blackwell_backend/chebyshev.py:26-78
   (c_k = (2 - delta_k0) (-i)^k J_k(alpha), K_est = alpha + 2 ln(1/eps) + 20,
   tail trimmed while the cumulative tail is below eps, K >= 2, conjugated for
@@ -48,6 +55,47 @@ def pauli_sparse(labels, coeffs):
         m = m + c * t
     m.eliminate_zeros()
     return m.tocsr()
+
+
+def _parity(x):
+    """popcount(x) & 1 for a uint64 array."""
+    x = x.copy()
+    s = 32
+    while s:
+        x ^= x >> np.uint64(s)
+        s >>= 1
+    return (x & np.uint64(1)).astype(np.int64)
+
+
+def pauli_csr_grouped(labels, coeffs, n):
+    """(row_ptr int32 [2^n+1], col_idx int32 [nnz], values complex128 [nnz])."""
+    phase = (1 + 0j, -1j, -1 + 0j, 1j)
+    groups = {}
+    for lab, c in zip(labels, coeffs):
+        xm = ym = zm = 0
+        for pos, ch in enumerate(reversed(lab)):
+            if ch in "XY":
+                xm |= 1 << pos
+            if ch == "Y":
+                ym |= 1 << pos
+            if ch == "Z":
+                zm |= 1 << pos
+        groups.setdefault(xm, []).append((complex(c) * phase[bin(ym).count("1") % 4], ym | zm))
+    N = 1 << n
+    rows = np.arange(N, dtype=np.uint64)
+    masks = sorted(groups)
+    vals = np.zeros((N, len(masks)), dtype=np.complex128)
+    cols = np.zeros((N, len(masks)), dtype=np.int64)
+    for j, xm in enumerate(masks):
+        v = np.zeros(N, dtype=np.complex128)
+        for coef, sm in groups[xm]:
+            v += coef * (1 - 2 * _parity(rows & np.uint64(sm)))
+        vals[:, j] = v
+        cols[:, j] = (rows ^ np.uint64(xm)).astype(np.int64)
+    keep = vals != 0
+    row_ptr = np.zeros(N + 1, dtype=np.int64)
+    np.cumsum(keep.sum(axis=1), out=row_ptr[1:])
+    return row_ptr.astype(np.int32), cols[keep].astype(np.int32), vals[keep]
 
 
 def chebyshev_coefficients(t, spectral_bound, precision=1e-10):

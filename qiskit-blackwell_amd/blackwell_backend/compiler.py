@@ -313,11 +313,38 @@ def plan_blocks(ops, n, k=5):
     return out
 
 
+def plan_blocks_arrays(n, op_nq, op_qubits, op_mats, k=5):
+    """The native planner on flat arrays: op_nq int32 [m], op_qubits int32
+    [m, 8] (-1 padded), op_mats complex128 (each op's 2^nq x 2^nq matrix,
+    row bit j <-> op_qubits[j], concatenated).  Returns (refs, block_qubits
+    int32 [B, 8], block_mats complex128 [B, D, D]): refs[e] >= 0 is block
+    refs[e], refs[e] < 0 is op -1 - refs[e] passed through.  Pure host code;
+    the library call releases the GIL, so plans can be built on worker
+    threads."""
+    import ctypes
+    from blackwell_kernels import _lib
+    m = len(op_nq)
+    nq = np.ascontiguousarray(op_nq, dtype=np.int32)
+    qb = np.ascontiguousarray(op_qubits, dtype=np.int32).reshape(m, 8)
+    mv = np.ascontiguousarray(op_mats, dtype=np.complex128).reshape(-1)
+    if mv.size == 0:
+        mv = np.zeros(1, dtype=np.complex128)
+    D = 1 << min(k, n)
+    refs = np.zeros(max(m, 1), dtype=np.int32)
+    bqs = np.zeros((max(m, 1), 8), dtype=np.int32)
+    bm = np.empty((max(m, 1), D, D), dtype=np.complex128)
+    n_out = ctypes.c_int(0)
+    vp = lambda a: ctypes.c_void_p(a.ctypes.data)
+    _lib.call("qsim_plan_blocks", n, k, m, vp(nq), vp(qb), vp(mv.view(np.float64)), vp(refs),
+              vp(bqs), vp(bm), ctypes.byref(n_out))
+    refs = refs[:n_out.value]
+    nb = int((refs >= 0).sum())
+    return refs, bqs[:nb], bm[:nb]
+
+
 def _plan_segment(ops, n, k):
     if not ops:
         return []
-    import ctypes
-    from blackwell_kernels import _lib
     m = len(ops)
     nq = np.zeros(m, dtype=np.int32)
     qb = np.full((m, 8), -1, dtype=np.int32)
@@ -327,21 +354,12 @@ def _plan_segment(ops, n, k):
         nq[i] = len(qs)
         qb[i, :len(qs)] = qs
         mats.append(np.asarray(op.full_matrix(), dtype=np.complex128).reshape(-1))
-    mv = np.ascontiguousarray(np.concatenate(mats)).view(np.float64)
-    kk = min(k, n)
-    D = 1 << kk
-    out_ref = np.zeros(m, dtype=np.int32)
-    bqs = np.zeros((m, 8), dtype=np.int32)
-    bm = np.zeros((m, D * D), dtype=np.complex128)
-    n_out = ctypes.c_int(0)
-    vp = lambda a: ctypes.c_void_p(a.ctypes.data)
-    _lib.call("qsim_plan_blocks", n, k, m, vp(nq), vp(qb), vp(mv), vp(out_ref), vp(bqs),
-              vp(bm), ctypes.byref(n_out))
+    refs, bqs, bm = plan_blocks_arrays(n, nq, qb, np.concatenate(mats), k)
     res = []
-    for e in out_ref[:n_out.value]:
+    for e in refs:
         if e >= 0:
             qs = [int(q) for q in bqs[e] if q >= 0]
-            res.append(Block(qs, None, bm[e].reshape(D, D).copy()))
+            res.append(Block(qs, None, bm[e]))
         else:
             res.append(ops[-1 - int(e)])
     return res

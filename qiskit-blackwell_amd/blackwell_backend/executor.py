@@ -173,15 +173,28 @@ class CompiledCircuit:
         n = self.n_qubits
         mats = []
         steps = []
+        run = []                  # consecutive 5-qubit blocks -> one qsim_apply_blocks call
+
+        def flush():
+            if run:
+                qs = np.full((len(run), 8), -1, dtype=np.int32)
+                for i, b in enumerate(run):
+                    qs[i, :5] = b.qubits
+                steps.append(("blocks", (len(mats), len(run), qs)))
+                mats.extend(b.mat.astype(np.complex64).reshape(-1) for b in run)
+                run.clear()
+
         for e in entries:
+            if isinstance(e, C.Block) and len(e.qubits) == 5:
+                run.append(e)
+                continue
+            flush()
             if isinstance(e, C.Block):
                 mats.append(e.mat.astype(np.complex64).reshape(-1))
-                if len(e.qubits) == 5:
-                    steps.append(("mfma", (len(mats) - 1, _lib.int_array(e.qubits))))
-                else:
-                    steps.append(("block", (len(mats) - 1, list(e.qubits))))
+                steps.append(("block", (len(mats) - 1, list(e.qubits))))
             else:
                 steps.append(("sub", self._build_plain([e], device)))
+        flush()
         return _Plan(self, steps, mats, None, None, device)
 
     def _build_tiles(self, entries, device):
@@ -219,55 +232,68 @@ class _Plan:
                 off += m.size
             mats_arr = np.concatenate(mats).astype(np.complex64)
         self.dev = _Dev(device, mats_arr, ops, coefs)
-        for kind, _ in steps:
-            self.n_launches += 1 if kind != "sub" else 0
         for kind, p in steps:
             if kind == "sub":
                 self.n_launches += p.n_launches
+            elif kind == "blocks":
+                self.n_launches += p[1]
+            else:
+                self.n_launches += 1
 
     def _mat_ptr(self, idx):
         return ctypes.c_void_p(self.dev.mats.data_ptr() + 8 * self.mat_offsets[idx])
 
-    def run(self, state, rand=None, clbits=None):
+    def run(self, state, rand=None, clbits=None, step_timer=None):
         """Apply the plan to `state` (complex64 [2^n] on the plan's device).
-        rand() supplies uniforms for measure/reset."""
+        rand() supplies uniforms for measure/reset.  step_timer(kind, args),
+        when given, returns a context manager entered around each step (for
+        per-kernel-class timing with events on the launch stream)."""
         n = self.cc.n_qubits
         s = _lib.stream_handle(state.device)
         sp = ctypes.c_void_p(state.data_ptr())
         lib = _lib.load()
         for kind, a in self.steps:
-            if kind.startswith("qsim_"):
-                rc = getattr(lib, kind)(s, sp, *a)
-                if rc:
-                    call(kind, s, sp, *a)  # re-raise with the message
-            elif kind == "mfma":
-                call("qsim_apply_block_mfma", s, sp, n, self._mat_ptr(a[0]), a[1], 5)
-            elif kind == "tile":
-                mask, off, cnt = a
-                call("qsim_tile_program", s, sp, n, mask,
-                     ctypes.c_void_p(self.dev.ops.data_ptr() + 16 * off), cnt,
-                     ctypes.c_void_p(self.dev.coefs.data_ptr()))
-            elif kind == "mcgate":
-                idx, ctl, nc, t = a
-                call("qsim_apply_mcgate", s, sp, n, self._mat_ptr(idx), ctl, nc, t)
-            elif kind == "block":
-                idx, qs = a
-                k = len(qs)
-                blk = self.dev.mats[self.mat_offsets[idx]:self.mat_offsets[idx] + (1 << (2 * k))]
-                from blackwell_kernels import apply_block
-                apply_block(state, blk.reshape(1 << k, 1 << k), qs)
-            elif kind == "sub":
-                a.run(state, rand, clbits)
-            elif kind in ("measure", "reset"):
-                from blackwell_kernels import measure_qubit, apply_mcx
-                op = a
-                u = rand() if rand is not None else float(torch.rand(1).item())
-                o = measure_qubit(state, op.qubits[0], u)
-                if kind == "measure":
-                    if clbits is not None and op.clbit is not None and op.clbit < len(clbits):
-                        clbits[op.clbit] = o
-                elif o == 1:
-                    apply_mcx(state, [], op.qubits[0])
+            if step_timer is not None and kind != "sub":
+                with step_timer(kind, a):
+                    self._step(kind, a, n, s, sp, lib, state, rand, clbits, None)
             else:
-                raise RuntimeError(f"unknown step {kind}")
+                self._step(kind, a, n, s, sp, lib, state, rand, clbits, step_timer)
         return state
+
+    def _step(self, kind, a, n, s, sp, lib, state, rand, clbits, step_timer):
+        if kind.startswith("qsim_"):
+            rc = getattr(lib, kind)(s, sp, *a)
+            if rc:
+                call(kind, s, sp, *a)  # re-raise with the message
+        elif kind == "blocks":
+            first, count, qs = a
+            call("qsim_apply_blocks", s, sp, n, self._mat_ptr(first),
+                 ctypes.c_void_p(qs.ctypes.data), count)
+        elif kind == "tile":
+            mask, off, cnt = a
+            call("qsim_tile_program", s, sp, n, mask,
+                 ctypes.c_void_p(self.dev.ops.data_ptr() + 16 * off), cnt,
+                 ctypes.c_void_p(self.dev.coefs.data_ptr()))
+        elif kind == "mcgate":
+            idx, ctl, nc, t = a
+            call("qsim_apply_mcgate", s, sp, n, self._mat_ptr(idx), ctl, nc, t)
+        elif kind == "block":
+            idx, qs = a
+            k = len(qs)
+            blk = self.dev.mats[self.mat_offsets[idx]:self.mat_offsets[idx] + (1 << (2 * k))]
+            from blackwell_kernels import apply_block
+            apply_block(state, blk.reshape(1 << k, 1 << k), qs)
+        elif kind == "sub":
+            a.run(state, rand, clbits, step_timer)
+        elif kind in ("measure", "reset"):
+            from blackwell_kernels import measure_qubit, apply_mcx
+            op = a
+            u = rand() if rand is not None else float(torch.rand(1).item())
+            o = measure_qubit(state, op.qubits[0], u)
+            if kind == "measure":
+                if clbits is not None and op.clbit is not None and op.clbit < len(clbits):
+                    clbits[op.clbit] = o
+            elif o == 1:
+                apply_mcx(state, [], op.qubits[0])
+        else:
+            raise RuntimeError(f"unknown step {kind}")

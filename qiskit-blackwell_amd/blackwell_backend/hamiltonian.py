@@ -2,15 +2,17 @@
 (blackwell_backend/hamiltonian.py:20-79).
 
 The reference converts a qiskit SparsePauliOp with
-spo.to_matrix(sparse=True).tocsr().  This module builds the same operator
-without qiskit, straight from the Pauli strings: a term with label L
-(rightmost character = qubit 0) and coefficient c has exactly one nonzero
-per row, at column i ^ xmask (xmask = X or Y positions), with value
+spo.to_matrix(sparse=True).tocsr() on the CPU and copies the CSR to the
+device.  Here the CSR is built on the device, straight from the Pauli
+strings: a term with label L (rightmost character = qubit 0) and
+coefficient c has exactly one nonzero per row, at column i ^ xmask
+(xmask = X or Y positions), with value
     c * (-i)^{#Y} * (-1)^{popcount(i & (ymask | zmask))}
 (each Y contributes <i_q|Y|i_q^1> = -i (-1)^{i_q}, each Z (-1)^{i_q}).
-Terms sharing an xmask are summed, exact zeros dropped, and each row holds
-its entries in ascending xmask order.  The spectral bound is sum |c_k|
-(hamiltonian.py:66).
+The host groups the terms by xmask (pauli_groups); the library's
+qsim_pauli_csr_count / qsim_pauli_csr_fill sum each group per row in FP64,
+drop exact zeros and write the rows with their entries in ascending xmask
+order.  The spectral bound is sum |c_k| (hamiltonian.py:66).
 """
 import numpy as np
 import torch
@@ -50,18 +52,15 @@ def _labels_and_coeffs(spo):
     return labels, np.asarray(spo.coeffs, dtype=np.complex128), spo.num_qubits
 
 
-def _parity(x):
-    """popcount(x) & 1 for an int64 array."""
-    x = x.copy()
-    s = 32
-    while s:
-        x ^= x >> s
-        s >>= 1
-    return x & 1
+# (-i)^k for k = 0..3, exact
+_MINUS_I_POW = (1 + 0j, -1j, -1 + 0j, 1j)
 
 
-def pauli_to_csr_numpy(labels, coeffs, n):
-    """(row_ptr int32 [N+1], col_idx int32 [nnz], values complex128 [nnz])."""
+def pauli_groups(labels, coeffs):
+    """Terms grouped by X mask, in the layout the device CSR builder takes:
+    (xm uint64 [G] ascending, toff int32 [G+1], sm uint64 [T] (Y|Z masks),
+    coef complex128 [T] with (-i)^#Y folded in), terms of a group in input
+    order."""
     groups = {}
     for lab, c in zip(labels, coeffs):
         xm = ym = zm = 0
@@ -73,24 +72,41 @@ def pauli_to_csr_numpy(labels, coeffs, n):
                 ym |= 1 << pos
             elif ch == "Z":
                 zm |= 1 << pos
-        groups.setdefault(xm, []).append((c * (-1j) ** bin(ym).count("1"), ym | zm))
-    N = 1 << n
-    rows = np.arange(N, dtype=np.int64)
+        groups.setdefault(xm, []).append((complex(c) * _MINUS_I_POW[bin(ym).count("1") % 4], ym | zm))
     masks = sorted(groups)
-    vals = np.zeros((N, len(masks)), dtype=np.complex128)
-    cols = np.zeros((N, len(masks)), dtype=np.int64)
-    for j, xm in enumerate(masks):
-        v = np.zeros(N, dtype=np.complex128)
-        for coef, sm in groups[xm]:
-            sign = 1 - 2 * _parity(rows & sm)
-            v += coef * sign
-        vals[:, j] = v
-        cols[:, j] = rows ^ xm
-    keep = vals != 0
-    counts = keep.sum(axis=1)
-    row_ptr = np.zeros(N + 1, dtype=np.int64)
-    np.cumsum(counts, out=row_ptr[1:])
-    return (row_ptr.astype(np.int32), cols[keep].astype(np.int32), vals[keep])
+    toff = np.zeros(len(masks) + 1, dtype=np.int32)
+    sm, coef = [], []
+    for g, xm in enumerate(masks):
+        for cf, m in groups[xm]:
+            coef.append(cf)
+            sm.append(m)
+        toff[g + 1] = len(sm)
+    return (np.array(masks, dtype=np.uint64), toff, np.array(sm, dtype=np.uint64),
+            np.array(coef, dtype=np.complex128))
+
+
+def pauli_to_csr_device(labels, coeffs, n, device):
+    """(row_ptr int32 [2^n+1], col_idx int32 [nnz], values complex64 [nnz])
+    as device tensors, built by the library from pauli_groups."""
+    import ctypes
+    from blackwell_kernels import _lib
+    xm, toff, sm, coef = pauli_groups(labels, coeffs)
+    dev = torch.device(device)
+    d_xm = torch.from_numpy(xm.view(np.int64)).to(dev)
+    d_toff = torch.from_numpy(toff).to(dev)
+    d_sm = torch.from_numpy(sm.view(np.int64) if sm.size else np.zeros(1, np.int64)).to(dev)
+    d_cf = torch.from_numpy(coef.view(np.float64) if coef.size else np.zeros(2)).to(dev)
+    ws = torch.empty(int(_lib.load().qsim_pauli_csr_workspace_bytes(n)), dtype=torch.uint8, device=dev)
+    s = _lib.stream_handle(dev)
+    args = (n, len(xm), _lib.ptr(d_xm), _lib.ptr(d_toff), _lib.ptr(d_sm), _lib.ptr(d_cf), _lib.ptr(ws))
+    nnz = ctypes.c_int64(0)
+    _lib.call("qsim_pauli_csr_count", s, *args, ctypes.byref(nnz))
+    row_ptr = torch.empty((1 << n) + 1, dtype=torch.int32, device=dev)
+    col_idx = torch.empty(max(nnz.value, 1), dtype=torch.int32, device=dev)
+    values = torch.empty(max(nnz.value, 1), dtype=torch.complex64, device=dev)
+    _lib.call("qsim_pauli_csr_fill", s, *args, nnz.value, _lib.ptr(row_ptr), _lib.ptr(col_idx),
+              _lib.ptr(values))
+    return row_ptr, col_idx[:nnz.value], values[:nnz.value]
 
 
 class HamiltonianCSR:
@@ -108,10 +124,8 @@ class HamiltonianCSR:
     @staticmethod
     def from_sparse_pauli_op(spo, device="cuda"):
         labels, coeffs, n = _labels_and_coeffs(spo)
-        rp, ci, v = pauli_to_csr_numpy(labels, coeffs, n)
-        return HamiltonianCSR(torch.from_numpy(rp).to(device), torch.from_numpy(ci).to(device),
-                              torch.from_numpy(v.astype(np.complex64)).to(device), n,
-                              float(np.sum(np.abs(coeffs))))
+        rp, ci, v = pauli_to_csr_device(labels, coeffs, n, device)
+        return HamiltonianCSR(rp, ci, v, n, float(np.sum(np.abs(coeffs))))
 
     def memory_bytes(self):
         return self.row_ptr.nbytes + self.col_idx.nbytes + self.values.nbytes

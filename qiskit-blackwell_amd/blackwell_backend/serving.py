@@ -9,7 +9,10 @@ concurrently on separate HIP streams:
     workgroup per circuit with its whole state in LDS, its own gate table,
     and in-kernel sampling.  Nothing but the samples touches HBM.
   * HBM lane (n > 14): each circuit in its own slab of HBM, run as fused
-    5-qubit blocks on FP32 MFMA, then sampled with qsim_sample.
+    5-qubit blocks on FP32 MFMA (one qsim_apply_blocks call per circuit),
+    then sampled with qsim_sample.  Block plans are built ahead of time on a
+    pool of host threads (the native planner releases the GIL), so planning
+    overlaps the device work of the circuits before it.
 
 Admission is by statevector bytes against an HBM budget (default 80 % of
 the free device memory; tests inject a small one).  A request that arrives
@@ -60,10 +63,10 @@ def splitmix_uniforms(seed, count):
 class Request:
     """One circuit to serve.  Either `circuit` (any circuit with Qiskit's
     data layout) or a (`program`, `n_qubits`) pair from a program builder."""
-    __slots__ = ("rid", "arrival", "n_qubits", "shots", "seed", "circuit", "program",
+    __slots__ = ("rid", "arrival", "n_qubits", "shots", "seed", "circuit", "program", "gates",
                  "t_admit", "t_done", "samples", "lane")
 
-    def __init__(self, rid, arrival, n_qubits, shots, seed, circuit=None, program=None):
+    def __init__(self, rid, arrival, n_qubits, shots, seed, circuit=None, program=None, gates=None):
         self.rid = rid
         self.arrival = float(arrival)
         self.n_qubits = int(n_qubits)
@@ -71,6 +74,9 @@ class Request:
         self.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
         self.circuit = circuit
         self.program = program
+        # (pairs int [G, 2], unitaries complex128 [G, 4, 4]): a circuit of
+        # two-qubit gates as the client sends it (row bit j <-> pairs[g, j])
+        self.gates = gates
         self.t_admit = None
         self.t_done = None
         self.samples = None
@@ -85,11 +91,9 @@ class Request:
 
 
 # ---------------------------------------------------------------- programs
-def su4_program(n, depth, seed):
-    """Whole-state LDS program (OP_DTYPE ops, complex128 coefs) of
-    workloads.su4_volume(n, depth, seed), built without per-gate objects."""
-    pairs, us = W.su4_arrays(n, depth, seed)
-    us = us.reshape(-1, 16)
+def gates_program(pairs, us):
+    """Whole-state LDS program (OP_DTYPE ops, complex128 coefs) of a list of
+    two-qubit gates, built without per-gate objects."""
     g = len(pairs)
     ops = np.zeros(g, dtype=OP_DTYPE)
     ops["kind"] = _lib.OP_M2
@@ -97,7 +101,12 @@ def su4_program(n, depth, seed):
     ops["q"][:, 0] = pairs[:, 0]
     ops["q"][:, 1] = pairs[:, 1]
     ops["coef"] = np.arange(g, dtype=np.uint32) * 16
-    return ops, us.reshape(-1)
+    return ops, np.asarray(us, dtype=np.complex128).reshape(-1)
+
+
+def su4_program(n, depth, seed):
+    """gates_program of workloads.su4_volume(n, depth, seed)."""
+    return gates_program(*W.su4_arrays(n, depth, seed))
 
 
 def circuit_program(circuit):
@@ -110,12 +119,16 @@ def circuit_program(circuit):
 
 
 def su4_request(rid, arrival, n, shots, seed):
-    """Request for the random-SU(4) volume circuit of depth n."""
-    return Request(rid, arrival, n, shots, seed, program=("su4", n, n, seed))
+    """Request for the random-SU(4) volume circuit of depth n, carrying its
+    gates (built here, on the client side, before it is submitted)."""
+    return Request(rid, arrival, n, shots, seed, program=("su4", n, n, seed),
+                   gates=W.su4_arrays(n, n, seed))
 
 
 def _materialise(req):
     """(ops, coefs) for the LDS lane."""
+    if req.gates is not None:
+        return gates_program(*req.gates)
     p = req.program
     if p is not None and isinstance(p, tuple) and p and p[0] == "su4":
         return su4_program(p[1], p[2], p[3])
@@ -203,18 +216,45 @@ class _HbmJob:
         self.sim = sim
         self.stream = stream
         self.event = None
+        self.prepared = None      # future of prepare() when planned ahead
+        self.blocks = None        # (block qubits int32 [B, 8], matrices complex64 [B, 32, 32])
+
+    def prepare(self):
+        """Host-only block planning for gate-payload requests (thread safe,
+        touches no device state)."""
+        r = self.req
+        if r.gates is not None:
+            pairs, us = r.gates
+            g = len(pairs)
+            qb = np.full((g, 8), -1, dtype=np.int32)
+            qb[:, :2] = pairs
+            refs, bqs, bm = C.plan_blocks_arrays(r.n_qubits, np.full(g, 2, np.int32), qb, us, k=5)
+            assert (refs >= 0).all()          # two-qubit gates always fit a 5-qubit block
+            self.blocks = (np.ascontiguousarray(bqs), np.ascontiguousarray(bm, dtype=np.complex64))
+        return self
 
     def launch(self):
         r = self.req
         dev = self.sim.device
-        cc = self.sim.compile(_circuit_of(r))
-        plan = cc.plan(cc.resolve_mode("blocks"), dev)
+        if self.prepared is not None:
+            self.prepared.result()
+        elif r.gates is not None:
+            self.prepare()
+        sh = ctypes.c_void_p(self.stream.cuda_stream)
         with torch.cuda.stream(self.stream):
             self.state = torch.empty(1 << r.n_qubits, dtype=torch.complex64, device=dev)
-            call("qsim_state_init", ctypes.c_void_p(self.stream.cuda_stream),
-                 ctypes.c_void_p(self.state.data_ptr()), r.n_qubits)
-            plan.run(self.state)
-            u = torch.from_numpy(splitmix_uniforms(r.seed, r.shots)).to(dev)
+            call("qsim_state_init", sh, ctypes.c_void_p(self.state.data_ptr()), r.n_qubits)
+            if self.blocks is not None:
+                bqs, bm = self.blocks
+                h = torch.from_numpy(bm.view(np.float32).reshape(-1)).pin_memory()
+                self.d_mats = h.to(dev, non_blocking=True)
+                call("qsim_apply_blocks", sh, ctypes.c_void_p(self.state.data_ptr()), r.n_qubits,
+                     ctypes.c_void_p(self.d_mats.data_ptr()), ctypes.c_void_p(bqs.ctypes.data),
+                     len(bqs))
+            else:
+                cc = self.sim.compile(_circuit_of(r))
+                cc.plan(cc.resolve_mode("blocks"), dev).run(self.state)
+            u = torch.from_numpy(splitmix_uniforms(r.seed, r.shots)).pin_memory().to(dev, non_blocking=True)
             from blackwell_kernels import sample_with_uniforms
             self.d_out = sample_with_uniforms(self.state, u)
             self.h_out = torch.empty(r.shots, dtype=torch.int64, pin_memory=True)
@@ -229,20 +269,23 @@ class _HbmJob:
         self.event.synchronize()
         self.req.samples = self.h_out.numpy().copy()
         self.req.t_done = now
-        self.state = None
+        self.state = self.d_mats = None
 
 
 # ---------------------------------------------------------------- server
 class Server:
     """Continuous-batching server.
 
-    budget_bytes   HBM budget for resident states (None: 80 % of free memory)
-    max_batch      most requests per LDS launch
-    max_hbm_jobs   HBM-lane circuits in flight at once
+    budget_bytes     HBM budget for resident states (None: 80 % of free memory)
+    max_batch        most requests per LDS launch
+    max_hbm_jobs     HBM-lane circuits in flight at once
+    planner_threads  host threads that build HBM-lane block plans ahead
+    plan_ahead       most queued HBM-lane circuits planned ahead of admission
     """
 
     def __init__(self, sim=None, budget_bytes=None, max_batch=2048, max_hbm_jobs=4,
-                 device="cuda:0", lds_factory=None, hbm_factory=None):
+                 device="cuda:0", lds_factory=None, hbm_factory=None, planner_threads=8,
+                 plan_ahead=64):
         """lds_factory(reqs) / hbm_factory(req) build lane jobs with launch(),
         done() and finish(now); the defaults run on the GPU, tests inject
         host-only fakes to drive the scheduling logic."""
@@ -258,6 +301,8 @@ class Server:
         self.budget = int(budget_bytes)
         self.max_batch = int(max_batch)
         self.max_hbm_jobs = int(max_hbm_jobs)
+        self.planner_threads = int(planner_threads)
+        self.plan_ahead = int(plan_ahead)
         self.in_use = 0
         self.peak_in_use = 0
         if fake:
@@ -285,18 +330,39 @@ class Server:
         self.rejected = [r for r in reqs if self.admit_bytes(r) > self.budget]
         rej = {id(r) for r in self.rejected}
         pending = collections.deque(r for r in reqs if id(r) not in rej)
+        pool = None
+        if self.planner_threads > 0:
+            from concurrent.futures import ThreadPoolExecutor
+            pool = ThreadPoolExecutor(self.planner_threads, thread_name_prefix="qsim-plan")
+        try:
+            return self._serve(reqs, pending, pool, realtime, timeout_s)
+        finally:
+            if pool is not None:
+                pool.shutdown(wait=True, cancel_futures=True)
+
+    def _serve(self, reqs, pending, pool, realtime, timeout_s):
         queue_lds, queue_hbm = collections.deque(), collections.deque()
         lds_inflight, hbm_inflight = [], []
+        n_planned = 0             # queue_hbm[:n_planned] have plans under way
         t0 = time.perf_counter()
         clock = (lambda: time.perf_counter() - t0) if realtime else None
         done = 0
-        total = len(pending) + 0
+        total = len(pending)
         while done < total:
             now = clock() if realtime else float("inf")
             while pending and pending[0].arrival <= now:
                 r = pending.popleft()
                 r.lane = self.lane_of(r)
-                (queue_lds if r.lane == "lds" else queue_hbm).append(r)
+                if r.lane == "lds":
+                    queue_lds.append(r)
+                else:
+                    queue_hbm.append(self.hbm_factory(r))
+            # plan queued HBM-lane circuits ahead, on the host threads
+            while pool is not None and n_planned < min(len(queue_hbm), self.plan_ahead):
+                j = queue_hbm[n_planned]
+                if hasattr(j, "prepare"):
+                    j.prepared = pool.submit(j.prepare)
+                n_planned += 1
             # retire finished work, freeing budget
             for b in [b for b in lds_inflight if b.done()]:
                 b.finish(clock() if realtime else time.perf_counter() - t0)
@@ -332,14 +398,18 @@ class Server:
                     launched = True
             # HBM lane
             while queue_hbm and len(hbm_inflight) < self.max_hbm_jobs:
-                r = queue_hbm[0]
+                j = queue_hbm[0]
+                fut = getattr(j, "prepared", None)
+                if fut is not None and not fut.done():
+                    break                 # its plan is still being built
+                r = j.reqs[0]
                 need = self.admit_bytes(r)
                 if self.in_use + need > self.budget:
                     break
                 queue_hbm.popleft()
+                n_planned = max(0, n_planned - 1)
                 self.in_use += need
                 r.t_admit = time.perf_counter() - t0
-                j = self.hbm_factory(r)
                 j.launch()
                 self.launches += 1
                 self.peak_in_use = max(self.peak_in_use, self.in_use)
@@ -348,7 +418,7 @@ class Server:
             if time.perf_counter() - t0 > timeout_s:
                 raise TimeoutError(f"serving did not finish within {timeout_s} s ({done}/{total} done)")
             if not launched:
-                if lds_inflight or hbm_inflight:
+                if lds_inflight or hbm_inflight or queue_hbm:
                     time.sleep(0.0002)
                 elif pending and realtime:
                     time.sleep(max(0.0, min(0.001, pending[0].arrival - clock())))
@@ -373,9 +443,31 @@ def summarize(reqs, elapsed, rejected=()):
     }
 
 
+def _plain_gates(state, req, stream):
+    """One qsim_apply_gate_2q_h launch per gate of a gate-payload request."""
+    pairs, us = req.gates
+    g = len(pairs)
+    km = np.array(us, dtype=np.complex128)
+    # the 2Q kernel's row index is 2*bit(max qubit) + bit(min qubit)
+    swap = pairs[:, 0] > pairs[:, 1]
+    perm = [0, 2, 1, 3]
+    km[swap] = km[swap][:, perm][:, :, perm]
+    d = np.ascontiguousarray(km).view(np.float64).reshape(g, 32)
+    fn = _lib.load().qsim_apply_gate_2q_h
+    sp = ctypes.c_void_p(state.data_ptr())
+    base = d.ctypes.data
+    pd = ctypes.POINTER(ctypes.c_double)
+    n = req.n_qubits
+    for i in range(g):
+        rc = fn(stream, sp, n, ctypes.cast(base + 256 * i, pd), int(pairs[i, 0]), int(pairs[i, 1]))
+        if rc:
+            raise _lib.QsimError("qsim_apply_gate_2q_h: " + _lib.load().qsim_last_error().decode())
+
+
 def serve_plain(requests, sim, realtime=True):
     """The plain baseline: one circuit at a time, in arrival order, through
-    the per-gate kernels, then qsim_sample with the same uniforms."""
+    the per-gate kernels (one launch per gate, matrices by value), then
+    qsim_sample with the same uniforms, synchronising on each result."""
     from blackwell_kernels import sample_with_uniforms
     reqs = sorted(requests, key=lambda r: r.arrival)
     dev = sim.device
@@ -386,11 +478,14 @@ def serve_plain(requests, sim, realtime=True):
             if wait > 0:
                 time.sleep(wait)
         r.t_admit = time.perf_counter() - t0
-        cc = sim.compile(_circuit_of(r))
-        plan = cc.plan("plain", dev)
+        sh = _lib.stream_handle(dev)
         st = torch.empty(1 << r.n_qubits, dtype=torch.complex64, device=dev)
-        call("qsim_state_init", _lib.stream_handle(dev), ctypes.c_void_p(st.data_ptr()), r.n_qubits)
-        plan.run(st)
+        call("qsim_state_init", sh, ctypes.c_void_p(st.data_ptr()), r.n_qubits)
+        if r.gates is not None:
+            _plain_gates(st, r, sh)
+        else:
+            cc = sim.compile(_circuit_of(r))
+            cc.plan("plain", dev).run(st)
         u = torch.from_numpy(splitmix_uniforms(r.seed, r.shots)).to(dev)
         r.samples = sample_with_uniforms(st, u).cpu().numpy()
         r.t_done = time.perf_counter() - t0

@@ -64,6 +64,9 @@ PROTOTYPES = {
     "qsim_run_instances": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _i]),
     "qsim_qv8_simulate": (_i, [_vp, _vp, _vp, _i64, _i, _vp]),
     "qsim_qv4_simulate": (_i, [_vp, _vp, _vp, _i64, _i, _vp]),
+    "qsim_pauli_csr_workspace_bytes": (_i64, [_i]),
+    "qsim_pauli_csr_count": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _pi64]),
+    "qsim_pauli_csr_fill": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
     "qsim_spmv_csr": (_i, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _d, _d, _d, _d]),
     "qsim_chebyshev_step": (_i, [_vp, _vp, _vp, _vp, _i64]),
     "qsim_chebyshev_accum": (_i, [_vp, _vp, _vp, _i64, _d, _d]),

@@ -13,25 +13,23 @@
 // qubits to exactly min(k, n) qubits.  An op wider than k becomes a
 // passthrough entry for the caller to run on its own kernel.
 #include "qsim_internal.hpp"
-#include <complex>
+#include <algorithm>
 #include <string.h>
 #include <vector>
 
 namespace qsim {
 
-typedef std::complex<double> cd;
-
-// rows of m (D x D, row bit b <-> block bit b) <- u (on block bits pos[0..kq)) * rows.
-// Plain real arithmetic: std::complex products would go through the
-// IEEE-special-case helper on every multiply.
-static void apply_rows(cd* m, int D, const cd* u, const int* pos, int kq) {
+// Block matrix kept as separate real and imaginary D x D arrays so that the
+// row updates below run as contiguous loops over the columns.
+//
+// rows of (re, im) (row bit b <-> block bit b) <- u (on block bits pos[0..kq))
+// times those rows; u is row-major complex (re, im interleaved).
+static void apply_rows(double* re, double* im, int D, const double* u, const int* pos, int kq,
+                       double* tre, double* tim) {
   const int d = 1 << kq;
   uint32_t pm = 0;
   for (int j = 0; j < kq; ++j) pm |= 1u << pos[j];
   int idx[256];
-  double cr[256], ci[256];
-  const double* U = reinterpret_cast<const double*>(u);
-  double* M = reinterpret_cast<double*>(m);
   for (int base = 0; base < D; ++base) {
     if (base & pm) continue;
     for (int s = 0; s < d; ++s) {
@@ -40,20 +38,23 @@ static void apply_rows(cd* m, int D, const cd* u, const int* pos, int kq) {
         if (s >> j & 1) r |= 1 << pos[j];
       idx[s] = r;
     }
-    for (int c = 0; c < D; ++c) {
-      for (int s = 0; s < d; ++s) {
-        cr[s] = M[2 * (idx[s] * D + c)];
-        ci[s] = M[2 * (idx[s] * D + c) + 1];
-      }
-      for (int s = 0; s < d; ++s) {
-        double ar = 0.0, ai = 0.0;
-        const double* ur = U + 2 * (size_t)s * d;
-        for (int t = 0; t < d; ++t) {
-          ar += ur[2 * t] * cr[t] - ur[2 * t + 1] * ci[t];
-          ai += ur[2 * t] * ci[t] + ur[2 * t + 1] * cr[t];
+    for (int t = 0; t < d; ++t) {
+      memcpy(tre + (size_t)t * D, re + (size_t)idx[t] * D, sizeof(double) * D);
+      memcpy(tim + (size_t)t * D, im + (size_t)idx[t] * D, sizeof(double) * D);
+    }
+    for (int s = 0; s < d; ++s) {
+      double* ore = re + (size_t)idx[s] * D;
+      double* oim = im + (size_t)idx[s] * D;
+      const double* us = u + 2 * (size_t)s * d;
+      for (int c = 0; c < D; ++c) ore[c] = oim[c] = 0.0;
+      for (int t = 0; t < d; ++t) {
+        const double ur = us[2 * t], ui = us[2 * t + 1];
+        const double* xr = tre + (size_t)t * D;
+        const double* xi = tim + (size_t)t * D;
+        for (int c = 0; c < D; ++c) {
+          ore[c] += ur * xr[c] - ui * xi[c];
+          oim[c] += ur * xi[c] + ui * xr[c];
         }
-        M[2 * (idx[s] * D + c)] = ar;
-        M[2 * (idx[s] * D + c) + 1] = ai;
       }
     }
   }
@@ -98,7 +99,8 @@ int qsim_plan_blocks(int n_qubits, int k, int n_ops, const int32_t* op_nq, const
   for (int i = 0; i < n_ops; ++i) rem[i] = i;
   std::vector<int> chosen;
   int n_entries = 0, n_blocks = 0;
-  std::vector<cd> mat((size_t)D * D), u;
+  std::vector<double> mre((size_t)D * D), mim((size_t)D * D), tre((size_t)256 * D),
+      tim((size_t)256 * D);
   while (!rem.empty()) {
     uint64_t bq = 0, blocked = 0;
     chosen.clear();
@@ -140,27 +142,24 @@ int qsim_plan_blocks(int n_qubits, int k, int n_ops, const int32_t* op_nq, const
     for (int a = 0; a < nb; ++a)
       for (int b = a + 1; b < nb; ++b)
         if (bqs[b] < bqs[a]) { int t = bqs[a]; bqs[a] = bqs[b]; bqs[b] = t; }
-    std::fill(mat.begin(), mat.end(), cd(0));
-    for (int r = 0; r < D; ++r) mat[(size_t)r * D + r] = 1;
+    std::fill(mre.begin(), mre.end(), 0.0);
+    std::fill(mim.begin(), mim.end(), 0.0);
+    for (int r = 0; r < D; ++r) mre[(size_t)r * D + r] = 1.0;
     for (int i : chosen) {
       const int nq = op_nq[i];
-      const int d = 1 << nq;
-      u.resize((size_t)d * d);
-      const double* src = op_mats + 2 * moff[i];
-      for (int e = 0; e < d * d; ++e) u[e] = cd(src[2 * e], src[2 * e + 1]);
       int pos[8];
       for (int j = 0; j < nq; ++j) {
         const int q = op_qubits[i * 8 + j];
         for (int b = 0; b < nb; ++b)
           if (bqs[b] == q) pos[j] = b;
       }
-      apply_rows(mat.data(), D, u.data(), pos, nq);
+      apply_rows(mre.data(), mim.data(), D, op_mats + 2 * moff[i], pos, nq, tre.data(), tim.data());
     }
     for (int b = 0; b < 8; ++b) block_qubits[n_blocks * 8 + b] = b < nb ? bqs[b] : -1;
     double* dst = block_mats + (size_t)n_blocks * D * D * 2;
     for (size_t e = 0; e < (size_t)D * D; ++e) {
-      dst[2 * e] = mat[e].real();
-      dst[2 * e + 1] = mat[e].imag();
+      dst[2 * e] = mre[e];
+      dst[2 * e + 1] = mim[e];
     }
     out_ref[n_entries++] = n_blocks++;
   }

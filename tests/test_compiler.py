@@ -143,24 +143,28 @@ def test_tile_op_encoding():
     assert list(arr["coef"]) == [0, 4, 20, 20] and cf.size == 21
 
 
-def test_pauli_to_csr_matches_kronecker():
-    from blackwell_backend.hamiltonian import SparsePauliOp, pauli_to_csr_numpy
-    import scipy.sparse as sp
+def test_pauli_groups_evaluate_to_kronecker():
+    """The host grouping handed to the device CSR builder, evaluated here
+    row by row, is the brute-force Kronecker sum."""
+    from blackwell_backend.hamiltonian import pauli_groups
     rng = np.random.default_rng(4)
+    cases = []
     for n in (1, 3, 6):
         labels = ["".join(rng.choice(list("IXYZ"), n)) for _ in range(12)]
-        coeffs = rng.uniform(-1, 1, 12) + 1j * rng.uniform(-1, 1, 12) * (n % 2)
-        rp, ci, v = pauli_to_csr_numpy(labels, coeffs, n)
-        m = sp.csr_matrix((v, ci, rp), shape=(1 << n, 1 << n)).toarray()
-        assert np.allclose(m, OH.pauli_dense(labels, coeffs), atol=1e-14)
+        cases.append((labels, rng.uniform(-1, 1, 12) + 1j * rng.uniform(-1, 1, 12) * (n % 2), n))
     from blackwell_backend import workloads as W
     spo = W.heisenberg_chain(6, seed=1)
-    rp, ci, v = pauli_to_csr_numpy(spo.labels, spo.coeffs, 6)
-    m = sp.csr_matrix((v, ci, rp), shape=(64, 64)).toarray()
-    assert np.allclose(m, OH.pauli_dense(spo.labels, spo.coeffs), atol=1e-14)
-    assert np.allclose(m, m.conj().T)
-    # exact zeros removed: XX+YY cancel on rows where the two bits agree
-    assert len(ci) < 64 * (1 + 5 + 6)
+    cases.append((spo.labels, spo.coeffs, 6))
+    for labels, coeffs, n in cases:
+        xm, toff, sm, coef = pauli_groups(labels, coeffs)
+        assert np.all(np.diff(xm.astype(np.int64)) > 0) and toff[-1] == len(labels)
+        m = np.zeros((1 << n, 1 << n), dtype=np.complex128)
+        for i in range(1 << n):
+            for g in range(len(xm)):
+                for t in range(toff[g], toff[g + 1]):
+                    s = -1.0 if bin(i & int(sm[t])).count("1") % 2 else 1.0
+                    m[i, i ^ int(xm[g])] += coef[t] * s
+        assert np.allclose(m, OH.pauli_dense(labels, coeffs), atol=1e-14)
 
 
 def test_chebyshev_coefficients_match_oracle():

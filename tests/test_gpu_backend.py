@@ -165,6 +165,37 @@ def test_midcircuit_measure_and_reset(sim):
         assert on in ({0, 4}, {3, 7})
 
 
+def _csr_cases():
+    from blackwell_backend import workloads as W
+    rng = np.random.default_rng(11)
+    labels = ["".join(rng.choice(list("IXYZ"), 10)) for _ in range(40)]
+    labels += labels[:6]                                   # repeated strings sum
+    coeffs = rng.uniform(-1, 1, 46) + 1j * rng.uniform(-1, 1, 46)
+    h18 = W.heisenberg_chain(18, seed=2)
+    return {
+        "random_10q": (labels, coeffs, 10),
+        "heisenberg_12q": (W.heisenberg_chain(12, seed=3).labels, W.heisenberg_chain(12, seed=3).coeffs, 12),
+        "heisenberg_18q": (h18.labels, h18.coeffs, 18),    # 1024 row blocks: multi-chunk scan
+        "cancel_2q": (["XX", "YY", "ZZ", "ZZ"], [1.0, 1.0, 0.5, -0.5], 2),
+        "all_zero_3q": (["ZIX"], [0.0], 3),
+        "single_1q": (["Y"], [0.3 + 0.2j], 1),
+    }
+
+
+@pytest.mark.parametrize("case", sorted(_csr_cases()))
+def test_pauli_csr_device_matches_oracle(case):
+    """Device CSR = the oracle's grouped CSR: row_ptr and col_idx equal,
+    values equal after rounding the FP64 sums to complex64."""
+    from blackwell_backend.hamiltonian import pauli_to_csr_device
+    labels, coeffs, n = _csr_cases()[case]
+    print(f"[csr] {case} n={n} terms={len(labels)}", flush=True)
+    rp, ci, v = pauli_to_csr_device(labels, coeffs, n, "cuda:0")
+    rp0, ci0, v0 = OH.pauli_csr_grouped(labels, coeffs, n)
+    assert np.array_equal(rp.cpu().numpy(), rp0)
+    assert np.array_equal(ci.cpu().numpy(), ci0)
+    assert np.array_equal(v.cpu().numpy(), v0.astype(np.complex64))
+
+
 def test_hamiltonian_evolution_matches_expm(sim):
     from blackwell_backend import workloads as W
     from blackwell_backend.hamiltonian import HamiltonianCSR

@@ -63,3 +63,18 @@ def test_budget_and_lanes_on_device(sim):
     st = srv.serve(reqs, realtime=False)
     assert st["rejected"] == 0 and st["served"] == 4
     assert srv.peak_in_use <= srv.budget
+
+
+def test_gate_payload_equals_circuit_path(sim):
+    """An HBM-lane request planned from its gate payload (native planner on
+    a host thread, one qsim_apply_blocks call) and the same circuit compiled
+    from its circuit object give bitwise the same samples."""
+    from blackwell_backend import serving as S
+    for n in (15, 17):
+        a = S.su4_request(0, 0.0, n, 2048, 4242 + n)
+        b = S.Request(1, 0.0, n, 2048, 4242 + n, program=("su4", n, n, 4242 + n))
+        assert b.gates is None
+        S.Server(sim).serve([a], realtime=False)
+        S.Server(sim, planner_threads=0).serve([b], realtime=False)
+        assert a.lane == b.lane == "hbm"
+        assert np.array_equal(a.samples, b.samples), n

@@ -308,3 +308,23 @@ def test_chebyshev_evolution_matches_expm():
     psi = np.array([0.6, 0.8j])
     out = OH.chebyshev_evolve_fp64(hz, psi, 1.3, 1.0, 1e-13)
     assert np.allclose(out, [0.6 * cmath.exp(-1.3j), 0.8j * cmath.exp(1.3j)], atol=1e-12)
+
+
+def test_pauli_csr_grouped_matches_kronecker():
+    import scipy.sparse as sp
+    from blackwell_backend import workloads as W
+    rng = np.random.default_rng(4)
+    for n in (1, 3, 6):
+        labels = ["".join(rng.choice(list("IXYZ"), n)) for _ in range(12)]
+        coeffs = rng.uniform(-1, 1, 12) + 1j * rng.uniform(-1, 1, 12) * (n % 2)
+        rp, ci, v = OH.pauli_csr_grouped(labels, coeffs, n)
+        m = sp.csr_matrix((v, ci, rp), shape=(1 << n, 1 << n)).toarray()
+        assert np.allclose(m, OH.pauli_dense(labels, coeffs), atol=1e-14)
+    spo = W.heisenberg_chain(6, seed=1)
+    rp, ci, v = OH.pauli_csr_grouped(spo.labels, spo.coeffs, 6)
+    m = sp.csr_matrix((v, ci, rp), shape=(64, 64)).toarray()
+    assert np.allclose(m, OH.pauli_dense(spo.labels, spo.coeffs), atol=1e-14)
+    assert np.allclose(m, m.conj().T)
+    # exact zeros removed: XX+YY cancel on rows where the two bits agree
+    assert len(ci) < 64 * (1 + 5 + 6)
+    assert np.all(v != 0)
