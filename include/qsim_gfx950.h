@@ -211,6 +211,11 @@ int64_t qsim_sample_workspace_bytes(int n_qubits);
  * Replaces sample, bwk/csrc/sample_sm120.cu:73-132 (FP32 CDF there).     */
 int qsim_sample(void* stream, const void* state, int n_qubits, const void* uniforms_dev,
                 int64_t n_shots, void* workspace, void* out_dev);
+/* uniforms_dev[k] = (SplitMix64(seed + (k+1) * 0x9E3779B97F4A7C15) >> 11) * 2^-53,
+ * k < count (device double[count]): the stream qsim_run_instances samples
+ * from in-kernel, so qsim_sample can draw the same shots for a state in HBM.
+ * (new)                                                                  */
+int qsim_uniforms_splitmix(void* stream, uint64_t seed, int64_t count, void* uniforms_dev);
 
 /* ---- batched same-size states ------------------------------------------- */
 /* Same 2x2 gate on each of n_states states of 2^n amplitudes.

@@ -50,7 +50,8 @@ GOLDEN = 0x9E3779B97F4A7C15
 
 
 def splitmix_uniforms(seed, count):
-    """Host copy of the in-kernel uniforms (the HBM lane's sampler input)."""
+    """Host copy of the in-kernel uniforms (qsim_uniforms_splitmix draws the
+    same numbers on the device)."""
     with np.errstate(over="ignore"):
         k = np.arange(1, count + 1, dtype=np.uint64)
         z = np.uint64(seed) + k * np.uint64(GOLDEN)
@@ -209,12 +210,33 @@ class _LdsBatch:
 
 
 # ---------------------------------------------------------------- HBM lane
+class _PinnedPool:
+    """Reusable pinned host buffers for the block-matrix uploads: a buffer
+    goes back to the pool when the job that used it has finished on the
+    device, so no page-locking happens on the launch path."""
+
+    def __init__(self):
+        self.free = []
+
+    def acquire(self, nbytes):
+        for i, b in enumerate(self.free):
+            if b.numel() >= nbytes:
+                return self.free.pop(i)
+        return torch.empty(max(int(nbytes), 1 << 20), dtype=torch.uint8, pin_memory=True)
+
+    def release(self, buf):
+        if buf is not None:
+            self.free.append(buf)
+
+
 class _HbmJob:
-    def __init__(self, req, sim, stream):
+    def __init__(self, req, sim, stream, pool=None):
         self.req = req
         self.reqs = [req]
         self.sim = sim
         self.stream = stream
+        self.pool = pool if pool is not None else _PinnedPool()
+        self.staging = None
         self.event = None
         self.prepared = None      # future of prepare() when planned ahead
         self.blocks = None        # (block qubits int32 [B, 8], matrices complex64 [B, 32, 32])
@@ -246,16 +268,19 @@ class _HbmJob:
             call("qsim_state_init", sh, ctypes.c_void_p(self.state.data_ptr()), r.n_qubits)
             if self.blocks is not None:
                 bqs, bm = self.blocks
-                h = torch.from_numpy(bm.view(np.float32).reshape(-1)).pin_memory()
-                self.d_mats = h.to(dev, non_blocking=True)
+                raw = bm.view(np.uint8).reshape(-1)
+                self.staging = self.pool.acquire(raw.size)
+                self.staging[:raw.size].numpy()[:] = raw
+                self.d_mats = torch.empty(raw.size, dtype=torch.uint8, device=dev)
+                self.d_mats.copy_(self.staging[:raw.size], non_blocking=True)
                 call("qsim_apply_blocks", sh, ctypes.c_void_p(self.state.data_ptr()), r.n_qubits,
                      ctypes.c_void_p(self.d_mats.data_ptr()), ctypes.c_void_p(bqs.ctypes.data),
                      len(bqs))
             else:
                 cc = self.sim.compile(_circuit_of(r))
                 cc.plan(cc.resolve_mode("blocks"), dev).run(self.state)
-            u = torch.from_numpy(splitmix_uniforms(r.seed, r.shots)).pin_memory().to(dev, non_blocking=True)
-            from blackwell_kernels import sample_with_uniforms
+            from blackwell_kernels import sample_with_uniforms, uniforms_splitmix
+            u = uniforms_splitmix(r.seed, r.shots, dev)
             self.d_out = sample_with_uniforms(self.state, u)
             self.h_out = torch.empty(r.shots, dtype=torch.int64, pin_memory=True)
             self.h_out.copy_(self.d_out, non_blocking=True)
@@ -270,6 +295,8 @@ class _HbmJob:
         self.req.samples = self.h_out.numpy().copy()
         self.req.t_done = now
         self.state = self.d_mats = None
+        self.pool.release(self.staging)
+        self.staging = None
 
 
 # ---------------------------------------------------------------- server
@@ -311,7 +338,8 @@ class Server:
             self.lds_stream = torch.cuda.Stream(self.device)
             self.hbm_stream = torch.cuda.Stream(self.device)
             self.lds_factory = lambda reqs: _LdsBatch(reqs, self.device, self.lds_stream)
-            self.hbm_factory = lambda req: _HbmJob(req, self.sim, self.hbm_stream)
+            self.pinned = _PinnedPool()
+            self.hbm_factory = lambda req: _HbmJob(req, self.sim, self.hbm_stream, self.pinned)
         self.rejected = []
         self.launches = 0
 
@@ -468,7 +496,7 @@ def serve_plain(requests, sim, realtime=True):
     """The plain baseline: one circuit at a time, in arrival order, through
     the per-gate kernels (one launch per gate, matrices by value), then
     qsim_sample with the same uniforms, synchronising on each result."""
-    from blackwell_kernels import sample_with_uniforms
+    from blackwell_kernels import sample_with_uniforms, uniforms_splitmix
     reqs = sorted(requests, key=lambda r: r.arrival)
     dev = sim.device
     t0 = time.perf_counter()
@@ -486,7 +514,7 @@ def serve_plain(requests, sim, realtime=True):
         else:
             cc = sim.compile(_circuit_of(r))
             cc.plan("plain", dev).run(st)
-        u = torch.from_numpy(splitmix_uniforms(r.seed, r.shots)).to(dev)
+        u = uniforms_splitmix(r.seed, r.shots, dev)
         r.samples = sample_with_uniforms(st, u).cpu().numpy()
         r.t_done = time.perf_counter() - t0
         r.lane = "plain"

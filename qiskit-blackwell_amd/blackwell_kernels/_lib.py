@@ -59,6 +59,7 @@ PROTOTYPES = {
     "qsim_renormalize": (_i, [_vp, _vp, _i, _vp]),
     "qsim_sample_workspace_bytes": (_i64, [_i]),
     "qsim_sample": (_i, [_vp, _vp, _i, _vp, _i64, _vp, _vp]),
+    "qsim_uniforms_splitmix": (_i, [_vp, _u64, _i64, _vp]),
     "qsim_apply_gate_batched": (_i, [_vp, _vp, _i64, _i, _vp, _i]),
     "qsim_apply_gates_batched_fused": (_i, [_vp, _vp, _i64, _i, _vp, _vp, _i]),
     "qsim_run_instances": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _i]),

@@ -327,6 +327,16 @@ def sample_with_uniforms(state, uniforms):
     return out
 
 
+def uniforms_splitmix(seed, count, device="cuda"):
+    """Device float64 [count] of the seeded SplitMix64 uniforms the LDS
+    serving lane draws in-kernel (u_k from seed + (k+1) * golden gamma)."""
+    out = torch.empty(int(count), dtype=torch.float64, device=device)
+    if count:
+        call("qsim_uniforms_splitmix", stream_handle(out.device), int(seed) & 0xFFFFFFFFFFFFFFFF,
+             int(count), ptr(out))
+    return out
+
+
 def sample(state, n_shots):
     """n_shots indices from |state|^2 (cuquantum.py:283-294); uniforms from
     torch's generator on the state's device, in FP64."""

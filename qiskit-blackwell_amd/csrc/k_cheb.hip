@@ -13,6 +13,12 @@
 
 namespace qsim {
 
+// Nonzeros whose column indices, values and gathers are issued together
+// before any of their products is accumulated: the gathers of a row are
+// independent, so batching them overlaps their latencies (the summation
+// order, and so every result bit, is unchanged).
+constexpr int kGather = 8;
+
 // Neumaier step: s + c holds the running sum with the low part in c.
 __device__ __forceinline__ void neu_add(float& s, float& c, float x) {
   const float t = s + x;
@@ -42,7 +48,18 @@ __global__ __launch_bounds__(kThreads) void k_spmv(const int32_t* __restrict__ r
   if (row >= n_rows) return;
   float sr = 0.f, cr = 0.f, si = 0.f, cc = 0.f;
   const int32_t b = rp[row], e = rp[row + 1];
-  for (int32_t k = b; k < e; ++k) cmac_acc(val[k], x[ci[k]], sr, cr, si, cc);
+  int32_t k = b;
+  for (; k + kGather <= e; k += kGather) {
+    int32_t c[kGather];
+    float2 v[kGather], xs[kGather];
+#pragma unroll
+    for (int u = 0; u < kGather; ++u) { c[u] = ci[k + u]; v[u] = val[k + u]; }
+#pragma unroll
+    for (int u = 0; u < kGather; ++u) xs[u] = x[c[u]];
+#pragma unroll
+    for (int u = 0; u < kGather; ++u) cmac_acc(v[u], xs[u], sr, cr, si, cc);
+  }
+  for (; k < e; ++k) cmac_acc(val[k], x[ci[k]], sr, cr, si, cc);
   const float2 dot = make_float2(sr + cr, si + cc);
   const float2 r = cmulc(alpha, dot);
   float2 out = r;
@@ -103,8 +120,18 @@ __global__ __launch_bounds__(kThreads) void k_cheb_spmm(const int32_t* __restric
   if (row >= n_rows) return;
   float sr = 0.f, cr = 0.f, si = 0.f, cc = 0.f;
   const int32_t b = rp[row], e = rp[row + 1];
-  for (int32_t k = b; k < e; ++k)
-    cmac_acc(val[k], tc[(int64_t)ci[k] * n_rhs + j], sr, cr, si, cc);
+  int32_t k = b;
+  for (; k + kGather <= e; k += kGather) {
+    int32_t c[kGather];
+    float2 v[kGather], xs[kGather];
+#pragma unroll
+    for (int u = 0; u < kGather; ++u) { c[u] = ci[k + u]; v[u] = val[k + u]; }
+#pragma unroll
+    for (int u = 0; u < kGather; ++u) xs[u] = tc[(int64_t)c[u] * n_rhs + j];
+#pragma unroll
+    for (int u = 0; u < kGather; ++u) cmac_acc(v[u], xs[u], sr, cr, si, cc);
+  }
+  for (; k < e; ++k) cmac_acc(val[k], tc[(int64_t)ci[k] * n_rhs + j], sr, cr, si, cc);
   // w = scale * (A t_cur): the exact product errors of the scaling are kept
   float p, q;
   float wr, wi;

@@ -347,6 +347,12 @@ static SamplePlan sample_plan(int n) {
   return p;
 }
 
+__global__ __launch_bounds__(kThreads) void k_splitmix(uint64_t seed, int64_t count,
+                                                      double* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (k < count) out[k] = splitmix_u(seed, (uint64_t)k);
+}
+
 }  // namespace qsim
 
 using namespace qsim;
@@ -446,6 +452,16 @@ int qsim_sample(void* stream, const void* state, int n_qubits, const void* unifo
                      (const double*)uniforms_dev,
                      (uint64_t)n_shots, (int64_t*)out_dev);
   return launch_status("sample: search");
+}
+
+int qsim_uniforms_splitmix(void* stream, uint64_t seed, int64_t count, void* uniforms_dev) {
+  clear_error();
+  if (count < 0) return fail(QSIM_EINVAL, "count < 0");
+  if (count == 0) return QSIM_OK;
+  if (!uniforms_dev) return fail(QSIM_EINVAL, "NULL pointer argument");
+  hipLaunchKernelGGL(k_splitmix, dim3((unsigned)ceil_div((uint64_t)count, kThreads)), dim3(kThreads), 0,
+                     as_stream(stream), seed, count, (double*)uniforms_dev);
+  return launch_status("uniforms_splitmix");
 }
 
 }  // extern "C"

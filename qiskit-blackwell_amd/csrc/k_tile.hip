@@ -378,14 +378,6 @@ static int reg_fused_impl(hipStream_t s, void* states, uint64_t n_states, int n,
 // ------------------------------------------------------------ instances
 // SplitMix64 (Steele, Lea and Flood, OOPSLA 2014) as a counter-based source:
 // uniform u_k = (mix(seed + (k+1) * golden) >> 11) * 2^-53.
-__device__ __forceinline__ double splitmix_u(uint64_t seed, uint64_t k) {
-  uint64_t z = seed + (k + 1) * 0x9E3779B97F4A7C15ull;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  z ^= z >> 31;
-  return (double)(z >> 11) * 0x1.0p-53;
-}
-
 // Sampling of an LDS-resident state: thread c owns the contiguous chunk of
 // 2^n / 256 amplitudes (n >= 8) or one amplitude (n < 8, threads >= 2^n own
 // nothing).  Chunk totals are serial FP64 sums; their prefix is a serial scan

@@ -153,6 +153,18 @@ __device__ __forceinline__ I xcd_block() {
   return (b & 7) * (nb >> 3) + (b >> 3);
 }
 
+// Uniform k of a seeded SplitMix64 stream (Steele, Lea and Flood, "Fast
+// splittable pseudorandom number generators", OOPSLA 2014): the 64-bit state
+// seed + (k+1) * golden gamma through the published mix, top 53 bits scaled
+// to [0, 1).
+__device__ __forceinline__ double splitmix_u(uint64_t seed, uint64_t k) {
+  uint64_t z = seed + (k + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * 0x1.0p-53;
+}
+
 // Gate coefficients passed by value (kernel arguments live in SGPRs).
 struct M2 { float2 g[4]; };
 struct M4 { float2 g[16]; };

@@ -264,6 +264,16 @@ def test_sample_matches_oracle(bk, dev, n):
     assert pval > 1e-3
 
 
+def test_uniforms_splitmix_bit_exact(bk, dev):
+    """Device SplitMix64 uniforms equal the oracle's, bit for bit (including
+    seeds that wrap 2^64 and the empty draw)."""
+    for seed, count in ((0, 1), (12345, 4097), (2**64 - 3, 1000), (99 * 100003 + 7, 1024)):
+        got = bk.uniforms_splitmix(seed, count, dev).cpu().numpy()
+        ref = O.splitmix_uniforms(seed, count)
+        assert got.dtype == np.float64 and np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+    assert bk.uniforms_splitmix(5, 0, dev).numel() == 0
+
+
 def test_tile_program_random(bk, dev):
     from blackwell_backend import compiler as C
     from blackwell_backend.executor import tile_ops
