@@ -316,6 +316,49 @@ def c5_bench(n=22, n_rhs=32, times=(1.0, 5.0, 10.0), seed=5, precision=1e-7):
             "csr_build_s": round(build_s, 2), "results": res}
 
 
+# The reference's end-to-end table: BlackwellSimulator.run(circuit + measure_all,
+# shots=8192), median of 5 after 3 warmups, milliseconds, RTX 5090
+# (README.md:121-152).
+REFERENCE_README_MS = {
+    "GHZ": (3.8, 4.0, 4.2, 4.3, 4.9),
+    "QFT": (4.0, 4.5, 5.8, 6.2, 7.8),
+    "EfficientSU2": (4.1, 4.3, 4.8, 6.2, 27.0),
+    "DeepRz": (6.6, 9.9, 12.8, 15.9, 25.3),
+    "Toffoli": (3.8, 4.1, 10.7, 11.0, 5.4),
+    "QV": (11.2, 8.9, 15.5, 29.0, 126.0),
+}
+AB_QUBITS = (4, 8, 12, 16, 20)
+
+
+def ab_suite_bench(shots=8192, warmup=3, repeats=5):
+    """The reference's own A/B performance suite, timed the way its harness
+    times it (tests/ab_test.py:293-335): run(circuit + measure_all, 8192
+    shots), median wall clock of 5 after 3 warmups."""
+    from blackwell_backend import BlackwellSimulator, workloads as W
+    sim = BlackwellSimulator("cuda:0")
+    rows = {}
+    ratios = []
+    for name, build in W.AB_SUITE.items():
+        for i, n in enumerate(AB_QUBITS):
+            qc = build(n)
+            qc.measure_all()
+            for _ in range(warmup):
+                sim.run(qc, shots=shots)
+            ts = []
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                sim.run(qc, shots=shots)
+                ts.append(time.perf_counter() - t0)
+            ms = float(np.median(ts)) * 1e3
+            ref = REFERENCE_README_MS[name][i]
+            ratios.append(ref / ms)
+            rows[f"{name}-{n}"] = {"ms": round(ms, 3), "reference_ms": ref, "speedup": round(ref / ms, 2)}
+    return {"workload": f"run(circuit + measure_all, shots={shots}), median of {repeats} after "
+                        f"{warmup} warmups; reference_ms from its README (RTX 5090)",
+            "geomean_speedup": float(np.exp(np.mean(np.log(ratios)))),
+            "min_speedup": float(min(ratios)), "circuits": rows}
+
+
 def cpu_baseline(n, depth, seed, budget_s=15.0):
     """FP64 C/OpenMP statevector on host cores: amp-updates/s on a bounded
     prefix of the same circuit (gates applied until budget_s elapses)."""
@@ -459,6 +502,11 @@ def main():
             line["per_gate_30q"] = per_gate_sweep(30)
         except Exception as e:
             line["per_gate_30q"] = {"error": repr(e)}
+        try:
+            log("[bench] reference A/B suite (run + 8192 shots, 4..20 qubits)")
+            line["ab_suite"] = ab_suite_bench()
+        except Exception as e:
+            line["ab_suite"] = {"error": repr(e)}
         try:
             log("[bench] 32-qubit rotation + MCX ansatz, per-class GB/s and 2^20 shots")
             line["c3_rotation_mcx_32q"] = c3_bench()

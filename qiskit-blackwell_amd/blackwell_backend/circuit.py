@@ -23,22 +23,40 @@ class Bit:
         return f"{self.kind}({self.index})"
 
 
-class Operation:
+class _Frozen:
+    """Attributes are set once in __init__; array parameters are stored as
+    read-only copies.  Instructions never change after they are built, so a
+    circuit's content is identified by the identities of its instructions
+    (see fingerprint)."""
+    __slots__ = ()
+
+    def __setattr__(self, name, value):
+        raise AttributeError(f"{type(self).__name__} is immutable")
+
+
+def _frozen_param(p):
+    if isinstance(p, np.ndarray):
+        p = np.array(p, copy=True)
+        p.setflags(write=False)
+    return p
+
+
+class Operation(_Frozen):
     __slots__ = ("name", "params", "num_qubits")
 
     def __init__(self, name, params=(), num_qubits=None):
-        self.name = name
-        self.params = list(params)
-        self.num_qubits = num_qubits
+        object.__setattr__(self, "name", name)
+        object.__setattr__(self, "params", tuple(_frozen_param(p) for p in params))
+        object.__setattr__(self, "num_qubits", num_qubits)
 
 
-class CircuitInstruction:
+class CircuitInstruction(_Frozen):
     __slots__ = ("operation", "qubits", "clbits")
 
     def __init__(self, operation, qubits, clbits=()):
-        self.operation = operation
-        self.qubits = tuple(qubits)
-        self.clbits = tuple(clbits)
+        object.__setattr__(self, "operation", operation)
+        object.__setattr__(self, "qubits", tuple(qubits))
+        object.__setattr__(self, "clbits", tuple(clbits))
 
 
 _ids = itertools.count()
@@ -157,6 +175,16 @@ class QuantumCircuit:
 
     def __repr__(self):
         return f"QuantumCircuit({self.num_qubits} qubits, {len(self.data)} instructions, name={self.name!r})"
+
+
+def fingerprint(circuit):
+    """Cheap identity of the content of one of this module's circuits
+    (instructions are immutable, so their identities and order determine the
+    content); None for circuits of other classes."""
+    if not isinstance(circuit, QuantumCircuit):
+        return None
+    return (circuit.num_qubits, circuit.num_clbits, len(circuit.data),
+            hash(tuple(map(id, circuit.data))))
 
 
 def instructions(circuit):

@@ -40,14 +40,29 @@ class Result:
         return self.results[experiment].data.to_dict()
 
 
+_QISKIT_RESULT = []
+
+
+def _qiskit_result_classes():
+    """(Result, ExperimentResult, ExperimentResultData) from qiskit, or None
+    when qiskit is not installed (probed once per process)."""
+    if not _QISKIT_RESULT:
+        try:
+            from qiskit.result import Result as QResult
+            from qiskit.result.models import ExperimentResult as QExp, ExperimentResultData as QData
+            _QISKIT_RESULT.append((QResult, QExp, QData))
+        except ImportError:
+            _QISKIT_RESULT.append(None)
+    return _QISKIT_RESULT[0]
+
+
 def make_result(counts, shots, header, time_taken, metadata=None):
-    try:
-        from qiskit.result import Result as QResult
-        from qiskit.result.models import ExperimentResult as QExp, ExperimentResultData as QData
+    q = _qiskit_result_classes()
+    if q is not None:
+        QResult, QExp, QData = q
         kw = {"metadata": metadata} if metadata is not None else {}
         exp = QExp(shots=shots, success=True, data=QData(counts=counts), header=header, **kw)
         return QResult(backend_name="blackwell_simulator", backend_version="0.1.0", qobj_id="",
                        job_id="", success=True, results=[exp], time_taken=time_taken)
-    except ImportError:
-        exp = ExperimentResult(shots, True, ExperimentResultData(counts), header, metadata)
-        return Result("blackwell_simulator", "0.1.0", "", "", True, [exp], time_taken)
+    exp = ExperimentResult(shots, True, ExperimentResultData(counts), header, metadata)
+    return Result("blackwell_simulator", "0.1.0", "", "", True, [exp], time_taken)

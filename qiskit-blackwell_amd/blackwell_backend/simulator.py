@@ -21,6 +21,7 @@ Differences by design:
 """
 import importlib
 import os
+import weakref
 import sys
 import time
 
@@ -28,7 +29,7 @@ import numpy as np
 import torch
 
 from . import compiler as C
-from .circuit import instructions
+from .circuit import fingerprint, instructions
 from .executor import CompiledCircuit
 from .result import make_result
 
@@ -57,8 +58,27 @@ def _load_kernels(kernel_path=None):
 def counts_from_samples(samples):
     """{hex(index): count} over all qubits (simulator.py:580-586)."""
     s = np.asarray(samples, dtype=np.int64)
-    vals, cnt = np.unique(s, return_counts=True)
-    return {hex(int(v)): int(c) for v, c in zip(vals, cnt)}
+    if s.size and int(s.max()) < (1 << 24):
+        bc = np.bincount(s)
+        vals = np.flatnonzero(bc)
+        cnt = bc[vals]
+    else:
+        vals, cnt = np.unique(s, return_counts=True)
+    return dict(zip(map(hex, vals.tolist()), cnt.tolist()))
+
+
+_QISKIT_QC = []
+
+
+def _qiskit_circuit_class():
+    """qiskit.QuantumCircuit, or None when qiskit is not installed (probed once)."""
+    if not _QISKIT_QC:
+        try:
+            from qiskit import QuantumCircuit as QC
+            _QISKIT_QC.append(QC)
+        except ImportError:
+            _QISKIT_QC.append(None)
+    return _QISKIT_QC[0]
 
 
 class BlackwellSimulator:
@@ -84,6 +104,8 @@ class BlackwellSimulator:
         self._native_mcx = {"ccx"}
         self._native_other = {"measure", "barrier", "reset"}
         self._compile_cache = {}
+        # per circuit object: (fingerprint, cache key, measurement-free copy)
+        self._memo = weakref.WeakKeyDictionary()
         self._pm = None
 
     # ---------------------------------------------------------------- compile
@@ -110,9 +132,31 @@ class BlackwellSimulator:
             parts.append((name, tuple(pv), tuple(qs), tuple(cs)))
         return (circuit.num_qubits, circuit.num_clbits, hash(tuple(parts)))
 
+    def _memo_entry(self, circuit):
+        """Memo of one of this module's circuits while its content is unchanged."""
+        fp = fingerprint(circuit)
+        if fp is None:
+            return None
+        try:
+            ent = self._memo.get(circuit)
+        except TypeError:
+            return None
+        if ent is None or ent[0] != fp:
+            ent = [fp, None, None]
+            self._memo[circuit] = ent
+        return ent
+
+    def _key(self, circuit):
+        ent = self._memo_entry(circuit)
+        if ent is None:
+            return self._circuit_cache_key(circuit)
+        if ent[1] is None:
+            ent[1] = self._circuit_cache_key(circuit)
+        return ent[1]
+
     def compile(self, circuit):
         """CompiledCircuit for `circuit` (memoised like simulator.py:517-527)."""
-        key = self._circuit_cache_key(circuit)
+        key = self._key(circuit)
         cc = self._compile_cache.get(key)
         if cc is not None:
             return cc
@@ -146,21 +190,22 @@ class BlackwellSimulator:
         torch.cuda.synchronize(self.device)
         return state, clbits, time.perf_counter() - t0
 
-    @staticmethod
-    def _strip_measurements(circuit):
+    def _strip_measurements(self, circuit):
         """Copy of `circuit` without measure/reset (simulator.py:546-553)."""
-        try:
-            from qiskit import QuantumCircuit as QC
-            if type(circuit).__module__.startswith("qiskit"):
-                new = QC(circuit.num_qubits, circuit.num_clbits, name=circuit.name)
-                for ins in circuit.data:
-                    if ins.operation.name.lower() not in ("measure", "reset"):
-                        new.append(ins.operation, ins.qubits, ins.clbits)
-                return new
-        except ImportError:
-            pass
+        ent = self._memo_entry(circuit)
+        if ent is not None and ent[2] is not None:
+            return ent[2]
+        QC = _qiskit_circuit_class()
+        if QC is not None and type(circuit).__module__.startswith("qiskit"):
+            new = QC(circuit.num_qubits, circuit.num_clbits, name=circuit.name)
+            for ins in circuit.data:
+                if ins.operation.name.lower() not in ("measure", "reset"):
+                    new.append(ins.operation, ins.qubits, ins.clbits)
+            return new
         new = circuit.copy()
         new.data = [ins for ins in new.data if ins.operation.name.lower() not in ("measure", "reset")]
+        if ent is not None:
+            ent[2] = new
         return new
 
     def run(self, circuit, shots=1024, seed=None, mode=None):

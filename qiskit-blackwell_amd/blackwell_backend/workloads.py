@@ -157,3 +157,95 @@ def heisenberg_chain(n=22, seed=5):
     for i in range(n):
         terms.append((lab([(i, "X")]), h[i]))
     return SparsePauliOp.from_list(terms)
+
+
+# ------------------------------------------------ the reference's A/B circuit suite
+# The circuit families of the reference's performance table (tests/ab_test.py:43-118,
+# README.md:121-152), built without qiskit's circuit library: the same gate
+# structure and counts; angles and Haar draws come from this module's seeded
+# generators.
+
+def ghz(n):
+    """H on qubit 0, then CNOT q -> q+1 (ab_test.py:43-49)."""
+    qc = QuantumCircuit(n, name=f"ghz_{n}")
+    qc.h(0)
+    for i in range(n - 1):
+        qc.cx(i, i + 1)
+    return qc
+
+
+def qft(n):
+    """Textbook QFT as H, controlled phases and the final swaps: the gates of
+    qiskit's QFT(n).decompose() used at ab_test.py:52-55 (n(n+1)/2 + n//2)."""
+    qc = QuantumCircuit(n, name=f"qft_{n}")
+    for j in reversed(range(n)):
+        qc.h(j)
+        for k in reversed(range(j)):
+            qc.cp(math.pi / (1 << (j - k)), j, k)
+    for i in range(n // 2):
+        qc.swap(i, n - 1 - i)
+    return qc
+
+
+def efficient_su2(n, reps=2, seed=42):
+    """RY and RZ layers on every qubit around `reps` reverse-linear CX
+    entanglers, angles U[0, 2pi) in parameter order (ab_test.py:74-79)."""
+    rng = np.random.default_rng(seed)
+    theta = rng.uniform(0, 2 * math.pi, 2 * n * (reps + 1))
+    qc = QuantumCircuit(n, name=f"su2_{n}_r{reps}")
+    k = 0
+    for layer in range(reps + 1):
+        for q in range(n):
+            qc.ry(theta[k + q], q)
+        for q in range(n):
+            qc.rz(theta[k + n + q], q)
+        k += 2 * n
+        if layer < reps:
+            for q in reversed(range(n - 1)):
+                qc.cx(q, q + 1)
+    return qc
+
+
+def deep_rz(n, depth=50, seed=42):
+    """Rz on every qubit, then CX on even and odd neighbour pairs, `depth`
+    times (ab_test.py:105-118)."""
+    rng = np.random.default_rng(seed)
+    qc = QuantumCircuit(n, name=f"deep_rz_{n}_d{depth}")
+    for _ in range(depth):
+        for q in range(n):
+            qc.rz(rng.uniform(0, 2 * math.pi), q)
+        for q in range(0, n - 1, 2):
+            qc.cx(q, q + 1)
+        for q in range(1, n - 1, 2):
+            qc.cx(q, q + 1)
+    return qc
+
+
+def toffoli_chain(n):
+    """H on every qubit, then CCX(i, i+1, i+2) (ab_test.py:91-102)."""
+    if n < 3:
+        raise ValueError("a Toffoli chain needs at least 3 qubits")
+    qc = QuantumCircuit(n, name=f"toffoli_{n}")
+    for i in range(n):
+        qc.h(i)
+    for i in range(n - 2):
+        qc.ccx(i, i + 1, i + 2)
+    return qc
+
+
+def quantum_volume(n, seed=42):
+    """Depth-n square circuit of Haar SU(4) on seeded random pairs, the form
+    of QuantumVolume(n, seed) at ab_test.py:82-88."""
+    qc = su4_volume(n, n, seed)
+    qc.name = f"qv_{n}_d{n}"
+    return qc
+
+
+AB_SUITE = {
+    "GHZ": ghz,
+    "QFT": qft,
+    "EfficientSU2": efficient_su2,
+    "DeepRz": deep_rz,
+    "Toffoli": toffoli_chain,
+    "QV": quantum_volume,
+}

@@ -185,6 +185,60 @@ __device__ void apply_op(float2* sh, int m, const qsim_op& op, const float2* __r
   }
 }
 
+// Coefficients an op reads from the table: up to kStaged of them are staged
+// in LDS one op ahead; -1 means more than that (read from the table in place).
+constexpr int kStaged = 16;
+__device__ __forceinline__ int op_ncoef(const qsim_op& op) {
+  switch (op.kind) {
+    case QSIM_OP_M1: return 4;
+    case QSIM_OP_M2: return 16;
+    case QSIM_OP_D1: return 2;
+    case QSIM_OP_D2: return 4;
+    case QSIM_OP_DK: return op.nq <= 4 ? (1 << op.nq) : -1;
+    case QSIM_OP_PH: return 1;
+    case QSIM_OP_MK: return op.nq <= 2 ? (1 << (2 * op.nq)) : -1;
+    default: return 0;
+  }
+}
+
+// Runs ops get(0..n_ops-1) on the LDS state with a two-stage software
+// pipeline: the descriptor of op k+2 and the coefficients of op k+1 are
+// fetched from global memory while op k runs, so an op costs its LDS work
+// instead of two dependent global-memory latencies.  Op k+1's coefficients
+// land in the LDS slot that op k-1 used, before the barrier that ends op k.
+template <class GetOp>
+__device__ __forceinline__ void run_ops(float2* sh, int m, int n_ops, GetOp get,
+                                        const float2* __restrict__ cf) {
+  __shared__ float2 cslot[2][kStaged];
+  if (n_ops <= 0) return;
+  const int t = threadIdx.x;
+  qsim_op cur = get(0);
+  {
+    const int nc = op_ncoef(cur);
+    if (t < nc) cslot[0][t] = cf[cur.coef + t];
+  }
+  qsim_op nxt = n_ops > 1 ? get(1) : cur;
+  __syncthreads();
+  for (int k = 0; k < n_ops; ++k) {
+    qsim_op nn = nxt;
+    if (k + 2 < n_ops) nn = get(k + 2);
+    const int ncn = k + 1 < n_ops ? op_ncoef(nxt) : 0;
+    float2 pre = make_float2(0.f, 0.f);
+    if (t < ncn) pre = cf[nxt.coef + t];
+    if (op_ncoef(cur) >= 0) {
+      qsim_op o = cur;
+      o.coef = 0;
+      apply_op(sh, m, o, cslot[k & 1]);
+    } else {
+      apply_op(sh, m, cur, cf);
+    }
+    if (t < ncn) cslot[(k + 1) & 1][t] = pre;
+    __syncthreads();
+    cur = nxt;
+    nxt = nn;
+  }
+}
+
 // ------------------------------------------------------------ tile geometry
 struct TileGeom {
   int n, m, L;          // state qubits, tile bits, contiguous low bits
@@ -236,11 +290,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_program(float4* __restric
   const uint64_t base = tile_base(blockIdx.x, g);
   tile_load(sh, st, base, g);
   __syncthreads();
-  for (int k = 0; k < n_ops; ++k) {
-    const qsim_op op = ops[k];
-    apply_op(sh, g.m, op, cf);
-    __syncthreads();
-  }
+  run_ops(sh, g.m, n_ops, [&](int k) { return ops[k]; }, cf);
   tile_store(sh, st, base, g);
 }
 
@@ -257,16 +307,15 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_fused1q(float4* __restric
   const uint64_t base = tile_base(blockIdx.x, g);
   tile_load(sh, st, base, g);
   __syncthreads();
-  for (int k = 0; k < f.count; ++k) {
+  run_ops(sh, g.m, f.count, [&](int k) {
     qsim_op op;
     op.kind = QSIM_OP_M1;
     op.nq = 1;
     op.q[0] = f.q[k];
     op.cmask = 0;
     op.coef = (uint32_t)(4 * (f.first + k));
-    apply_op(sh, g.m, op, gates);
-    __syncthreads();
-  }
+    return op;
+  }, gates);
   tile_store(sh, st, base, g);
 }
 
@@ -447,11 +496,8 @@ __global__ __launch_bounds__(kTileThreads) void k_instances(float2* __restrict__
     for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) sh[i] = gst[i];
   }
   __syncthreads();
-  for (uint32_t k = 0; k < in.n_ops; ++k) {
-    const qsim_op op = ops[in.op_off + k];
-    apply_op(sh, n, op, cf);
-    __syncthreads();
-  }
+  const qsim_op* iops = ops + in.op_off;
+  run_ops(sh, n, (int)in.n_ops, [&](int k) { return iops[k]; }, cf);
   if (in.want & 1)
     for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) gst[i] = sh[i];
   if (in.shots) lds_sample(sh, n, in.seed, in.shots, shots_out + in.shot_off, pre);

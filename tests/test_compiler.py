@@ -203,3 +203,52 @@ def test_workloads_shapes_and_determinism():
     assert abs(np.mean(ns == 10) - 0.25 / (1 - 0.75 ** 15)) < 0.02
     spo = W.heisenberg_chain(22)
     assert spo.num_qubits == 22 and len(spo.labels) == 3 * 21 + 22
+
+
+def test_ab_suite_gate_counts_and_qft():
+    """The A/B suite circuits have the gate counts of the reference's table
+    (README.md:121-152), and qft(n) maps |x> to the DFT column."""
+    from blackwell_backend import workloads as W
+    from blackwell_backend.circuit import instructions
+    counts = {"GHZ": [4, 8, 12, 16, 20], "QFT": [12, 40, 84, 144, 220],
+              "EfficientSU2": [30, 62, 94, 126, 158], "DeepRz": [350, 750, 1150, 1550, 1950],
+              "Toffoli": [6, 14, 22, 30, 38], "QV": [8, 32, 72, 128, 200]}
+    for name, want in counts.items():
+        assert [len(W.AB_SUITE[name](n).data) for n in (4, 8, 12, 16, 20)] == want, name
+    n = 5
+    N = 1 << n
+    for x in (0, 3, 17):
+        qc = W.qft(n)
+        psi0 = np.zeros(N, dtype=np.complex128)
+        psi0[x] = 1
+        ins = [(nm, q, p) for nm, q, _, p in instructions(qc)]
+        psi = O.run_instructions(n, ins, psi0)
+        ref = np.exp(2j * np.pi * x * np.arange(N) / N) / np.sqrt(N)
+        assert np.allclose(psi, ref, atol=1e-12)
+
+
+def test_circuit_fingerprint_tracks_content():
+    """The per-circuit memo keys on immutable instructions: appending or
+    replacing an instruction changes the fingerprint, and instructions and
+    array parameters cannot be changed in place."""
+    from blackwell_backend.circuit import QuantumCircuit, fingerprint, CircuitInstruction
+    qc = QuantumCircuit(3)
+    qc.h(0).cx(0, 1)
+    u = np.eye(4, dtype=np.complex128)
+    qc.unitary(u, [1, 2])
+    f0 = fingerprint(qc)
+    assert fingerprint(qc) == f0
+    qc.rz(0.3, 2)
+    f1 = fingerprint(qc)
+    assert f1 != f0
+    qc.data[1] = CircuitInstruction(qc.data[1].operation, qc.data[1].qubits)
+    assert fingerprint(qc) != f1
+    with pytest.raises(AttributeError):
+        qc.data[0].operation.params = (1.0,)
+    with pytest.raises(AttributeError):
+        qc.data[0].qubits = ()
+    with pytest.raises(ValueError):
+        qc.data[2].operation.params[0][0, 0] = 2.0     # read-only copy
+    u[0, 0] = 5.0                                       # the caller's array is not aliased
+    assert qc.data[2].operation.params[0][0, 0] == 1.0
+    assert fingerprint(object()) is None
