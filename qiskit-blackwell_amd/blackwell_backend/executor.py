@@ -23,6 +23,12 @@ from blackwell_kernels._lib import call, OP_DTYPE
 from . import compiler as C
 
 WHOLE_TILE_MAX = 13     # n <= 13: the whole state is one LDS tile (64 KiB)
+BLOCK_MIN_QUBITS = 10   # the MFMA block kernel needs 2^10 amplitudes
+# auto-mode cost model for 10 <= n <= 13 (MI355X measurements): one op of the
+# one-workgroup LDS program costs ~2 us at 12 qubits and scales with the
+# amplitudes per thread; one fused block costs ~5 us (launch-bound at this size)
+WHOLE_OP_US_12Q = 2.0
+BLOCK_US = 5.0
 
 
 def _d8(m):
@@ -97,7 +103,14 @@ class CompiledCircuit:
         self.n_clbits = n_clbits
         self.ops = ops
         self._plans = {}
+        self._blocks = None
         self.has_stochastic = any(o.kind in ("measure", "reset") for o in ops)
+
+    def block_entries(self):
+        """plan_blocks of the ops (computed once per compiled circuit)."""
+        if self._blocks is None:
+            self._blocks = C.plan_blocks(self.ops, self.n_qubits, k=5)
+        return self._blocks
 
     # ---------------------------------------------------------------- plans
     def plan(self, mode, device):
@@ -109,9 +122,19 @@ class CompiledCircuit:
         return p
 
     def resolve_mode(self, mode):
+        """auto: the whole state as one LDS program below 10 qubits, MFMA
+        blocks above 13, and between the two whichever the cost model says
+        is cheaper (deep circuits favour blocks: one block absorbs many ops)."""
         if mode != "auto":
             return mode
-        return "whole" if self.n_qubits <= WHOLE_TILE_MAX else "blocks"
+        n = self.n_qubits
+        if n < BLOCK_MIN_QUBITS:
+            return "whole"
+        if n > WHOLE_TILE_MAX:
+            return "blocks"
+        t_whole = len(self.ops) * WHOLE_OP_US_12Q * 2.0 ** (n - 12)
+        t_blocks = len(self.block_entries()) * BLOCK_US
+        return "blocks" if t_blocks < t_whole else "whole"
 
     def _build(self, mode, device):
         n = self.n_qubits
@@ -128,7 +151,7 @@ class CompiledCircuit:
         if mode == "blocks":
             if n < 10:
                 return self._build_tiles(C.plan_tiles(self.ops, n, m=n), device)
-            return self._build_blocks(C.plan_blocks(self.ops, n, k=5), device)
+            return self._build_blocks(self.block_entries(), device)
         raise ValueError(f"unknown execution mode {mode!r}")
 
     def _build_plain(self, ops, device):

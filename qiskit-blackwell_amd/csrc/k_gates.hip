@@ -568,6 +568,141 @@ __global__ __launch_bounds__(kThreads) void k_mc4(float4* __restrict__ st, I n_i
   }
 }
 
+// Controlled 1-target gates (X and dense 2x2) in the lane-contiguous layout
+// of k_gate1q: one wave owns a window of 64 float4 slots (128 amplitudes,
+// index bits 0..6), or a pair of windows 2^(t-7) apart when t >= 7.
+//   * controls >= 7 choose the windows: the window index gets zeros inserted
+//     at those bits (and at the target in MODE 2) and then has them set, so
+//     only windows with every high control set are visited;
+//   * controls 1..6 are slot bits inside the window.  On slot bits 3..5 a
+//     control switches whole 128 B lines on or off, so only active lanes load
+//     and store.  On slot bits 0..2 (FULL) it splits every line, and the wave
+//     rewrites whole lines instead (inactive lanes store what they loaded):
+//     partial-line traffic costs more than the untouched half;
+//   * a control on qubit 0 restricts the update to the odd amplitude (.zw)
+//     of each slot.
+struct Mc1Args {
+  int t;
+  int nh;                              // window-level zero insertions
+  int hpos[QSIM_MAX_CONTROLS + 1];     // ascending, in window-index bits (qubit - 7)
+  uint64_t hset;                       // window-index bits of the high controls
+  uint32_t lmask;                      // slot bits of the controls 1..6
+  int c0;                              // control on qubit 0
+};
+
+template <typename I>
+__device__ __forceinline__ I mc1_window(I item, const Mc1Args& a) {
+  I w = item;
+#pragma unroll
+  for (int j = 0; j < QSIM_MAX_CONTROLS + 1; ++j)
+    if (j < a.nh) w = ins0(w, a.hpos[j]);
+  return w | (I)a.hset;
+}
+
+__device__ __forceinline__ float4 pick4(bool c, float4 a, float4 b) { return c ? a : b; }
+
+// OP 0: dense 2x2 (compensated), OP 1: X.
+template <typename I, int MODE, int OP, int FULL>
+__global__ __launch_bounds__(kThreads) void k_mc1(float4* __restrict__ st, I n_items, Mc1Args a,
+                                                  const float2* __restrict__ gdev, M2 gv) {
+  float2 g[4];
+  if (OP == 0) load_m2(gdev, gv, g);
+  const I item = xcd_block<I>() * (kThreads / 64) + (threadIdx.x >> 6);
+  if (item >= n_items) return;      // uniform per wave
+  const int lane = threadIdx.x & 63;
+  const I w = mc1_window(item, a);
+  const I s0 = (w << 6) | (I)lane;
+  const bool act = ((uint32_t)lane & a.lmask) == a.lmask;
+  if (!FULL && !act && MODE != 1) return;
+  if (MODE == 0) {
+    const float4 v = ld_nt(st + s0);
+    float4 r;
+    if (OP == 1) r = cat4(hi2(v), lo2(v));
+    else r = cat4(crow2(g[0], lo2(v), g[1], hi2(v)), crow2(g[2], lo2(v), g[3], hi2(v)));
+    st_nt(st + s0, FULL ? pick4(act, r, v) : r);
+  } else if (MODE == 1) {
+    // the partner lane has the same control bits, so it is active with this one
+    const bool ld = FULL || act;
+    const float4 v = ld ? ld_nt(st + s0) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int m = 1 << (a.t - 1);
+    const float4 p = shfl_xor4(v, m);
+    if (!ld) return;
+    float4 r;
+    if (OP == 1) {
+      r = p;
+    } else {
+      const bool low = (lane & m) == 0;
+      const float2 ga = low ? g[0] : g[3], gb = low ? g[1] : g[2];
+      r = cat4(crow2(ga, lo2(v), gb, lo2(p)), crow2(ga, hi2(v), gb, hi2(p)));
+    }
+    if (a.c0) r = cat4(lo2(v), hi2(r));
+    st_nt(st + s0, FULL ? pick4(act, r, v) : r);
+  } else {
+    const I s1 = s0 + (I(1) << (a.t - 1));
+    const float4 v0 = ld_nt(st + s0), v1 = ld_nt(st + s1);
+    float4 r0, r1;
+    if (OP == 1) {
+      r0 = v1; r1 = v0;
+    } else {
+      r0 = cat4(crow2(g[0], lo2(v0), g[1], lo2(v1)), crow2(g[0], hi2(v0), g[1], hi2(v1)));
+      r1 = cat4(crow2(g[2], lo2(v0), g[3], lo2(v1)), crow2(g[2], hi2(v0), g[3], hi2(v1)));
+    }
+    if (a.c0) { r0 = cat4(lo2(v0), hi2(r0)); r1 = cat4(lo2(v1), hi2(r1)); }
+    st_nt(st + s0, FULL ? pick4(act, r0, v0) : r0);
+    st_nt(st + s1, FULL ? pick4(act, r1, v1) : r1);
+  }
+}
+
+template <typename I, int MODE, int OP>
+static void launch_mc1(hipStream_t s, void* state, uint64_t items, const Mc1Args& a, bool full,
+                       const float2* gdev, const M2& gv) {
+  const dim3 grid(ceil_div(items, kThreads / 64)), block(kThreads);
+  if (full)
+    hipLaunchKernelGGL((k_mc1<I, MODE, OP, 1>), grid, block, 0, s, (float4*)state, (I)items, a, gdev, gv);
+  else
+    hipLaunchKernelGGL((k_mc1<I, MODE, OP, 0>), grid, block, 0, s, (float4*)state, (I)items, a, gdev, gv);
+}
+
+template <typename I, int OP>
+static void launch_mc1_mode(hipStream_t s, void* state, uint64_t items, const Mc1Args& a, int mode,
+                            bool full, const float2* gdev, const M2& gv) {
+  if (mode == 0) launch_mc1<I, 0, OP>(s, state, items, a, full, gdev, gv);
+  if (mode == 1) launch_mc1<I, 1, OP>(s, state, items, a, full, gdev, gv);
+  if (mode == 2) launch_mc1<I, 2, OP>(s, state, items, a, full, gdev, gv);
+}
+
+// Returns false when the state is too small for 64-slot windows.
+static bool mc1_launch(hipStream_t s, void* state, int n, const McSpec& spec, int t, int op,
+                       const float2* gdev) {
+  Mc1Args a;
+  memset(&a, 0, sizeof a);
+  a.t = t;
+  const int mode = t == 0 ? 0 : (t < 7 ? 1 : 2);
+  for (int j = 0; j < spec.n_pos; ++j) {
+    const int q = spec.pos[j];
+    if (q == t) {
+      if (mode == 2) a.hpos[a.nh++] = q - 7;
+      continue;
+    }
+    if (q == 0) a.c0 = 1;
+    else if (q < 7) a.lmask |= 1u << (q - 1);
+    else { a.hpos[a.nh++] = q - 7; a.hset |= 1ull << (q - 7); }
+  }
+  if (n < 8 || n - 7 - a.nh < 0) return false;
+  const uint64_t items = 1ull << (n - 7 - a.nh);
+  const bool full = (a.lmask & 7u) != 0;
+  const bool wide = (1ull << (n - 1)) > 0xffffffffull;
+  M2 gv = {};
+  if (op == 0) {
+    if (wide) launch_mc1_mode<uint64_t, 0>(s, state, items, a, mode, full, gdev, gv);
+    else launch_mc1_mode<uint32_t, 0>(s, state, items, a, mode, full, gdev, gv);
+  } else {
+    if (wide) launch_mc1_mode<uint64_t, 1>(s, state, items, a, mode, full, gdev, gv);
+    else launch_mc1_mode<uint32_t, 1>(s, state, items, a, mode, full, gdev, gv);
+  }
+  return true;
+}
+
 constexpr int kUm = 2;
 
 static int mc_impl(void* stream, void* state, int n, const float2* gdev, const int64_t* controls,
@@ -593,6 +728,7 @@ static int mc_impl(void* stream, void* state, int n, const float2* gdev, const i
   a.t1 = tg[1];
   a.op = op;
   hipStream_t s = as_stream(stream);
+  if (op < 2 && mc1_launch(s, state, n, spec, tg[0], op, gdev)) return launch_status("apply_mc");
   const bool wide = n > 32;
   if (spec.pos[0] >= 1 && n - spec.n_pos >= 1) {
     const uint64_t items = 1ull << (n - spec.n_pos - 1);

@@ -149,6 +149,33 @@ def test_mcx_seven_controls_wide(bk, dev):
         assert np.array_equal(_bits(st.cpu().numpy()), _bits(psi[O.mcx_perm(n, ctl, t)]))
 
 
+@pytest.mark.parametrize("n", [8, 9, 11])
+def test_controlled_1q_all_layouts(bk, dev, n):
+    """Controlled X (bit-exact) and controlled dense 2x2 for targets in the
+    float4, cross-lane and stream layouts, with controls on qubit 0, on
+    lane bits (1..6) and on window bits (>= 7), against the oracle."""
+    rng = np.random.default_rng(100 + n)
+    psi = rand_state(n, 9 + n)
+    u = rand_unitary(2, 3 + n).astype(np.complex64)
+    ud = torch.from_numpy(u).to(dev)
+    cases = [([], 0), ([], 5), ([], n - 1), ([0], 7), ([0], 3), ([3], 0), ([2], n - 1),
+             ([n - 1], 2), ([n - 1], 0), ([0, n - 1], 4), ([1, 2, 3], 7), ([4, 5], 6),
+             ([7], 6), ([0, 6, 7], 1)]
+    for _ in range(12):
+        k = int(rng.integers(0, 4))
+        qs = [int(q) for q in rng.choice(n, k + 1, replace=False)]
+        cases.append((qs[:-1], qs[-1]))
+    for ctl, t in cases:
+        print(f"[mc1] n={n} controls={ctl} target={t}", flush=True)
+        st = to_dev(psi, dev)
+        bk.apply_mcx(st, ctl, t)
+        assert np.array_equal(_bits(st.cpu().numpy()), _bits(psi[O.mcx_perm(n, ctl, t)])), (ctl, t)
+        st = to_dev(psi, dev)
+        bk.apply_mcgate(st, ud, ctl, t)
+        ref = O.apply_mc(psi.astype(np.complex128), u.astype(np.complex128), ctl, [t])
+        assert _rel_err(ref, st.cpu().numpy()) < 1e-6, (ctl, t)
+
+
 def test_mcgate_and_mc2qgate(bk, dev):
     n = 7
     for ctl, t in [([1], 0), ([0], 3), ([2, 5], 6), ([6, 0, 3], 1), ([], 4)]:
