@@ -32,6 +32,11 @@ import numpy as np  # noqa: E402
 import torch  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per amplitude update that k_gate2q moves, measured with two
+# rocprofv3 --pmc runs of this script (FETCH_SIZE, then WRITE_SIZE, both scaled
+# by the counts of a 2 GiB device-to-device copy run first with
+# --pmc-calibration): one read and one write of the state, nothing more.
+GATE2Q_TRAFFIC_B_PER_AMP = 16.0
 MFMA_FP32_PEAK_TFLOPS = 157.3
 
 
@@ -434,6 +439,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--qubits", type=int, default=28)
     ap.add_argument("--no-extras", action="store_true", help="skip the 30q sweep and CPU baseline")
+    ap.add_argument("--pmc-calibration", action="store_true",
+                    help="first run one plain device copy of a 2^qubits complex64 vector (a kernel "
+                         "of known traffic, for scaling HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -447,6 +455,13 @@ def main():
     stream = torch.cuda.current_stream(sim.device)
     qc = W.su4_volume(n, depth, seed)
     n_gates = (n // 2) * depth
+    if args.pmc_calibration:
+        src = torch.ones(1 << n, dtype=torch.complex64, device=sim.device)
+        dst = torch.empty_like(src)
+        dst.copy_(src)                       # reads and writes 8 * 2^n bytes
+        torch.cuda.synchronize()
+        del src, dst
+        log(f"[bench] calibration copy of {(8 << n) >> 20} MiB done")
     log(f"[bench] compile + warmup plain ({n} qubits, {n_gates} gates)")
     _, total, wall, kern, launches = run_steps(sim, qc, "plain", args.steps, args.warmup, stream)
     ms_step = total / args.steps * 1e3
@@ -481,7 +496,10 @@ def main():
                                f"{n_gates} gates, complex64 state {(8 << n) >> 30} GiB, plain per-gate",
                    "qubits": n, "depth": depth, "gates": n_gates, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": GATE2Q_TRAFFIC_B_PER_AMP * (1 << n),
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per k_gate2q dispatch, "
+                                       "scaled by a 2 GiB device copy (--pmc-calibration)",
                      "kernel": "k_gate2q (dense 4x4, 16 B/amp-update)"},
         "fused": {"value": fused_value, "unit": "gate-equivalent amp-updates/s",
                   "ms_per_step": ftotal / args.steps * 1e3, "blocks": flaunch,
