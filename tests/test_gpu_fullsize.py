@@ -1,0 +1,126 @@
+"""Parity at the benchmark workloads' own sizes (28-qubit SU(4), 32-qubit
+rotation + MCX ansatz), through properties that need no FP64 statevector of
+that size:
+  * a circuit followed by its inverse returns |0...0>: |<0|psi>|^2 >= 1 - 1e-6
+    and the FP64 norm stays within 1e-6 of 1;
+  * a layer of permutation gates (CNOT ladder, MCX with controls on window,
+    lane and float4 bits) followed by its reverse restores a random state
+    bit for bit;
+  * the FP64 leaf sampler at 32 qubits returns only indices whose amplitude
+    is nonzero and reproduces them for the same uniforms.
+The tests keep every state on the device (no 2^32-element host copies)."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sim():
+    from blackwell_backend import BlackwellSimulator
+    return BlackwellSimulator("cuda:0")
+
+
+def _inverse_appended(qc):
+    """qc followed by its inverse (reverse order, each gate inverted)."""
+    from blackwell_backend.circuit import QuantumCircuit, instructions
+    out = QuantumCircuit(qc.num_qubits, name=qc.name + "_and_inverse")
+    ins = instructions(qc)
+    for stage in (ins, list(reversed(ins))):
+        inverse = stage is not ins
+        for name, qs, _, params in stage:
+            if name == "unitary":
+                u = np.asarray(params[0])
+                out.unitary(u.conj().T if inverse else u, qs)
+            elif name in ("rx", "ry", "rz"):
+                getattr(out, name)(-params[0] if inverse else params[0], qs[0])
+            elif name == "cx":
+                out.cx(qs[0], qs[1])
+            elif name == "mcx":
+                out.mcx(qs[:-1], qs[-1])
+            else:
+                raise AssertionError(f"no inverse rule for {name}")
+    return out
+
+
+def _run_on_device(sim, qc, mode):
+    import blackwell_kernels as bk
+    cc = sim.compile(qc)
+    plan = cc.plan(mode, sim.device)
+    st = bk.state_init(qc.num_qubits, device=sim.device)
+    plan.run(st)
+    return st
+
+
+@pytest.mark.parametrize("mode", ["plain", "blocks"])
+def test_su4_28q_circuit_then_inverse(sim, mode):
+    import blackwell_kernels as bk
+    from blackwell_backend import workloads as W
+    print(f"[fullsize] su4 28q forward+inverse mode={mode}", flush=True)
+    st = _run_on_device(sim, _inverse_appended(W.su4_volume(28, 28, 2024)), mode)
+    p0 = abs(complex(st[0].item())) ** 2
+    nrm = float(bk.norm2(st))
+    del st
+    torch.cuda.empty_cache()
+    assert p0 >= 1 - 1e-6 and abs(nrm - 1) <= 1e-6, (p0, nrm)
+
+
+def test_c3_32q_circuit_then_inverse(sim):
+    import blackwell_kernels as bk
+    from blackwell_backend import workloads as W
+    print("[fullsize] rotation+MCX 32q forward+inverse (plain)", flush=True)
+    st = _run_on_device(sim, _inverse_appended(W.c3_rotation_mcx(32, 8, 77)), "plain")
+    p0 = abs(complex(st[0].item())) ** 2
+    nrm = float(bk.norm2(st))
+    del st
+    torch.cuda.empty_cache()
+    assert p0 >= 1 - 1e-6 and abs(nrm - 1) <= 1e-6, (p0, nrm)
+
+
+def test_permutation_layer_32q_round_trip_bit_exact():
+    import blackwell_kernels as bk
+    n = 32
+    print("[fullsize] permutation layer 32q round trip", flush=True)
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(7)
+    st = torch.randn(1 << n, dtype=torch.complex64, device="cuda:0", generator=g)
+    ref = st.clone()
+    rng = np.random.default_rng(3)
+    gates = [([q], q + 1) for q in range(n - 1)]                       # CNOT ladder
+    gates += [([0], 17), ([3], 29), ([2, 11, 25], 19), ([11, 19, 25], 2), ([31, 30, 29], 0),
+              ([5, 6], 31), ([1, 2, 3, 4, 20, 21, 22], 12)]
+    for _ in range(6):
+        qs = [int(q) for q in rng.choice(n, 4, replace=False)]
+        gates.append((qs[:-1], qs[-1]))
+    for ctl, t in gates:
+        bk.apply_mcx(st, ctl, t)
+    for ctl, t in reversed(gates):
+        bk.apply_mcx(st, ctl, t)
+    same = bool(torch.equal(st.view(torch.int32), ref.view(torch.int32)))
+    del st, ref
+    torch.cuda.empty_cache()
+    assert same
+
+
+def test_sampler_32q_support_and_determinism():
+    import blackwell_kernels as bk
+    n = 32
+    print("[fullsize] sampler 32q", flush=True)
+    st = bk.state_init(n, device="cuda:0")
+    # GHZ-like support {0, 2^n - 1, 2^31} with known weights 1/2, 1/4, 1/4
+    st[0] = math.sqrt(0.5)
+    st[(1 << n) - 1] = math.sqrt(0.25)
+    st[1 << 31] = -math.sqrt(0.25)
+    u = bk.uniforms_splitmix(2024, 1 << 20, "cuda:0")
+    a = bk.sample_with_uniforms(st, u)
+    b = bk.sample_with_uniforms(st, u)
+    vals, cnt = torch.unique(a, return_counts=True)
+    del st
+    torch.cuda.empty_cache()
+    assert torch.equal(a, b)
+    assert set(vals.tolist()) <= {0, (1 << n) - 1, 1 << 31}
+    frac = {int(v): int(c) / (1 << 20) for v, c in zip(vals.tolist(), cnt.tolist())}
+    assert abs(frac.get(0, 0) - 0.5) < 0.005 and abs(frac.get(1 << 31, 0) - 0.25) < 0.005
