@@ -432,6 +432,16 @@ def serving_bench(n_requests=4096, seed=99, shots=1024):
     }
 
 
+# extra sections: name -> (JSON key, function)
+SECTIONS = {
+    "per_gate": ("per_gate_30q", lambda: per_gate_sweep(30)),
+    "ab": ("ab_suite", lambda: ab_suite_bench()),
+    "c3": ("c3_rotation_mcx_32q", lambda: c3_bench()),
+    "c5": ("c5_chebyshev_22q", lambda: c5_bench()),
+    "serving": ("serving", lambda: serving_bench()),
+}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -439,6 +449,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--qubits", type=int, default=28)
     ap.add_argument("--no-extras", action="store_true", help="skip the 30q sweep and CPU baseline")
+    ap.add_argument("--only", choices=sorted(SECTIONS),
+                    help="run one extra section alone and print its JSON (for profiling it)")
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="first run one plain device copy of a 2^qubits complex64 vector (a kernel "
                          "of known traffic, for scaling HBM byte counters)")
@@ -447,6 +459,11 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the statevector path here runs on a single MI355X)", file=sys.stderr)
         sys.exit(2)
+    if args.only:
+        key, fn = SECTIONS[args.only]
+        log(f"[bench] section {args.only} alone")
+        print(json.dumps({key: fn()}), flush=True)
+        return
     from blackwell_backend import BlackwellSimulator, workloads as W
     n = args.qubits
     depth = n
@@ -515,31 +532,13 @@ def main():
     except Exception as e:  # reported, not hidden
         line["parity"] = {"error": repr(e)}
     if not args.no_extras:
-        try:
-            log("[bench] per-gate sweep at 30 qubits")
-            line["per_gate_30q"] = per_gate_sweep(30)
-        except Exception as e:
-            line["per_gate_30q"] = {"error": repr(e)}
-        try:
-            log("[bench] reference A/B suite (run + 8192 shots, 4..20 qubits)")
-            line["ab_suite"] = ab_suite_bench()
-        except Exception as e:
-            line["ab_suite"] = {"error": repr(e)}
-        try:
-            log("[bench] 32-qubit rotation + MCX ansatz, per-class GB/s and 2^20 shots")
-            line["c3_rotation_mcx_32q"] = c3_bench()
-        except Exception as e:
-            line["c3_rotation_mcx_32q"] = {"error": repr(e)}
-        try:
-            log("[bench] 22-qubit Chebyshev evolution, 32 RHS")
-            line["c5_chebyshev_22q"] = c5_bench()
-        except Exception as e:
-            line["c5_chebyshev_22q"] = {"error": repr(e)}
-        try:
-            log("[bench] serving stream")
-            line["serving"] = serving_bench()
-        except Exception as e:
-            line["serving"] = {"error": repr(e)}
+        for name in ("per_gate", "ab", "c3", "c5", "serving"):
+            key, fn = SECTIONS[name]
+            try:
+                log(f"[bench] {key}")
+                line[key] = fn()
+            except Exception as e:  # reported, not hidden
+                line[key] = {"error": repr(e)}
         try:
             log("[bench] CPU baseline")
             line["cpu_baseline"] = cpu_baseline(n, depth, seed)

@@ -83,7 +83,7 @@ def test_gate_2q_all_pairs(bk, dev, n):
 
 def test_gate_2q_wide(bk, dev):
     n = 14
-    for q0, q1 in [(0, 13), (13, 0), (1, 2), (7, 3), (12, 13), (0, 1)]:
+    for q0, q1 in [(0, 13), (13, 0), (1, 2), (7, 3), (12, 13), (0, 1), (7, 8), (13, 9), (8, 11)]:
         psi = rand_state(n, q0 + 17 * q1)
         u = rand_unitary(4, q0 + q1).astype(np.complex64)
         st = to_dev(psi, dev)
