@@ -252,3 +252,34 @@ def test_circuit_fingerprint_tracks_content():
     u[0, 0] = 5.0                                       # the caller's array is not aliased
     assert qc.data[2].operation.params[0][0, 0] == 1.0
     assert fingerprint(object()) is None
+
+
+def test_plan_blocks_seed_lookahead_with_wide_ops():
+    """Blocks may start from any op that no earlier remaining op touches,
+    also past an op too wide for a block; the result still equals the circuit,
+    and the depth-28 SU(4) circuit on 28 qubits packs into at most 100 blocks."""
+    from blackwell_backend import compiler as C, workloads as W
+    from blackwell_backend.circuit import QuantumCircuit, instructions
+    n = 10
+    rng = np.random.default_rng(8)
+    qc = QuantumCircuit(n)
+    for layer in range(6):
+        for _ in range(4):
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            qc.unitary(W.haar_unitary(rng, 4), [a, b])
+        if layer % 2 == 0:
+            qs = [int(x) for x in rng.choice(n, 8, replace=False)]
+            qc.mcx(qs[:-1], qs[-1])                  # 8 qubits: passes through
+        qc.h(int(rng.integers(n)))
+    ops = C.compile_ops(instructions(qc))
+    entries = C.plan_blocks(ops, n, k=5)
+    assert any(not isinstance(e, C.Block) for e in entries)
+    psi = O.zero_state(n)
+    for e in entries:
+        if isinstance(e, C.Block):
+            psi = O.apply_matrix(psi, e.mat, e.qubits)
+        else:
+            psi = O.apply_matrix(psi, e.full_matrix(), list(e.all_qubits))
+    assert O.max_abs_diff_phase_aligned(_sim_ops(n, ops), psi) < 1e-12
+    big = C.compile_ops(instructions(W.su4_volume(28, 28, 2024)))
+    assert len(C.plan_blocks(big, 28, k=5)) <= 100
