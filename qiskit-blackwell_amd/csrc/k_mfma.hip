@@ -109,6 +109,128 @@ __global__ __launch_bounds__(64 * W) void k_block5(float2* __restrict__ st, uint
   }
 }
 
+// Wide variant (n >= 11, n <= 32): 16-byte lanes over sets of 64 groups.
+//   Q0 = 0 (qubit 0 outside the block): lane (h, c) holds groups 2c and 2c+1
+//     of row 2s+h as one float4 (they are adjacent amplitudes); the even and
+//     odd groups run as two MFMA column tiles.
+//   Q0 = 1 (block row bit 0 = qubit 0): lane (h, c) holds rows 2s and 2s+1 of
+//     group c + 32h as one float4; one cross-lane swap of the halves per K
+//     slice gives the MFMA layout, and the results come out with rows i, i+1
+//     of a group in the same lane, so stores need no exchange.
+// Each wave walks a contiguous range of sets (XCD-major workgroup order).
+// PF: the next set's loads are issued before the current MFMA chains.
+// NT: nontemporal loads and stores, used only when every access covers whole
+// 128-byte lines (no block qubit among 0..3).
+template <int PF, int NT, int Q0>
+__global__ __launch_bounds__(256) void k_block5w(float2* __restrict__ st, uint64_t n_sets, BlockArgs a,
+                                                 const float2* __restrict__ U) {
+  const int lane = threadIdx.x & 63;
+  const int h = lane >> 5, c = lane & 31;
+  float ur[16], ui[16];
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const float2 u = U[c * 32 + 2 * s + h];
+    ur[s] = u.x;
+    ui[s] = u.y;
+  }
+  uint32_t rof[16];
+#pragma unroll
+  for (int s = 0; s < 16; ++s) rof[s] = (uint32_t)row_off(Q0 ? 2 * s : 2 * s + h, a);
+  uint32_t wof[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r)
+    wof[r] = Q0 ? (uint32_t)row_off(8 * ((r >> 1) >> 1) + 4 * h + 2 * ((r >> 1) & 1), a)
+                : (uint32_t)row_off(8 * (r >> 2) + 4 * h + (r & 3), a);
+  const uint64_t tw = (uint64_t)gridDim.x * 4;
+  const uint64_t gw = (uint64_t)xcd_block<uint32_t>() * 4 + (threadIdx.x >> 6);
+  const uint64_t per = (n_sets + tw - 1) / tw;
+  const uint64_t s0 = gw * per, s1 = s0 + per < n_sets ? s0 + per : n_sets;
+  if (s0 >= s1) return;
+  const uint64_t gl = Q0 ? (uint64_t)(c + 32 * h) : (uint64_t)(2 * c);
+  f32x4_t x[16], y[16];
+  uint64_t base = group_base(s0 * 64 + gl, a);
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const f32x4_t* p = reinterpret_cast<const f32x4_t*>(st + (base | rof[s]));
+    x[s] = NT ? __builtin_nontemporal_load(p) : *p;
+  }
+  for (uint64_t set = s0; set < s1; ++set) {
+    const bool more = set + 1 < s1;
+    uint64_t nbase = 0;
+    if (PF && more) {
+      nbase = group_base((set + 1) * 64 + gl, a);
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const f32x4_t* p = reinterpret_cast<const f32x4_t*>(st + (nbase | rof[s]));
+        y[s] = NT ? __builtin_nontemporal_load(p) : *p;
+      }
+    }
+    f32x16 er = {0.f}, ei = {0.f}, orr = {0.f}, oi = {0.f};
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      float ar, ai, br, bi;   // operands of column tile 1 and column tile 2
+      if (Q0) {
+        const float sr = h ? x[s].x : x[s].z, si = h ? x[s].y : x[s].w;
+        const float rr = __shfl_xor(sr, 32, 64), ri = __shfl_xor(si, 32, 64);
+        ar = h ? rr : x[s].x;
+        ai = h ? ri : x[s].y;
+        br = h ? x[s].z : rr;
+        bi = h ? x[s].w : ri;
+      } else {
+        ar = x[s].x; ai = x[s].y; br = x[s].z; bi = x[s].w;
+      }
+      er = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[s], ar, er, 0, 0, 0);
+      er = __builtin_amdgcn_mfma_f32_32x32x2f32(-ui[s], ai, er, 0, 0, 0);
+      ei = __builtin_amdgcn_mfma_f32_32x32x2f32(ui[s], ar, ei, 0, 0, 0);
+      ei = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[s], ai, ei, 0, 0, 0);
+      orr = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[s], br, orr, 0, 0, 0);
+      orr = __builtin_amdgcn_mfma_f32_32x32x2f32(-ui[s], bi, orr, 0, 0, 0);
+      oi = __builtin_amdgcn_mfma_f32_32x32x2f32(ui[s], br, oi, 0, 0, 0);
+      oi = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[s], bi, oi, 0, 0, 0);
+    }
+    if (Q0) {
+      // registers r, r+1 (r even) hold rows i, i+1 of group c (tile 1) and c + 32 (tile 2)
+      const uint64_t b1 = group_base(set * 64 + c, a), b2 = group_base(set * 64 + 32 + c, a);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int r = 2 * k;
+        const f32x4_t o1 = {er[r], ei[r], er[r + 1], ei[r + 1]};
+        const f32x4_t o2 = {orr[r], oi[r], orr[r + 1], oi[r + 1]};
+        f32x4_t* p1 = reinterpret_cast<f32x4_t*>(st + (b1 | wof[r]));
+        f32x4_t* p2 = reinterpret_cast<f32x4_t*>(st + (b2 | wof[r]));
+        if (NT) {
+          __builtin_nontemporal_store(o1, p1);
+          __builtin_nontemporal_store(o2, p2);
+        } else {
+          *p1 = o1;
+          *p2 = o2;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const f32x4_t o = {er[r], ei[r], orr[r], oi[r]};
+        f32x4_t* p = reinterpret_cast<f32x4_t*>(st + (base | wof[r]));
+        if (NT) __builtin_nontemporal_store(o, p);
+        else *p = o;
+      }
+    }
+    if (!more) break;
+    if (PF) {
+      base = nbase;
+#pragma unroll
+      for (int s = 0; s < 16; ++s) x[s] = y[s];
+    } else {
+      base = group_base((set + 1) * 64 + gl, a);
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const f32x4_t* p = reinterpret_cast<const f32x4_t*>(st + (base | rof[s]));
+        x[s] = NT ? __builtin_nontemporal_load(p) : *p;
+      }
+    }
+  }
+}
+
 // LDS-staged variant (n >= 12).  A workgroup of 4 waves owns 128 groups:
 // the tile is the 4096 amplitudes spanned by the 5 block qubits and the 7
 // lowest non-block qubits.  Those 12 qubits always include qubits 0..6, so
@@ -246,6 +368,27 @@ int qsim_apply_block_mfma(void* stream, void* state, int n_qubits, const void* b
   // across many cache lines per instruction; they go through the LDS tile.
   int low = 0;
   for (int j = 0; j < 5; ++j) low += a.q[j] < 5;
+  int qmin = a.q[0];
+  for (int j = 1; j < 5; ++j) qmin = std::min(qmin, a.q[j]);
+  // Wide kernel (16-byte lanes); variants chosen from 30-qubit measurements:
+  //   no block qubit in 0..3: prefetch + nontemporal, two workgroups per CU;
+  //   lowest block qubit in 1..3: prefetch, cached accesses, one per CU;
+  //   qubit 0 in the block: no prefetch, cached accesses, two per CU.
+  if (n_qubits >= 11 && n_qubits <= 32 && low <= 2 && (qmin != 0 || a.q[0] == 0)) {
+    const uint64_t n_sets = (1ull << (n_qubits - 5)) / 64;
+    hipStream_t s = as_stream(stream);
+    auto grid = [&](int per_cu) { return dim3((unsigned)std::min<uint64_t>(ceil_div(n_sets, 4), 256u * per_cu)); };
+    if (qmin == 0)
+      hipLaunchKernelGGL((k_block5w<0, 0, 1>), grid(2), dim3(256), 0, s, (float2*)state, n_sets, a,
+                         (const float2*)block_dev);
+    else if (qmin < 4)
+      hipLaunchKernelGGL((k_block5w<1, 0, 0>), grid(1), dim3(256), 0, s, (float2*)state, n_sets, a,
+                         (const float2*)block_dev);
+    else
+      hipLaunchKernelGGL((k_block5w<1, 1, 0>), grid(2), dim3(256), 0, s, (float2*)state, n_sets, a,
+                         (const float2*)block_dev);
+    return launch_status("apply_block_mfma");
+  }
   if (n_qubits >= 12 && (low >= 3 || n_qubits > 32)) {
     Block5Tile tl;
     memset(&tl, 0, sizeof tl);

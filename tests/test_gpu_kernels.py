@@ -343,8 +343,12 @@ def test_tile_program_random(bk, dev):
         assert _rel_err(ref, st.cpu().numpy()) < 2e-5, trial
 
 
-@pytest.mark.parametrize("n,qubits", [(10, [0, 1, 2, 3, 4]), (10, [9, 5, 0, 7, 2]),
-                                      (12, [11, 10, 9, 8, 7]), (16, [3, 15, 8, 0, 11])])
+@pytest.mark.parametrize("n,qubits", [
+    (10, [0, 1, 2, 3, 4]), (10, [9, 5, 0, 7, 2]), (12, [11, 10, 9, 8, 7]), (16, [3, 15, 8, 0, 11]),
+    # 16-byte-lane kernel: qubit 0 outside the block (lowest block qubit >= 4 or in 1..3) and
+    # qubit 0 as block row bit 0; n = 11 is the smallest state it takes (one set of 64 groups)
+    (11, [4, 5, 6, 9, 10]), (14, [1, 6, 8, 12, 13]), (14, [2, 3, 7, 9, 13]), (14, [0, 5, 6, 9, 12]),
+    (13, [0, 1, 7, 10, 12]), (17, [16, 12, 8, 5, 4]), (15, [0, 14, 3, 9, 6]), (11, [0, 6, 7, 9, 10])])
 def test_block_mfma(bk, dev, n, qubits):
     psi = rand_state(n, n + qubits[0])
     u = rand_unitary(32, sum(qubits)).astype(np.complex64)
