@@ -13,9 +13,11 @@
 // and takes an inclusive scan across the wave in a fixed order; the last
 // value is the leaf total.  An exclusive FP64 scan of the leaf totals gives
 // the leaf prefix.  For each shot x = u * total: a binary search on the leaf
-// prefix finds the leaf, and the same wave scan, recomputed, finds the
-// smallest index whose cumulative probability exceeds x.  Because the leaf
-// total and the in-leaf search use the identical scan, they agree bit for bit.
+// prefix finds the leaf, and a wave scan of that leaf finds the smallest
+// index with nonzero probability whose cumulative probability exceeds
+// x - prefix.  Where rounding puts x past the leaf's scanned total, the leaf's
+// last nonzero amplitude is taken, and the nonzero test keeps a rounding step
+// in the cumulative sums from ever selecting a zero-probability amplitude.
 // (Rule: smallest i with cdf(i) > u * total; the reference returns the
 // smallest i with cdf[i] >= u on an FP32 CDF, bwk/csrc/sample_sm120.cu:48-69.)
 #include "qsim_internal.hpp"
@@ -23,6 +25,7 @@
 namespace qsim {
 
 constexpr int kRedBlocks = 1024;   // pass-1 workgroups for large states
+constexpr int kRedUnroll = 4;      // slots in flight per lane in pass 1
 constexpr int kLeaf = 256;         // amplitudes per sampling leaf
 constexpr int kScanSeg = 4096;     // leaf totals per scan segment
 
@@ -58,7 +61,25 @@ __global__ __launch_bounds__(kThreads) void k_red1(const float4* __restrict__ st
   const uint64_t beg = (uint64_t)blockIdx.x * chunk;
   const uint64_t end = beg + chunk < n4 ? beg + chunk : n4;
   double s = 0.0, s0 = 0.0;
-  for (uint64_t f = beg + threadIdx.x; f < end; f += kThreads) {
+  // kRedUnroll slots per lane are loaded before any is summed (more bytes in
+  // flight per wave); the summation order stays fixed
+  uint64_t f = beg + threadIdx.x;
+  for (; f + (kRedUnroll - 1) * kThreads < end; f += kRedUnroll * kThreads) {
+    float4 v[kRedUnroll];
+#pragma unroll
+    for (int u = 0; u < kRedUnroll; ++u) v[u] = ld_nt(st + f + u * kThreads);
+#pragma unroll
+    for (int u = 0; u < kRedUnroll; ++u) {
+      const double a = p2(lo2(v[u])), b = p2(hi2(v[u]));
+      s += a + b;
+      if (MODE == 1) {
+        const uint64_t i0 = 2 * (f + u * kThreads);
+        if (t == 0) s0 += a;
+        else if (!((i0 >> t) & 1)) s0 += a + b;
+      }
+    }
+  }
+  for (; f < end; f += kThreads) {
     const float4 v = ld_nt(st + f);
     const double a = p2(lo2(v)), b = p2(hi2(v));
     s += a + b;
@@ -167,35 +188,59 @@ __global__ __launch_bounds__(kThreads) void k_probs(const float4* __restrict__ s
 }
 
 // ---------------------------------------------------------------- sampling
-// Wave-level view of one leaf: lane l holds amplitudes 4l..4l+3 (two float4
-// slots).  c[0..3] receive the inclusive cumulative |a|^2 of those four
-// amplitudes within the leaf: lane-local running sums plus the inclusive
-// Hillis-Steele scan of the lanes below (fixed order, so every caller gets
-// the same bits).  nl = amplitudes in the leaf (256, or 2^n for tiny states).
-__device__ __forceinline__ void leaf_scan(const float4* __restrict__ leaf, int lane, int nl,
-                                          double c[4]) {
-  double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
-  if (4 * lane < nl) {
-    const float4 v0 = leaf[2 * lane];
-    q0 = p2(lo2(v0)); q1 = p2(hi2(v0));
-  }
-  if (4 * lane + 2 < nl) {
-    const float4 v1 = leaf[2 * lane + 1];
-    q2 = p2(lo2(v1)); q3 = p2(hi2(v1));
-  }
-  c[0] = q0; c[1] = c[0] + q1; c[2] = c[1] + q2; c[3] = c[2] + q3;
-  double incl = c[3];
+// Wave-level view of one leaf of 256 amplitudes: lane l holds slot l
+// (amplitudes 2l, 2l+1) and slot 64 + l (amplitudes 128 + 2l, 129 + 2l), so
+// each of the two loads reads 1 KiB contiguous.  q[0..3] are their |a|^2 in
+// FP64, c[0..3] the inclusive cumulative sums within the leaf, in amplitude
+// order: the first half by a fixed-order Hillis-Steele scan of the lane pairs,
+// the second half the same way on top of the first half's last cumulative
+// value (taken from lane 63, so the sequence is exactly non-decreasing across
+// the halves).  nl = amplitudes in the leaf (256, or 2^n for tiny states);
+// missing amplitudes count as zero.
+struct LeafScan {
+  double q[4], c[4];
+};
+
+__device__ __forceinline__ double wave_incl_scan(double v, int lane) {
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
-    const double y = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += y;
+    const double y = __shfl_up(v, d, 64);
+    if (lane >= d) v += y;
   }
-  const double prev = __shfl_up(incl, 1, 64);
-  const double base = lane == 0 ? 0.0 : prev;
-  c[0] += base; c[1] += base; c[2] += base; c[3] += base;
+  return v;
 }
 
-// Leaf totals: one wave per leaf; total = last lane's c[3].
+__device__ __forceinline__ void leaf_scan(const float4* __restrict__ leaf, int lane, int nl,
+                                          LeafScan& L) {
+  L.q[0] = L.q[1] = L.q[2] = L.q[3] = 0.0;
+  if (2 * lane < nl) {
+    const float4 v0 = leaf[lane];
+    L.q[0] = p2(lo2(v0)); L.q[1] = p2(hi2(v0));
+  }
+  if (128 + 2 * lane < nl) {
+    const float4 v1 = leaf[64 + lane];
+    L.q[2] = p2(lo2(v1)); L.q[3] = p2(hi2(v1));
+  }
+  const double pa = L.q[0] + L.q[1];
+  const double ia = wave_incl_scan(pa, lane);
+  double prev = __shfl_up(ia, 1, 64);
+  double base = lane == 0 ? 0.0 : prev;
+  L.c[0] = base + L.q[0];
+  L.c[1] = L.c[0] + L.q[1];
+  const double half = __shfl(L.c[1], 63, 64);
+  const double pb = L.q[2] + L.q[3];
+  const double ib = wave_incl_scan(pb, lane);
+  prev = __shfl_up(ib, 1, 64);
+  base = half + (lane == 0 ? 0.0 : prev);
+  L.c[2] = base + L.q[2];
+  L.c[3] = L.c[2] + L.q[3];
+}
+
+// Leaf totals: one wave per leaf, the four |a|^2 of each lane (the
+// leaf_scan layout, 1 KiB per load) summed across the wave by the fixed
+// butterfly of wave_sum.  The in-leaf search recomputes the cumulative sums
+// with leaf_scan; where rounding makes the two disagree, an x past the scan's
+// total takes the leaf's last nonzero amplitude.
 __global__ __launch_bounds__(kThreads) void k_leaf_totals(const float4* __restrict__ st,
                                                           uint64_t n_amps, uint64_t n_leaves,
                                                           double* __restrict__ tot) {
@@ -203,10 +248,18 @@ __global__ __launch_bounds__(kThreads) void k_leaf_totals(const float4* __restri
   const uint64_t leaf = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (leaf >= n_leaves) return;
   const int nl = (int)(n_amps < (uint64_t)kLeaf ? n_amps : kLeaf);
-  double c[4];
-  leaf_scan(st + leaf * (kLeaf / 2), lane, nl, c);
-  const double last = __shfl(c[3], 63, 64);
-  if (lane == 0) tot[leaf] = last;
+  const float4* lf = st + leaf * (kLeaf / 2);
+  double v = 0.0;
+  if (2 * lane < nl) {
+    const float4 a = ld_nt(lf + lane);
+    v = p2(lo2(a)) + p2(hi2(a));
+  }
+  if (128 + 2 * lane < nl) {
+    const float4 b = ld_nt(lf + 64 + lane);
+    v += p2(lo2(b)) + p2(hi2(b));
+  }
+  v = wave_sum(v);
+  if (lane == 0) tot[leaf] = v;
 }
 
 // Exclusive scan of the leaf totals in segments of kScanSeg: pass A writes
@@ -279,22 +332,17 @@ __global__ __launch_bounds__(kThreads) void k_leaf_prefix(const double* __restri
   }
 }
 
-// Last amplitude of the leaf with nonzero probability (-1 if none), from the
-// cumulative values of leaf_scan.
-__device__ __forceinline__ int64_t leaf_last_nonzero(const double c[4], int lane, int nl) {
-  double prevc = __shfl_up(c[3], 1, 64);
-  if (lane == 0) prevc = 0.0;
-  int lastk = -1;
-  if (4 * lane < nl) {
-    if (c[0] > prevc) lastk = 0;
-    if (c[1] > c[0]) lastk = 1;
-    if (c[2] > c[1]) lastk = 2;
-    if (c[3] > c[2]) lastk = 3;
+// Last amplitude of the leaf with nonzero probability (-1 if none).
+__device__ __forceinline__ int64_t leaf_last_nonzero(const LeafScan& L, int lane) {
+  const uint64_t hi = __ballot(L.q[2] > 0.0 || L.q[3] > 0.0);
+  if (hi) {
+    const int fl = 63 - __clzll((unsigned long long)hi);
+    return 128 + 2 * fl + (__shfl(L.q[3] > 0.0 ? 1 : 0, fl, 64));
   }
-  const uint64_t any = __ballot(lastk >= 0);
-  if (!any) return -1;
-  const int fl = 63 - __clzll((unsigned long long)any);
-  return 4 * fl + __shfl(lastk, fl, 64);
+  const uint64_t lo = __ballot(L.q[0] > 0.0 || L.q[1] > 0.0);
+  if (!lo) return -1;
+  const int fl = 63 - __clzll((unsigned long long)lo);
+  return 2 * fl + (__shfl(L.q[1] > 0.0 ? 1 : 0, fl, 64));
 }
 
 // One wave per shot.  Leaf j = largest j with pre[j] <= x; a leaf with zero
@@ -320,17 +368,22 @@ __global__ __launch_bounds__(kThreads) void k_sample(const float4* __restrict__ 
   while (leaf > 0 && !(tot[leaf] > 0.0)) --leaf;
   const double y = x - pre[leaf];
   const int nla = (int)(n_amps < (uint64_t)kLeaf ? n_amps : kLeaf);
-  double c[4];
-  leaf_scan(st + leaf * (kLeaf / 2), lane, nla, c);
-  int k = 4;
-  if (4 * lane < nla && c[3] > y) k = c[0] > y ? 0 : (c[1] > y ? 1 : (c[2] > y ? 2 : 3));
-  const uint64_t hit = __ballot(k < 4);
+  LeafScan L;
+  leaf_scan(st + leaf * (kLeaf / 2), lane, nla, L);
+  // smallest index with nonzero probability whose cumulative exceeds y:
+  // first half (amplitudes 2l, 2l+1), then second half (128 + 2l, 129 + 2l)
+  const bool h0 = L.q[0] > 0.0 && L.c[0] > y, h1 = L.q[1] > 0.0 && L.c[1] > y;
+  const bool h2 = L.q[2] > 0.0 && L.c[2] > y, h3 = L.q[3] > 0.0 && L.c[3] > y;
+  const uint64_t hit_lo = __ballot(h0 || h1), hit_hi = __ballot(h2 || h3);
   int64_t local;
-  if (hit) {
-    const int fl = __ffsll((unsigned long long)hit) - 1;
-    local = 4 * fl + __shfl(k, fl, 64);
+  if (hit_lo) {
+    const int fl = __ffsll((unsigned long long)hit_lo) - 1;
+    local = 2 * fl + __shfl(h0 ? 0 : 1, fl, 64);
+  } else if (hit_hi) {
+    const int fl = __ffsll((unsigned long long)hit_hi) - 1;
+    local = 128 + 2 * fl + __shfl(h2 ? 0 : 1, fl, 64);
   } else {
-    local = leaf_last_nonzero(c, lane, nla);
+    local = leaf_last_nonzero(L, lane);
     if (local < 0) local = 0;  // all-zero state: index 0 (as the reference)
   }
   if (lane == 0) out[shot] = (int64_t)(leaf * kLeaf) + local;
