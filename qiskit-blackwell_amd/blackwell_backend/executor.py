@@ -29,6 +29,10 @@ BLOCK_MIN_QUBITS = 10   # the MFMA block kernel needs 2^10 amplitudes
 # amplitudes per thread; one fused block costs ~5 us (launch-bound at this size)
 WHOLE_OP_US_12Q = 2.0
 BLOCK_US = 5.0
+# plan steps a HIP graph can hold: library launches on the current stream with
+# device-resident arguments only (no host uploads, no host-side decisions)
+GRAPH_SAFE_STEPS = ("blocks", "tile", "mcgate")
+TUNE_REPS = 3
 
 
 def _d8(m):
@@ -104,7 +108,59 @@ class CompiledCircuit:
         self.ops = ops
         self._plans = {}
         self._blocks = None
+        self._graphs = {}
+        self._tuned = {}
         self.has_stochastic = any(o.kind in ("measure", "reset") for o in ops)
+
+    # ---------------------------------------------------------------- graphs
+    def graphed(self, mode, device):
+        """GraphedRun of |0...0> + the plan for `mode` (built once), or None
+        when the plan has a step a graph cannot hold (mid-circuit
+        measure/reset, host-uploaded op tables)."""
+        key = (mode, str(device))
+        if key not in self._graphs:
+            plan = self.plan(mode, device)
+            self._graphs[key] = (GraphedRun(plan, self.n_qubits, device)
+                                 if not self.has_stochastic and plan.graph_safe() else None)
+        return self._graphs[key]
+
+    # ---------------------------------------------------------------- tuning
+    def tuned_mode(self, device):
+        """The fastest execution mode for this circuit on `device`, measured
+        once: every applicable mode runs once to warm up and TUNE_REPS times
+        timed with HIP events on the launch stream, from |0...0>."""
+        key = str(device)
+        if key in self._tuned:
+            return self._tuned[key]
+        n = self.n_qubits
+        cands = ["plain"]
+        if n <= WHOLE_TILE_MAX:
+            cands.append("whole")
+        else:
+            cands.append("tiles")
+        if n >= BLOCK_MIN_QUBITS:
+            cands.append("blocks")
+        state = torch.empty(1 << n, dtype=torch.complex64, device=device)
+        stream = torch.cuda.current_stream(device)
+        best, best_ms = None, None
+        for mode in cands:
+            plan = self.plan(mode, device)
+            times = []
+            for r in range(TUNE_REPS + 1):
+                call("qsim_state_init", _lib.stream_handle(device), ctypes.c_void_p(state.data_ptr()), n)
+                a = torch.cuda.Event(enable_timing=True)
+                b = torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                plan.run(state)
+                b.record(stream)
+                b.synchronize()
+                if r:
+                    times.append(a.elapsed_time(b))
+            ms = float(np.median(times))
+            if best_ms is None or ms < best_ms:
+                best, best_ms = mode, ms
+        self._tuned[key] = best
+        return best
 
     def block_entries(self):
         """plan_blocks of the ops (computed once per compiled circuit)."""
@@ -121,10 +177,15 @@ class CompiledCircuit:
             self._plans[key] = p
         return p
 
-    def resolve_mode(self, mode):
+    def resolve_mode(self, mode, device=None):
         """auto: the whole state as one LDS program below 10 qubits, MFMA
         blocks above 13, and between the two whichever the cost model says
-        is cheaper (deep circuits favour blocks: one block absorbs many ops)."""
+        is cheaper (deep circuits favour blocks: one block absorbs many ops).
+        tune: the mode tuned_mode measured fastest on `device`."""
+        if mode == "tune":
+            if device is None:
+                raise ValueError("mode 'tune' needs the device to measure on")
+            return self.tuned_mode(device)
         if mode != "auto":
             return mode
         n = self.n_qubits
@@ -263,6 +324,15 @@ class _Plan:
             else:
                 self.n_launches += 1
 
+    def graph_safe(self):
+        for kind, a in self.steps:
+            if kind == "sub":
+                if not a.graph_safe():
+                    return False
+            elif not (kind.startswith("qsim_") or kind in GRAPH_SAFE_STEPS):
+                return False
+        return True
+
     def _mat_ptr(self, idx):
         return ctypes.c_void_p(self.dev.mats.data_ptr() + 8 * self.mat_offsets[idx])
 
@@ -320,3 +390,35 @@ class _Plan:
                 apply_mcx(state, [], op.qubits[0])
         else:
             raise RuntimeError(f"unknown step {kind}")
+
+
+class GraphedRun:
+    """A HIP graph (torch.cuda.CUDAGraph) of qsim_state_init + one plan over a
+    state buffer the graph owns.  replay() relaunches the whole circuit with
+    one host call; the buffer holds the result until the next replay.  The
+    plan runs once eagerly on a side stream before capture, so every lazy
+    allocation (workspaces, plan tables) exists before the graph records
+    pointers."""
+
+    def __init__(self, plan, n_qubits, device):
+        self.n = n_qubits
+        self.buf = torch.empty(1 << n_qubits, dtype=torch.complex64, device=device)
+        side = torch.cuda.Stream(device)
+        side.wait_stream(torch.cuda.current_stream(device))
+        with torch.cuda.stream(side):
+            self._body(plan)
+        torch.cuda.current_stream(device).wait_stream(side)
+        torch.cuda.synchronize(device)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._body(plan)
+        torch.cuda.synchronize(device)
+
+    def _body(self, plan):
+        call("qsim_state_init", _lib.stream_handle(self.buf.device), ctypes.c_void_p(self.buf.data_ptr()),
+             self.n)
+        plan.run(self.buf)
+
+    def replay(self):
+        self.graph.replay()
+        return self.buf

@@ -89,9 +89,12 @@ class BlackwellSimulator:
         result = sim.run(circuit, shots=1024)
     """
 
-    MODES = ("auto", "plain", "blocks", "tiles", "whole")
+    MODES = ("auto", "plain", "blocks", "tiles", "whole", "tune")
+    # largest circuit run() replays as a HIP graph (the graph owns a state
+    # buffer of 8 * 2^n bytes per cached circuit)
+    GRAPH_MAX_QUBITS = 20
 
-    def __init__(self, device="cuda", kernel_path=None, mode="auto"):
+    def __init__(self, device="cuda", kernel_path=None, mode="auto", use_graphs=True):
         if mode not in self.MODES:
             raise ValueError(f"mode must be one of {self.MODES}")
         self.device = torch.device(device)
@@ -100,6 +103,7 @@ class BlackwellSimulator:
         self.kernel_path = kernel_path
         self.bk = _load_kernels(kernel_path)
         self.mode = mode
+        self.use_graphs = bool(use_graphs)
         self._native_1q = set(C._NATIVE_1Q)
         self._native_2q = {"cx", "cz", "swap", "cp", "crz", "cu"}
         self._native_mcx = {"ccx"}
@@ -177,10 +181,23 @@ class BlackwellSimulator:
         return cc
 
     # ---------------------------------------------------------------- execute
-    def _execute_circuit(self, circuit, mode=None):
-        """(state, clbits, elapsed_s) — state stays on the device."""
+    def _execute_circuit(self, circuit, mode=None, graphed=False):
+        """(state, clbits, elapsed_s) — state stays on the device.
+
+        graphed: callers that consume the state before the next execution
+        (run, run_statevector, run_and_time) get circuits of at most
+        GRAPH_MAX_QUBITS qubits without mid-circuit measurement replayed from a
+        HIP graph; the returned state is then the graph's own buffer."""
         cc = self.compile(circuit)
-        mode = cc.resolve_mode(mode or self.mode)
+        mode = cc.resolve_mode(mode or self.mode, self.device)
+        if graphed and self.use_graphs and cc.n_qubits <= self.GRAPH_MAX_QUBITS:
+            g = cc.graphed(mode, self.device)
+            if g is not None:
+                torch.cuda.synchronize(self.device)
+                t0 = time.perf_counter()
+                state = g.replay()
+                torch.cuda.synchronize(self.device)
+                return state, [0] * cc.n_clbits, time.perf_counter() - t0
         plan = cc.plan(mode, self.device)
         state = self.bk.state_init(cc.n_qubits, device=self.device)
         clbits = [0] * cc.n_clbits
@@ -215,7 +232,7 @@ class BlackwellSimulator:
             torch.manual_seed(seed)
             np.random.seed(seed)
         clean = self._strip_measurements(circuit)
-        state, _, elapsed = self._execute_circuit(clean, mode)
+        state, _, elapsed = self._execute_circuit(clean, mode, graphed=True)
         samples = self.bk.sample(state, shots)
         counts = counts_from_samples(samples.cpu().numpy())
         n = int(state.numel()).bit_length() - 1
@@ -223,7 +240,7 @@ class BlackwellSimulator:
 
     def run_statevector(self, circuit, mode=None):
         """(numpy complex64 statevector, elapsed_s)."""
-        state, _, elapsed = self._execute_circuit(circuit, mode)
+        state, _, elapsed = self._execute_circuit(circuit, mode, graphed=True)
         return state.cpu().numpy(), elapsed
 
     def run_and_time(self, circuit, shots=1024, seed=None, warmup=2, repeats=5, mode=None):
@@ -231,10 +248,10 @@ class BlackwellSimulator:
             torch.manual_seed(seed)
             np.random.seed(seed)
         for _ in range(warmup):
-            self._execute_circuit(circuit, mode)
+            self._execute_circuit(circuit, mode, graphed=True)
         times = []
         for _ in range(repeats):
-            _, _, el = self._execute_circuit(circuit, mode)
+            _, _, el = self._execute_circuit(circuit, mode, graphed=True)
             times.append(el)
         result = self.run(circuit, shots=shots, seed=seed, mode=mode)
         stats = {"times_s": times, "median_s": float(np.median(times)),

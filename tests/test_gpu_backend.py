@@ -232,3 +232,43 @@ def test_run_and_time_stats(sim):
     res, stats = sim.run_and_time(W.c1_rotation_ladder(10, 2), shots=64, warmup=1, repeats=3)
     assert len(stats["times_s"]) == 3 and stats["median_s"] > 0
     assert sum(res.get_counts().values()) == 64
+
+
+def test_graph_replays_match_eager():
+    """run_statevector / run replay HIP graphs of small circuits: every replay,
+    with eager work and synchronisations between replays, gives the eager
+    result bit for bit; plans a graph cannot hold run eagerly."""
+    from blackwell_backend import BlackwellSimulator, workloads as W
+    from blackwell_backend.circuit import QuantumCircuit
+    eager = BlackwellSimulator("cuda:0", use_graphs=False)
+    graphed = BlackwellSimulator("cuda:0", use_graphs=True)
+    other = W.su4_volume(12, 6, 5)
+    mixed = QuantumCircuit(11)
+    mixed.h(0).cx(0, 1).ccx(0, 1, 2).unitary(np.eye(8), [3, 5, 7]).rz(0.4, 9)
+    cases = [(W.su4_volume(14, 14, 3), "auto"), (W.qft(12), "auto"), (W.qft(12), "plain"),
+             (W.c3_rotation_mcx(16, 2, 4), "blocks"), (W.c3_rotation_mcx(12, 2, 9), "tiles"),
+             (mixed, "plain")]
+    for qc, mode in cases:
+        print(f"[graph] {qc.name} mode={mode}", flush=True)
+        ref, _ = eager.run_statevector(qc, mode=mode)
+        for rep in range(3):
+            got, _ = graphed.run_statevector(qc, mode=mode)
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (qc.name, mode, rep)
+            eager.run_statevector(other)                 # eager work between replays
+            torch.cuda.synchronize()
+    cc = graphed.compile(mixed)
+    assert cc.graphed("plain", graphed.device) is None    # 3-qubit dense op: host-uploaded table
+    assert graphed.compile(W.qft(12)).graphed("plain", graphed.device) is not None
+
+
+def test_tune_mode_picks_a_measured_mode(sim):
+    from blackwell_backend import BlackwellSimulator, workloads as W
+    tuned = BlackwellSimulator("cuda:0", mode="tune")
+    qc = W.deep_rz(12, depth=20)
+    sv, _ = tuned.run_statevector(qc)
+    ref = oracle_sv(qc)
+    check(ref, sv)
+    mode = tuned.compile(qc).tuned_mode(tuned.device)
+    assert mode in ("plain", "whole", "blocks")
+    res = tuned.run(qc, shots=1000, seed=3)
+    assert sum(res.get_counts().values()) == 1000
