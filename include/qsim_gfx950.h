@@ -217,6 +217,51 @@ int qsim_sample(void* stream, const void* state, int n_qubits, const void* unifo
  * (new)                                                                  */
 int qsim_uniforms_splitmix(void* stream, uint64_t seed, int64_t count, void* uniforms_dev);
 
+/* ---- Pauli strings, expectation values, noise channels ------------------ */
+/* A Pauli string is (x_mask, z_mask): qubit q carries X if only x bit q is
+ * set, Z if only z bit q, Y if both; P = i^popcount(x&z) X^x Z^z.
+ * Expectations are <psi|P|psi> of the unnormalised state, summed in FP64 in
+ * a fixed order (deterministic), written as doubles to out_dev.            */
+int64_t qsim_expectation_workspace_bytes(int n_qubits);
+/* out_dev[k] = <psi|P_k|psi> for n_terms strings.  Terms with the same
+ * x mask share one pass over the state (up to 8 per pass).  (new; the
+ * reference evaluates one Z string or one X/Y qubit per call)            */
+int qsim_expectation_paulis(void* stream, const void* state, int n_qubits, int64_t n_terms,
+                            const uint64_t* x_masks, const uint64_t* z_masks, void* workspace,
+                            void* out_dev);
+/* <Z on the z_mask qubits>.
+ * Replaces expectation_z, bwk/csrc/expectation_sm120.cu:180-204.         */
+int qsim_expectation_z(void* stream, const void* state, int n_qubits, uint64_t z_mask,
+                       void* workspace, void* out_dev);
+/* <(Z on z_mask) (op on xy_qubit)>, op 0 = I/Z string only, 1 = X, 2 = Y; a
+ * z bit on xy_qubit is ignored.
+ * Replaces expectation_pauli, bwk/csrc/expectation_sm120.cu:206-239.     */
+int qsim_expectation_pauli(void* stream, const void* state, int n_qubits, uint64_t z_mask,
+                           int xy_qubit, int op, void* workspace, void* out_dev);
+/* psi <- P psi, bit-exact (components are moved and negated only).  (new;
+ * the reference's Pauli kernels are internal to its depolarizing channel,
+ * bwk/csrc/noise_sm120.cu:39-101)                                        */
+int qsim_apply_pauli_string(void* stream, void* state, int n_qubits, uint64_t x_mask,
+                            uint64_t z_mask);
+/* Depolarizing channel, one trajectory step: *branch = 0 (identity) when
+ * random_val < 1 - p, else 1/2/3 (X/Y/Z on target) over three intervals of
+ * width p/3; the Pauli is applied bit-exactly.  Does not synchronise.
+ * Replaces apply_depolarizing, bwk/csrc/noise_sm120.cu:238-281.          */
+int qsim_apply_depolarizing(void* stream, void* state, int n_qubits, int target, double p,
+                            double random_val, int* branch);
+/* Amplitude damping, one trajectory step: branch 1 (decay, E1 = [[0,
+ * sqrt(gamma)], [0, 0]]) when random_val < gamma * P(target = 1), else
+ * branch 0 (E0 = diag(1, sqrt(1 - gamma))); the result is renormalised to
+ * unit norm.  Probabilities and scales are FP64.  workspace: see
+ * qsim_reduce_workspace_bytes.  Synchronises the stream.
+ * Replaces apply_amplitude_damping, bwk/csrc/noise_sm120.cu:284-329.     */
+int qsim_apply_amplitude_damping(void* stream, void* state, int n_qubits, int target,
+                                 double gamma, double random_val, void* workspace, int* branch);
+/* Dephasing, as amplitude damping with E1 = diag(0, sqrt(lambda)).
+ * Replaces apply_dephasing, bwk/csrc/noise_sm120.cu:332-373.             */
+int qsim_apply_dephasing(void* stream, void* state, int n_qubits, int target, double lam,
+                         double random_val, void* workspace, int* branch);
+
 /* ---- batched same-size states ------------------------------------------- */
 /* Same 2x2 gate on each of n_states states of 2^n amplitudes.
  * Replaces apply_gate_batched, bwk/csrc/batch_sm120.cu:186-228.          */

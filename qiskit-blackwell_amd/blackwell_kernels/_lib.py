@@ -60,6 +60,14 @@ PROTOTYPES = {
     "qsim_sample_workspace_bytes": (_i64, [_i]),
     "qsim_sample": (_i, [_vp, _vp, _i, _vp, _i64, _vp, _vp]),
     "qsim_uniforms_splitmix": (_i, [_vp, _u64, _i64, _vp]),
+    "qsim_expectation_workspace_bytes": (_i64, [_i]),
+    "qsim_expectation_paulis": (_i, [_vp, _vp, _i, _i64, _vp, _vp, _vp, _vp]),
+    "qsim_expectation_z": (_i, [_vp, _vp, _i, _u64, _vp, _vp]),
+    "qsim_expectation_pauli": (_i, [_vp, _vp, _i, _u64, _i, _i, _vp, _vp]),
+    "qsim_apply_pauli_string": (_i, [_vp, _vp, _i, _u64, _u64]),
+    "qsim_apply_depolarizing": (_i, [_vp, _vp, _i, _i, _d, _d, _pint]),
+    "qsim_apply_amplitude_damping": (_i, [_vp, _vp, _i, _i, _d, _d, _vp, _pint]),
+    "qsim_apply_dephasing": (_i, [_vp, _vp, _i, _i, _d, _d, _vp, _pint]),
     "qsim_apply_gate_batched": (_i, [_vp, _vp, _i64, _i, _vp, _i]),
     "qsim_apply_gates_batched_fused": (_i, [_vp, _vp, _i64, _i, _vp, _vp, _i]),
     "qsim_run_instances": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _i]),

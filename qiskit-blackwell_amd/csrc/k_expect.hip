@@ -1,0 +1,435 @@
+// Pauli-string expectation values, Pauli-string application and the
+// stochastic noise channels (one Kraus branch per call) for gfx950.
+//
+// A Pauli string is given by two masks: x (qubits carrying X or Y) and z
+// (qubits carrying Z or Y).  With y = popcount(x & z) (the number of Y's),
+//   P = i^y X^x Z^z,   (P psi)_i = i^y (-1)^popcount((i ^ x) & z) psi_{i ^ x}.
+// Its expectation pairs every index i0 whose highest x bit is clear with its
+// partner j = i0 ^ x.  With zc = conj(psi_i0) psi_j and s = (-1)^popcount(i0 & z):
+//   <psi|P|psi> = sum_{i0} s * ( (-1)^(y/2)     * 2 Re zc   if y is even,
+//                                (-1)^((y-1)/2) * 2 Im zc   if y is odd ),
+// and for x = 0 it is sum_i s(i) |psi_i|^2.  Products of FP32 amplitudes are
+// exact in FP64; every sum is FP64 in a fixed order (per lane, then a wave
+// butterfly, then the waves of a workgroup, then the workgroups in index
+// order), so results are deterministic.  The reference reduces in FP32 with
+// atomicAdd and handles a Z string or one X/Y qubit only
+// (bwk/csrc/expectation_sm120.cu:45-163).
+//
+// Memory layout follows the gate kernels: a wave owns a window of 64
+// float4 slots (128 amplitudes, 1 KiB).  When x < 128 the partner of every
+// amplitude is in the same window (a cross-lane swap by x >> 1, plus a swap
+// inside the float4 when x & 1); when x >= 128 the wave owns the window pair
+// (w, w ^ (x >> 7)) with bit h - 7 of w clear, h the highest bit of x.  Each
+// amplitude is read once, by whole-line 1 KiB loads.
+#include <algorithm>
+#include <vector>
+
+#include "qsim_internal.hpp"
+
+namespace qsim {
+
+constexpr int kExpMaxTerms = 8;   // terms sharing one x mask per state pass
+
+struct ExpTerms {
+  uint64_t z[kExpMaxTerms];
+  double fac[kExpMaxTerms];       // +-1 (x = 0) or +-2 (x != 0)
+  int out[kExpMaxTerms];          // output slot, -1 = discard (padding)
+  uint32_t im;                    // bit j set: term j takes Im(zc), else Re(zc)
+};
+
+__device__ __forceinline__ double wsum64(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// KIND 0: x == 0.  KIND 1: 0 < x < 128.  KIND 2: x >= 128 (window pairs).
+template <int NT, int KIND>
+__device__ __forceinline__ void expect_slot(double (&acc)[NT], const ExpTerms& a, float4 v, float4 p,
+                                            uint64_t s, int h, int csw) {
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const float2 av = c ? hi2(v) : lo2(v);
+    const uint64_t i = 2 * s + c;
+    if (KIND == 0) {
+      const double q = (double)av.x * av.x + (double)av.y * av.y;
+#pragma unroll
+      for (int j = 0; j < NT; ++j) acc[j] += (__popcll(i & a.z[j]) & 1) ? -q : q;
+    } else {
+      if (KIND == 1 && ((i >> h) & 1)) continue;
+      const float2 bv = (c ^ csw) ? hi2(p) : lo2(p);
+      const double ax = av.x, ay = av.y, bx = bv.x, by = bv.y;
+      const double re = ax * bx + ay * by, im = ax * by - ay * bx;
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const double val = (((a.im >> j) & 1) ? im : re) * a.fac[j];
+        acc[j] += (__popcll(i & a.z[j]) & 1) ? -val : val;
+      }
+    }
+  }
+}
+
+// Pass 1.  Units are windows (KIND 0, 1) or window pairs (KIND 2); workgroup
+// b reduces units [b*chunk, (b+1)*chunk), wave k of it takes units k, k+4, ...
+// part[b*NT + j] = the workgroup's sum for term j.
+template <int NT, int KIND>
+__global__ __launch_bounds__(kThreads) void k_expect(const float4* __restrict__ st, uint64_t n4,
+                                                     uint64_t n_units, uint64_t chunk, uint64_t x,
+                                                     int hw, ExpTerms a, double* __restrict__ part) {
+  __shared__ double sh[NT][4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint64_t beg = (uint64_t)blockIdx.x * chunk;
+  const uint64_t end = beg + chunk < n_units ? beg + chunk : n_units;
+  const int xl = (int)((x >> 1) & 63), csw = (int)(x & 1);
+  const int h = x ? 63 - __clzll(x) : 0;
+  const uint64_t xw = x >> 7;
+  double acc[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) acc[j] = 0.0;
+  constexpr int U = KIND == 2 ? 2 : 4;   // units in flight per wave
+  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (uint64_t u0 = beg + wid; u0 < end; u0 += 4 * U) {
+    float4 v[U], p[U];
+    uint64_t sl[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint64_t u = u0 + 4 * k;
+      const uint64_t w = KIND == 2 ? ins0(u, hw) : u;
+      sl[k] = w * 64 + lane;
+      const bool ok = u < end && sl[k] < n4;
+      v[k] = ok ? ld_nt(st + sl[k]) : zero;
+      if (KIND == 2) p[k] = ok ? ld_nt(st + ((w ^ xw) * 64 + lane)) : zero;
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      if (KIND == 1) p[k] = shfl_xor4(v[k], xl);
+      if (KIND == 2) p[k] = shfl_xor4(p[k], xl);
+      if (KIND == 0) p[k] = v[k];
+      if (u0 + 4 * k < end) expect_slot<NT, KIND>(acc, a, v[k], p[k], sl[k], h, csw);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const double r = wsum64(acc[j]);
+    if (lane == 0) sh[j][wid] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x < NT) {
+    const int j = threadIdx.x;
+    part[(uint64_t)blockIdx.x * NT + j] = (sh[j][0] + sh[j][1]) + (sh[j][2] + sh[j][3]);
+  }
+}
+
+// Pass 2: one workgroup sums each of the NT columns of part[nb][NT] in index
+// order and writes term j to out[a.out[j]] (padding terms are dropped).
+template <int NT>
+__global__ __launch_bounds__(kThreads) void k_expect_final(const double* __restrict__ part, int nb,
+                                                           ExpTerms a, double* __restrict__ out) {
+  __shared__ double sh[NT][4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double acc[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) acc[j] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kThreads) {
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[j] += part[(int64_t)b * NT + j];
+  }
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const double r = wsum64(acc[j]);
+    if (lane == 0) sh[j][wid] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x < NT) {
+    const int j = threadIdx.x;
+    if (a.out[j] >= 0) out[a.out[j]] = (sh[j][0] + sh[j][1]) + (sh[j][2] + sh[j][3]);
+  }
+}
+
+template <int NT, int KIND>
+static void launch_expect(hipStream_t s, const void* state, int n, uint64_t x, const ExpTerms& a,
+                          double* part, double* out) {
+  const uint64_t n4 = (1ull << n) / 2;
+  const uint64_t windows = ceil_div(n4, 64);
+  const uint64_t n_units = KIND == 2 ? windows / 2 : windows;
+  const int hw = KIND == 2 ? (63 - __builtin_clzll(x)) - 7 : 0;
+  const uint64_t want = ceil_div(n_units, 16);   // 4 waves x 4 units per workgroup
+  const int blocks = (int)(want < (uint64_t)kRedBlocks ? (want ? want : 1) : kRedBlocks);
+  const uint64_t chunk = ceil_div(n_units, blocks);
+  hipLaunchKernelGGL((k_expect<NT, KIND>), dim3(blocks), dim3(kThreads), 0, s,
+                     (const float4*)state, n4, n_units, chunk, x, hw, a, part);
+  hipLaunchKernelGGL((k_expect_final<NT>), dim3(1), dim3(kThreads), 0, s, part, blocks, a, out);
+}
+
+template <int NT>
+static void launch_expect_kind(hipStream_t s, const void* state, int n, uint64_t x,
+                               const ExpTerms& a, double* part, double* out) {
+  if (x == 0) launch_expect<NT, 0>(s, state, n, x, a, part, out);
+  else if (x < 128) launch_expect<NT, 1>(s, state, n, x, a, part, out);
+  else launch_expect<NT, 2>(s, state, n, x, a, part, out);
+}
+
+// Workspace (doubles): pass-1 partials [kRedBlocks][kExpMaxTerms].
+constexpr int64_t kExpWsDoubles = (int64_t)kRedBlocks * kExpMaxTerms;
+
+static int expectation_impl(hipStream_t s, const void* state, int n, int64_t n_terms,
+                            const uint64_t* xm, const uint64_t* zm, double* ws, double* out) {
+  if (int rc = check_n(n)) return rc;
+  if (n_terms < 0) return fail(QSIM_EINVAL, "n_terms < 0");
+  if (n_terms == 0) return QSIM_OK;
+  if (!state || !xm || !zm || !ws || !out) return fail(QSIM_EINVAL, "NULL pointer argument");
+  const uint64_t lim = 1ull << n;
+  for (int64_t k = 0; k < n_terms; ++k)
+    if (xm[k] >= lim || zm[k] >= lim)
+      return fail(QSIM_EINVAL, "Pauli term %lld: mask has a qubit >= n_qubits (%d)", (long long)k, n);
+  // terms sharing an x mask share a state pass: visit them grouped by x,
+  // in input order within a group
+  std::vector<int64_t> order(n_terms);
+  for (int64_t k = 0; k < n_terms; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(),
+                   [xm](int64_t a, int64_t b) { return xm[a] < xm[b]; });
+  int rc = QSIM_OK;
+  for (int64_t g = 0; g < n_terms && rc == QSIM_OK;) {
+    const uint64_t x = xm[order[g]];
+    int cnt = 0;
+    ExpTerms a = {};
+    while (g + cnt < n_terms && cnt < kExpMaxTerms && xm[order[g + cnt]] == x) {
+      const int64_t k = order[g + cnt];
+      const int y = __builtin_popcountll(x & zm[k]);
+      a.z[cnt] = zm[k];
+      a.out[cnt] = (int)k;
+      if (x == 0) {
+        a.fac[cnt] = 1.0;
+      } else {
+        a.fac[cnt] = ((y >> 1) & 1) ? -2.0 : 2.0;
+        if (y & 1) a.im |= 1u << cnt;
+      }
+      ++cnt;
+    }
+    for (int j = cnt; j < kExpMaxTerms; ++j) { a.out[j] = -1; a.fac[j] = 0.0; }
+    if (cnt == 1) launch_expect_kind<1>(s, state, n, x, a, ws, out);
+    else if (cnt == 2) launch_expect_kind<2>(s, state, n, x, a, ws, out);
+    else if (cnt <= 4) launch_expect_kind<4>(s, state, n, x, a, ws, out);
+    else launch_expect_kind<8>(s, state, n, x, a, ws, out);
+    rc = launch_status("expectation");
+    g += cnt;
+  }
+  return rc;
+}
+
+// ------------------------------------------------------------ apply P
+// psi <- P psi, exact: every output amplitude is an input amplitude with its
+// components swapped and/or negated.  KIND as for k_expect; one wave per
+// window (KIND 0, 1) or window pair (KIND 2).  KIND 0 with no z bit below 7
+// has one sign per window, so windows whose sign is +1 are not touched.
+__device__ __forceinline__ float2 quarter_turn(float2 a, int k) {
+  switch (k & 3) {
+    case 0: return a;
+    case 1: return make_float2(-a.y, a.x);     // * i
+    case 2: return make_float2(-a.x, -a.y);    // * -1
+    default: return make_float2(a.y, -a.x);    // * -i
+  }
+}
+
+// New slot s from partner float4 p: amplitude i = 2s + c takes
+// i^y (-1)^popcount((i ^ x) & z) psi_{i ^ x}.
+__device__ __forceinline__ float4 pauli_slot(float4 p, uint64_t s, uint64_t x, uint64_t z, int y,
+                                             int csw) {
+  float2 r[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const uint64_t i = 2 * s + c;
+    const float2 b = (c ^ csw) ? hi2(p) : lo2(p);
+    r[c] = quarter_turn(b, y + 2 * (int)(__popcll((i ^ x) & z) & 1));
+  }
+  return cat4(r[0], r[1]);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kThreads) void k_pauli_apply(float4* __restrict__ st, uint64_t n4,
+                                                          uint64_t n_units, uint64_t x, uint64_t z,
+                                                          int y, int hw, int whole_window_sign) {
+  const uint64_t u = xcd_block<uint64_t>() * 4 + (threadIdx.x >> 6);
+  if (u >= n_units) return;
+  const int lane = threadIdx.x & 63;
+  const int xl = (int)((x >> 1) & 63), csw = (int)(x & 1);
+  if (KIND == 2) {
+    const uint64_t w0 = ins0(u, hw), w1 = w0 ^ (x >> 7);
+    const uint64_t s0 = w0 * 64 + lane, s1 = w1 * 64 + lane;
+    const float4 v0 = ld_nt(st + s0), v1 = ld_nt(st + s1);
+    const float4 p0 = shfl_xor4(v1, xl), p1 = shfl_xor4(v0, xl);
+    st_nt(st + s0, pauli_slot(p0, s0, x, z, y, csw));
+    st_nt(st + s1, pauli_slot(p1, s1, x, z, y, csw));
+    return;
+  }
+  const uint64_t s = u * 64 + lane;
+  if (KIND == 0 && whole_window_sign && !(__popcll((u * 128) & z) & 1)) return;
+  const bool ok = s < n4;
+  const float4 v = ok ? ld_nt(st + s) : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 p = KIND == 1 ? shfl_xor4(v, xl) : v;
+  if (ok) st_nt(st + s, pauli_slot(p, s, x, z, y, csw));
+}
+
+static int pauli_apply_impl(hipStream_t s, void* state, int n, uint64_t x, uint64_t z) {
+  if (int rc = check_n(n)) return rc;
+  if (!state) return fail(QSIM_EINVAL, "state is NULL");
+  const uint64_t lim = 1ull << n;
+  if (x >= lim || z >= lim) return fail(QSIM_EINVAL, "Pauli mask has a qubit >= n_qubits (%d)", n);
+  if (x == 0 && z == 0) return QSIM_OK;
+  const uint64_t n4 = (1ull << n) / 2;
+  const uint64_t windows = ceil_div(n4, 64);
+  const int y = __builtin_popcountll(x & z) & 3;
+  if (x == 0) {
+    const int wws = (z & 127) == 0;
+    hipLaunchKernelGGL(k_pauli_apply<0>, dim3(ceil_div(windows, 4)), dim3(kThreads), 0, s,
+                       (float4*)state, n4, windows, x, z, y, 0, wws);
+  } else if (x < 128) {
+    hipLaunchKernelGGL(k_pauli_apply<1>, dim3(ceil_div(windows, 4)), dim3(kThreads), 0, s,
+                       (float4*)state, n4, windows, x, z, y, 0, 0);
+  } else {
+    const int hw = (63 - __builtin_clzll(x)) - 7;
+    hipLaunchKernelGGL(k_pauli_apply<2>, dim3(ceil_div(windows / 2, 4)), dim3(kThreads), 0, s,
+                       (float4*)state, n4, windows / 2, x, z, y, hw, 0);
+  }
+  return launch_status("apply_pauli_string");
+}
+
+// ------------------------------------------------------------ damping channels
+// One Kraus branch of amplitude damping (kind 0: E0 = diag(1, sqrt(1-g)),
+// E1 = [[0, sqrt(g)], [0, 0]]) or dephasing (kind 1: E0 as before,
+// E1 = diag(0, sqrt(g))) on qubit t.  From res = (total, S0) of the FP64
+// reduction, S1 = total - S0 is the weight with bit t = 1; branch 1 has
+// probability g * S1 / total.  The chosen branch is written as a 2x2 matrix
+// that also renormalises the state to unit norm (E0: 1/sqrt(S0 + (1-g) S1);
+// E1: 1/sqrt(g S1) times sqrt(g)), all in FP64, and applied by the dense
+// 2x2 gate kernel.
+__global__ void k_channel_branch(const double* __restrict__ res, int kind, double g, double u,
+                                 float2* __restrict__ mat, int* __restrict__ branch) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double tot = res[0], s0 = res[1];
+  const double s1 = tot - s0 > 0.0 ? tot - s0 : 0.0;
+  const double p1 = tot > 0.0 ? g * s1 / tot : 0.0;
+  const int b = u < p1 ? 1 : 0;
+  float2 m[4];
+  for (int k = 0; k < 4; ++k) m[k] = make_float2(0.f, 0.f);
+  if (b == 0) {
+    const double nrm = s0 + (1.0 - g) * s1;
+    const double r = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
+    m[0] = make_float2((float)r, 0.f);
+    m[3] = make_float2((float)(r * sqrt(1.0 - g)), 0.f);
+  } else {
+    const double r = s1 > 0.0 ? 1.0 / sqrt(s1) : 0.0;
+    m[kind == 0 ? 1 : 3] = make_float2((float)r, 0.f);
+  }
+  for (int k = 0; k < 4; ++k) mat[k] = m[k];
+  *branch = b;
+}
+
+static int damping_impl(void* stream, void* state, int n, int t, double g, double u, void* workspace,
+                        int* branch, int kind, const char* what) {
+  if (int rc = check_n(n)) return rc;
+  if (int rc = check_q(t, n, "target_qubit")) return rc;
+  if (!state || !workspace || !branch) return fail(QSIM_EINVAL, "NULL pointer argument");
+  if (!(g >= 0.0 && g <= 1.0)) return fail(QSIM_EINVAL, "%s: probability %g outside [0, 1]", what, g);
+  hipStream_t s = as_stream(stream);
+  double* ws = (double*)workspace;
+  if (int rc = reduce_impl(s, state, n, 1, t, ws)) return rc;
+  // the pass-1 partials are consumed by pass 2, so their space holds the
+  // branch matrix; the branch index goes next to the results
+  float2* mat = (float2*)ws;
+  int* dev_branch = (int*)(ws + 2 * kRedBlocks + 4);
+  hipLaunchKernelGGL(k_channel_branch, dim3(1), dim3(64), 0, s, ws + 2 * kRedBlocks, kind, g, u,
+                     mat, dev_branch);
+  if (int rc = launch_status(what)) return rc;
+  M2 unused = {};
+  if (int rc = gate1q_impl(stream, state, 1, n, mat, unused, t)) return rc;
+  int host = -1;
+  if (hipMemcpyAsync(&host, dev_branch, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess)
+    return fail(QSIM_EHIP, "%s: copy failed", what);
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(QSIM_EHIP, "%s: sync failed", what);
+  *branch = host;
+  return QSIM_OK;
+}
+
+}  // namespace qsim
+
+using namespace qsim;
+
+extern "C" {
+
+int64_t qsim_expectation_workspace_bytes(int n_qubits) {
+  (void)n_qubits;
+  return kExpWsDoubles * (int64_t)sizeof(double);
+}
+
+int qsim_expectation_paulis(void* stream, const void* state, int n_qubits, int64_t n_terms,
+                            const uint64_t* x_masks, const uint64_t* z_masks, void* workspace,
+                            void* out_dev) {
+  clear_error();
+  return expectation_impl(as_stream(stream), state, n_qubits, n_terms, x_masks, z_masks,
+                          (double*)workspace, (double*)out_dev);
+}
+
+int qsim_expectation_z(void* stream, const void* state, int n_qubits, uint64_t z_mask,
+                       void* workspace, void* out_dev) {
+  clear_error();
+  const uint64_t x = 0;
+  return expectation_impl(as_stream(stream), state, n_qubits, 1, &x, &z_mask, (double*)workspace,
+                          (double*)out_dev);
+}
+
+int qsim_expectation_pauli(void* stream, const void* state, int n_qubits, uint64_t z_mask,
+                           int xy_qubit, int op, void* workspace, void* out_dev) {
+  clear_error();
+  if (op < 0 || op > 2) return fail(QSIM_EINVAL, "op must be 0=I/Z, 1=X, 2=Y (got %d)", op);
+  if (op == 0) return qsim_expectation_z(stream, state, n_qubits, z_mask, workspace, out_dev);
+  if (int rc = check_n(n_qubits)) return rc;
+  if (int rc = check_q(xy_qubit, n_qubits, "xy_qubit")) return rc;
+  // the Z string acts on the other qubits: a z bit on xy_qubit is ignored
+  const uint64_t bit = 1ull << xy_qubit;
+  const uint64_t x = bit, z = (z_mask & ~bit) | (op == 2 ? bit : 0);
+  return expectation_impl(as_stream(stream), state, n_qubits, 1, &x, &z, (double*)workspace,
+                          (double*)out_dev);
+}
+
+int qsim_apply_pauli_string(void* stream, void* state, int n_qubits, uint64_t x_mask,
+                            uint64_t z_mask) {
+  clear_error();
+  return pauli_apply_impl(as_stream(stream), state, n_qubits, x_mask, z_mask);
+}
+
+int qsim_apply_depolarizing(void* stream, void* state, int n_qubits, int target, double p,
+                            double random_val, int* branch) {
+  clear_error();
+  if (int rc = check_n(n_qubits)) return rc;
+  if (int rc = check_q(target, n_qubits, "target_qubit")) return rc;
+  if (!state || !branch) return fail(QSIM_EINVAL, "NULL pointer argument");
+  if (!(p >= 0.0 && p <= 1.0)) return fail(QSIM_EINVAL, "depolarizing: p = %g outside [0, 1]", p);
+  // identity on [0, 1-p); then X, Y, Z on three intervals of width p/3
+  int b = 0;
+  if (!(random_val < 1.0 - p)) {
+    const double r = random_val - (1.0 - p), third = p / 3.0;
+    b = r < third ? 1 : (r < 2.0 * third ? 2 : 3);
+  }
+  *branch = b;
+  if (b == 0) return QSIM_OK;
+  const uint64_t bit = 1ull << target;
+  const uint64_t x = b == 3 ? 0 : bit, z = b == 1 ? 0 : bit;
+  return pauli_apply_impl(as_stream(stream), state, n_qubits, x, z);
+}
+
+int qsim_apply_amplitude_damping(void* stream, void* state, int n_qubits, int target, double gamma,
+                                 double random_val, void* workspace, int* branch) {
+  clear_error();
+  return damping_impl(stream, state, n_qubits, target, gamma, random_val, workspace, branch, 0,
+                      "amplitude_damping");
+}
+
+int qsim_apply_dephasing(void* stream, void* state, int n_qubits, int target, double lam,
+                         double random_val, void* workspace, int* branch) {
+  clear_error();
+  return damping_impl(stream, state, n_qubits, target, lam, random_val, workspace, branch, 1,
+                      "dephasing");
+}
+
+}  // extern "C"

@@ -24,7 +24,6 @@
 
 namespace qsim {
 
-constexpr int kRedBlocks = 1024;   // pass-1 workgroups for large states
 constexpr int kRedUnroll = 4;      // slots in flight per lane in pass 1
 constexpr int kLeaf = 256;         // amplitudes per sampling leaf
 constexpr int kScanSeg = 4096;     // leaf totals per scan segment
@@ -127,10 +126,7 @@ static RedPlan red_plan(int n) {
   return p;
 }
 
-// Workspace layout (doubles): [0, 2*kRedBlocks) partials, then res[4].
-constexpr int64_t kRedWsDoubles = 2 * kRedBlocks + 8;
-
-static int reduce_impl(hipStream_t s, const void* state, int n, int mode, int t, double* ws) {
+int reduce_impl(hipStream_t s, const void* state, int n, int mode, int t, double* ws) {
   const RedPlan p = red_plan(n);
   double* res = ws + 2 * kRedBlocks;
   if (mode == 0)

@@ -33,6 +33,19 @@ int check_q(int q, int n_qubits, const char* what);
 
 constexpr int kThreads = 256;
 
+// FP64 two-pass reduction of |a|^2 (k_reduce.hip).  Workspace (doubles):
+// [0, 2*kRedBlocks) pass-1 partials, then res[8]: res[0] = total and, for
+// mode 1, res[1] = the part with bit t = 0.  Launches only (asynchronous).
+constexpr int kRedBlocks = 1024;
+constexpr int64_t kRedWsDoubles = 2 * kRedBlocks + 8;
+int reduce_impl(hipStream_t s, const void* state, int n, int mode, int t, double* ws);
+
+// Dense 2x2 gate on qubit t of n_states contiguous states (k_gates.hip); the
+// matrix comes from gdev (device complex64[4]) when non-NULL, else from gv.
+struct M2;
+int gate1q_impl(void* stream, void* states, uint64_t n_states, int n, const float2* gdev,
+                const M2& gv, int t);
+
 inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 // Sorted affected-qubit list plus control mask for the multi-controlled kernels.

@@ -97,6 +97,22 @@ def test_meta_and_validation_without_device():
                                  ctypes.c_void_p(8)) == 1                 # mask without qubit 0
     assert lib.qsim_run_instances(None, None, ctypes.c_void_p(8), 1, ctypes.c_void_p(8),
                                   ctypes.c_void_p(8), None, 15) == 2
+    # Pauli strings and noise channels
+    v8 = ctypes.c_void_p(8)
+    assert lib.qsim_expectation_pauli(None, v8, 4, 0, 1, 3, v8, v8) == 1
+    assert b"op must be" in lib.qsim_last_error()
+    assert lib.qsim_expectation_pauli(None, v8, 4, 0, 4, 1, v8, v8) == 1
+    xm = (ctypes.c_uint64 * 2)(1, 16)
+    zm = (ctypes.c_uint64 * 2)(0, 0)
+    assert lib.qsim_expectation_paulis(None, v8, 4, 2, xm, zm, v8, v8) == 1
+    assert b"term 1" in lib.qsim_last_error()
+    assert lib.qsim_apply_pauli_string(None, v8, 4, 0, 32) == 1
+    b = ctypes.c_int(-1)
+    assert lib.qsim_apply_depolarizing(None, v8, 4, 0, 1.5, 0.5, ctypes.byref(b)) == 1
+    assert lib.qsim_apply_amplitude_damping(None, v8, 4, 0, -0.1, 0.5, v8, ctypes.byref(b)) == 1
+    assert lib.qsim_apply_dephasing(None, v8, 4, 4, 0.1, 0.5, v8, ctypes.byref(b)) == 1
+    assert b"target_qubit" in lib.qsim_last_error()
+    assert lib.qsim_expectation_workspace_bytes(20) >= 8 * 1024 * 8
 
 
 def test_python_layer_raises_runtime_error():
@@ -104,5 +120,7 @@ def test_python_layer_raises_runtime_error():
     import blackwell_kernels as bk
     with pytest.raises(RuntimeError):
         bk.apply_gate(torch.zeros(4, dtype=torch.complex64), bk.pauli_x_gate("cpu"), 0)
-    with pytest.raises(NotImplementedError):
-        bk.expectation_z(None, 1)
+    with pytest.raises(RuntimeError):
+        bk.expectation_z(torch.zeros(4, dtype=torch.complex64), 1)
+    with pytest.raises(RuntimeError):
+        bk.apply_depolarizing(torch.zeros(4, dtype=torch.complex64), 0, 0.1, 0.5)

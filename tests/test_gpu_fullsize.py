@@ -124,3 +124,31 @@ def test_sampler_32q_support_and_determinism():
     assert set(vals.tolist()) <= {0, (1 << n) - 1, 1 << 31}
     frac = {int(v): int(c) / (1 << 20) for v, c in zip(vals.tolist(), cnt.tolist())}
     assert abs(frac.get(0, 0) - 0.5) < 0.005 and abs(frac.get(1 << 31, 0) - 0.25) < 0.005
+
+
+def test_pauli_conjugation_30q():
+    """<Q psi|P|Q psi> = (-1)^[P,Q] <psi|P|psi> on an 8 GiB state, and Q twice
+    restores psi bit for bit."""
+    import blackwell_kernels as bk
+    n = 30
+    print("[fullsize] Pauli conjugation 30q", flush=True)
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(11)
+    st = torch.randn(1 << n, dtype=torch.complex64, device="cuda:0", generator=g)
+    ref = st.clone()
+    rng = np.random.default_rng(12)
+    xs = [0, 0, 1 << 29, 0b101 << 12, (1 << 30) - 1] + [int(v) for v in rng.integers(0, 1 << n, 5)]
+    zs = [1 << 5, (1 << 30) - 1, 1 << 29, 0b11 << 12, 0] + [int(v) for v in rng.integers(0, 1 << n, 5)]
+    e0 = bk.expectation_paulis(st, xs, zs).cpu().numpy()
+    qx, qz = (1 << 29) | (1 << 3) | 1, (1 << 17) | (1 << 3)
+    bk.apply_pauli_string(st, qx, qz)
+    e1 = bk.expectation_paulis(st, xs, zs).cpu().numpy()
+    bk.apply_pauli_string(st, qx, qz)
+    same = bool(torch.equal(st.view(torch.int32), ref.view(torch.int32)))
+    del st, ref
+    torch.cuda.empty_cache()
+    sign = np.array([(-1) ** ((bin(x & qz).count("1") + bin(z & qx).count("1")) & 1)
+                     for x, z in zip(xs, zs)])
+    scale = float(1 << n)   # |<P>| <= ||psi||^2 ~ 2^n for this unnormalised state
+    assert same
+    assert np.max(np.abs(e1 - sign * e0)) <= 1e-12 * scale, (e0, e1, sign)
