@@ -175,6 +175,59 @@ def per_gate_sweep(n=30, reps=4):
     return out
 
 
+def observables_sweep(n=30, reps=4):
+    """GB/s of the Pauli-string expectation, Pauli application and noise
+    channel kernels on an n-qubit state, plus the grouped energy of the
+    22-qubit chain Hamiltonian (85 Pauli terms)."""
+    import blackwell_kernels as bk
+    from blackwell_backend import workloads as W
+    from blackwell_backend.hamiltonian import expectation
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    N = 1 << n
+    g = torch.Generator(device=dev)
+    g.manual_seed(3)
+    st = torch.randn(N, dtype=torch.complex64, device=dev, generator=g)
+    bk.renormalize(st)
+    z8 = [(0b11 << (3 * k)) for k in range(8)]
+    cases = [
+        ("expect_z_string", lambda: bk.expectation_paulis(st, [0], [0b1011 << 20]), 8 * N, 1),
+        ("expect_x_low_y", lambda: bk.expectation_paulis(st, [0b101], [0b110]), 8 * N, 1),
+        ("expect_xy_high", lambda: bk.expectation_paulis(st, [(1 << 29) | 1], [1 << 29]), 8 * N, 1),
+        ("expect_8_z_terms_one_pass", lambda: bk.expectation_paulis(st, [0] * 8, z8), 8 * N, 8),
+        ("apply_pauli_x_low", lambda: bk.apply_pauli_string(st, 0b1010, 0b11), 16 * N, 1),
+        ("apply_pauli_xy_high", lambda: bk.apply_pauli_string(st, 1 << 28, 1 << 28), 16 * N, 1),
+        ("apply_pauli_z_high", lambda: bk.apply_pauli_string(st, 0, 1 << 25), 8 * N, 1),
+        ("depolarizing_y_t17", lambda: bk.apply_depolarizing(st, 17, 0.3, 0.85), 16 * N, 1),
+        ("amplitude_damping_t9", lambda: bk.apply_amplitude_damping(st, 9, 0.05, 0.99), 24 * N, 1),
+    ]
+    out = {}
+    for name, fn, nbytes, terms in cases:
+        fn()
+        torch.cuda.synchronize()
+        with EventTimer(stream) as ev:
+            for _ in range(reps):
+                fn()
+        ms = ev.ms() / reps
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        row = {"ms": round(ms, 4), "GB_per_s": round(gbs, 1), "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+        if terms > 1:
+            row["terms_per_s"] = terms / (ms * 1e-3)
+        out[name] = row
+    del st
+    torch.cuda.empty_cache()
+    h = W.heisenberg_chain(22, seed=5)
+    psi = torch.randn(1 << 22, dtype=torch.complex64, device=dev, generator=g)
+    bk.renormalize(psi)
+    expectation(psi, h)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e = expectation(psi, h)
+    dt = time.perf_counter() - t0
+    out["energy_chain_22q"] = {"terms": len(h.labels), "ms": round(dt * 1e3, 3), "energy": e}
+    return out
+
+
 class StepEvents:
     """step_timer for _Plan.run: a pair of HIP events around each step on
     the launch stream, kept for aggregation after one synchronize."""
@@ -439,6 +492,7 @@ SECTIONS = {
     "c3": ("c3_rotation_mcx_32q", lambda: c3_bench()),
     "c5": ("c5_chebyshev_22q", lambda: c5_bench()),
     "serving": ("serving", lambda: serving_bench()),
+    "observables": ("observables_30q", lambda: observables_sweep(30)),
 }
 
 
@@ -532,7 +586,7 @@ def main():
     except Exception as e:  # reported, not hidden
         line["parity"] = {"error": repr(e)}
     if not args.no_extras:
-        for name in ("per_gate", "ab", "c3", "c5", "serving"):
+        for name in ("per_gate", "observables", "ab", "c3", "c5", "serving"):
             key, fn = SECTIONS[name]
             try:
                 log(f"[bench] {key}")

@@ -4,7 +4,7 @@ reference exports BlackwellSimulator; its hybrid Aer router is not part of
 this package)."""
 from .simulator import BlackwellSimulator
 from .circuit import QuantumCircuit
-from .hamiltonian import HamiltonianCSR, SparsePauliOp
+from .hamiltonian import HamiltonianCSR, SparsePauliOp, expectation
 from .chebyshev import chebyshev_coefficients, chebyshev_evolve, chebyshev_evolve_batch
 
 __all__ = ["BlackwellSimulator", "QuantumCircuit", "HamiltonianCSR", "SparsePauliOp",

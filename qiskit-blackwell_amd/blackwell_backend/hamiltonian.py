@@ -133,3 +133,34 @@ class HamiltonianCSR:
     def __repr__(self):
         return (f"HamiltonianCSR(n_qubits={self.n_qubits}, dim={self.n_rows}, nnz={self.nnz}, "
                 f"spectral_bound={self.spectral_bound:.4f}, memory={self.memory_bytes()/1024:.1f} KB)")
+
+
+def pauli_masks(labels):
+    """(x masks, z masks) of Pauli labels (rightmost character = qubit 0):
+    X sets x, Z sets z, Y sets both."""
+    xs, zs = [], []
+    for lab in labels:
+        xm = zm = 0
+        for pos, ch in enumerate(reversed(lab)):
+            if ch in "XY":
+                xm |= 1 << pos
+            if ch in "ZY":
+                zm |= 1 << pos
+        xs.append(xm)
+        zs.append(zm)
+    return xs, zs
+
+
+def expectation(state, spo):
+    """<psi|H|psi> = sum_k c_k <psi|P_k|psi> for a SparsePauliOp (this
+    module's or qiskit's) on a device state, unnormalised.  Each <P_k> is an
+    FP64 device reduction; terms sharing an X mask share one pass over the
+    state.  Returns a float when every coefficient is real, else a complex."""
+    from blackwell_kernels import expectation_paulis
+    labels, coeffs, n = _labels_and_coeffs(spo)
+    if int(state.numel()) != 1 << n:
+        raise ValueError(f"state has {state.numel()} amplitudes, operator acts on {n} qubits")
+    xs, zs = pauli_masks(labels)
+    vals = expectation_paulis(state, xs, zs).cpu().numpy()
+    total = complex(np.sum(coeffs * vals))
+    return total.real if not np.any(coeffs.imag) else total

@@ -34,7 +34,7 @@ struct ExpTerms {
   uint64_t z[kExpMaxTerms];
   double fac[kExpMaxTerms];       // +-1 (x = 0) or +-2 (x != 0)
   int out[kExpMaxTerms];          // output slot, -1 = discard (padding)
-  uint32_t im;                    // bit j set: term j takes Im(zc), else Re(zc)
+  int im;                         // every term of the pass takes Im(zc) (else Re(zc))
 };
 
 __device__ __forceinline__ double wsum64(double v) {
@@ -43,69 +43,83 @@ __device__ __forceinline__ double wsum64(double v) {
   return v;
 }
 
-// KIND 0: x == 0.  KIND 1: 0 < x < 128.  KIND 2: x >= 128 (window pairs).
-template <int NT, int KIND>
-__device__ __forceinline__ void expect_slot(double (&acc)[NT], const ExpTerms& a, float4 v, float4 p,
-                                            uint64_t s, int h, int csw) {
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const float2 av = c ? hi2(v) : lo2(v);
-    const uint64_t i = 2 * s + c;
-    if (KIND == 0) {
-      const double q = (double)av.x * av.x + (double)av.y * av.y;
-#pragma unroll
-      for (int j = 0; j < NT; ++j) acc[j] += (__popcll(i & a.z[j]) & 1) ? -q : q;
-    } else {
-      if (KIND == 1 && ((i >> h) & 1)) continue;
-      const float2 bv = (c ^ csw) ? hi2(p) : lo2(p);
-      const double ax = av.x, ay = av.y, bx = bv.x, by = bv.y;
-      const double re = ax * bx + ay * by, im = ax * by - ay * bx;
-#pragma unroll
-      for (int j = 0; j < NT; ++j) {
-        const double val = (((a.im >> j) & 1) ? im : re) * a.fac[j];
-        acc[j] += (__popcll(i & a.z[j]) & 1) ? -val : val;
-      }
-    }
-  }
-}
-
-// Pass 1.  Units are windows (KIND 0, 1) or window pairs (KIND 2); workgroup
-// b reduces units [b*chunk, (b+1)*chunk), wave k of it takes units k, k+4, ...
-// part[b*NT + j] = the workgroup's sum for term j.
+// Pass 1.  Units are windows (KIND 0: x == 0; KIND 1: 0 < x < 128) or window
+// pairs (KIND 2: x >= 128); workgroup b reduces units [b*chunk, (b+1)*chunk),
+// wave k of it takes units k, k+4, ...  part[b*NT + j] = the workgroup's sum
+// for term j.
+//
+// Amplitude i = 128 w + 2 lane + c (window w, lane, c = 0/1 in the float4),
+// so the sign (-1)^popcount(i & z) factors into a lane constant (bits 0..6,
+// folded with the term's factor and, for KIND 1, with the "highest x bit
+// clear" selection into k_c) and a window sign that is the same for the
+// whole wave (computed on the scalar unit).  Per slot and term that leaves
+// t = v_0 k_0 + v_1 k_1 and acc += ws * t, with v_c = |a_c|^2 (KIND 0) or
+// Re / Im of conj(a_c) * partner_c.
 template <int NT, int KIND>
 __global__ __launch_bounds__(kThreads) void k_expect(const float4* __restrict__ st, uint64_t n4,
                                                      uint64_t n_units, uint64_t chunk, uint64_t x,
                                                      int hw, ExpTerms a, double* __restrict__ part) {
   __shared__ double sh[NT][4];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint64_t beg = (uint64_t)blockIdx.x * chunk;
   const uint64_t end = beg + chunk < n_units ? beg + chunk : n_units;
   const int xl = (int)((x >> 1) & 63), csw = (int)(x & 1);
   const int h = x ? 63 - __clzll(x) : 0;
   const uint64_t xw = x >> 7;
-  double acc[NT];
+  double acc[NT], k0[NT], k1[NT];
 #pragma unroll
-  for (int j = 0; j < NT; ++j) acc[j] = 0.0;
+  for (int j = 0; j < NT; ++j) {
+    acc[j] = 0.0;
+    const uint32_t zl = (uint32_t)(a.z[j] & 127);
+    const uint32_t i0 = 2 * lane, i1 = 2 * lane + 1;
+    const bool t0 = KIND != 1 || !((i0 >> h) & 1), t1 = KIND != 1 || !((i1 >> h) & 1);
+    k0[j] = t0 ? ((__popc(i0 & zl) & 1) ? -a.fac[j] : a.fac[j]) : 0.0;
+    k1[j] = t1 ? ((__popc(i1 & zl) & 1) ? -a.fac[j] : a.fac[j]) : 0.0;
+  }
   constexpr int U = KIND == 2 ? 2 : 4;   // units in flight per wave
   const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
   for (uint64_t u0 = beg + wid; u0 < end; u0 += 4 * U) {
     float4 v[U], p[U];
-    uint64_t sl[U];
+    uint64_t wk[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const uint64_t u = u0 + 4 * k;
-      const uint64_t w = KIND == 2 ? ins0(u, hw) : u;
-      sl[k] = w * 64 + lane;
-      const bool ok = u < end && sl[k] < n4;
-      v[k] = ok ? ld_nt(st + sl[k]) : zero;
-      if (KIND == 2) p[k] = ok ? ld_nt(st + ((w ^ xw) * 64 + lane)) : zero;
+      wk[k] = KIND == 2 ? ins0(u, hw) : u;
+      const uint64_t sl = wk[k] * 64 + lane;
+      const bool ok = u < end && sl < n4;
+      v[k] = ok ? ld_nt(st + sl) : zero;
+      if (KIND == 2) p[k] = ok ? ld_nt(st + ((wk[k] ^ xw) * 64 + lane)) : zero;
     }
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       if (KIND == 1) p[k] = shfl_xor4(v[k], xl);
       if (KIND == 2) p[k] = shfl_xor4(p[k], xl);
-      if (KIND == 0) p[k] = v[k];
-      if (u0 + 4 * k < end) expect_slot<NT, KIND>(acc, a, v[k], p[k], sl[k], h, csw);
+      if (u0 + 4 * k >= end) continue;
+      double v0, v1;
+      if (KIND == 0) {
+        v0 = (double)v[k].x * v[k].x + (double)v[k].y * v[k].y;
+        v1 = (double)v[k].z * v[k].z + (double)v[k].w * v[k].w;
+      } else {
+        const float2 a0 = lo2(v[k]), a1 = hi2(v[k]);
+        const float2 b0 = csw ? hi2(p[k]) : lo2(p[k]), b1 = csw ? lo2(p[k]) : hi2(p[k]);
+        if (a.im) {
+          v0 = (double)a0.x * b0.y - (double)a0.y * b0.x;
+          v1 = (double)a1.x * b1.y - (double)a1.y * b1.x;
+        } else {
+          v0 = (double)a0.x * b0.x + (double)a0.y * b0.y;
+          v1 = (double)a1.x * b1.x + (double)a1.y * b1.y;
+        }
+      }
+      const uint64_t wb = wk[k] * 128;
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const double ws = (__popcll(wb & a.z[j]) & 1) ? -1.0 : 1.0;
+        // k and ws are 0, +-1 or +-2, so the products are exact and the
+        // fused forms round exactly as the separate ones would
+        const double t = __builtin_fma(v1, k1[j], v0 * k0[j]);
+        acc[j] = __builtin_fma(ws, t, acc[j]);
+      }
     }
   }
 #pragma unroll
@@ -182,28 +196,29 @@ static int expectation_impl(hipStream_t s, const void* state, int n, int64_t n_t
   for (int64_t k = 0; k < n_terms; ++k)
     if (xm[k] >= lim || zm[k] >= lim)
       return fail(QSIM_EINVAL, "Pauli term %lld: mask has a qubit >= n_qubits (%d)", (long long)k, n);
-  // terms sharing an x mask share a state pass: visit them grouped by x,
-  // in input order within a group
+  // terms sharing an x mask and the parity of their Y count (Re or Im of
+  // the pair products) share a state pass: visit them grouped, in input
+  // order within a group
   std::vector<int64_t> order(n_terms);
   for (int64_t k = 0; k < n_terms; ++k) order[k] = k;
+  auto key = [xm, zm](int64_t k) {
+    return std::make_pair(xm[k], __builtin_popcountll(xm[k] & zm[k]) & 1);
+  };
   std::stable_sort(order.begin(), order.end(),
-                   [xm](int64_t a, int64_t b) { return xm[a] < xm[b]; });
+                   [&key](int64_t a, int64_t b) { return key(a) < key(b); });
   int rc = QSIM_OK;
   for (int64_t g = 0; g < n_terms && rc == QSIM_OK;) {
-    const uint64_t x = xm[order[g]];
+    const auto k0 = key(order[g]);
+    const uint64_t x = k0.first;
     int cnt = 0;
     ExpTerms a = {};
-    while (g + cnt < n_terms && cnt < kExpMaxTerms && xm[order[g + cnt]] == x) {
+    a.im = k0.second;
+    while (g + cnt < n_terms && cnt < kExpMaxTerms && key(order[g + cnt]) == k0) {
       const int64_t k = order[g + cnt];
       const int y = __builtin_popcountll(x & zm[k]);
       a.z[cnt] = zm[k];
       a.out[cnt] = (int)k;
-      if (x == 0) {
-        a.fac[cnt] = 1.0;
-      } else {
-        a.fac[cnt] = ((y >> 1) & 1) ? -2.0 : 2.0;
-        if (y & 1) a.im |= 1u << cnt;
-      }
+      a.fac[cnt] = x == 0 ? 1.0 : (((y >> 1) & 1) ? -2.0 : 2.0);
       ++cnt;
     }
     for (int j = cnt; j < kExpMaxTerms; ++j) { a.out[j] = -1; a.fac[j] = 0.0; }
@@ -220,8 +235,10 @@ static int expectation_impl(hipStream_t s, const void* state, int n, int64_t n_t
 // ------------------------------------------------------------ apply P
 // psi <- P psi, exact: every output amplitude is an input amplitude with its
 // components swapped and/or negated.  KIND as for k_expect; one wave per
-// window (KIND 0, 1) or window pair (KIND 2).  KIND 0 with no z bit below 7
-// has one sign per window, so windows whose sign is +1 are not touched.
+// window (KIND 0, 1) or window pair (KIND 2).  KIND 3 is x = 0 with no z bit
+// below 7: the sign is one per window, (-1)^popcount(w & (z >> 7)), so only
+// the half of the windows whose sign is -1 is visited (unit u inserts the
+// highest bit hw of z >> 7 into u, set so that the parity is odd).
 __device__ __forceinline__ float2 quarter_turn(float2 a, int k) {
   switch (k & 3) {
     case 0: return a;
@@ -248,10 +265,18 @@ __device__ __forceinline__ float4 pauli_slot(float4 p, uint64_t s, uint64_t x, u
 template <int KIND>
 __global__ __launch_bounds__(kThreads) void k_pauli_apply(float4* __restrict__ st, uint64_t n4,
                                                           uint64_t n_units, uint64_t x, uint64_t z,
-                                                          int y, int hw, int whole_window_sign) {
+                                                          int y, int hw) {
   const uint64_t u = xcd_block<uint64_t>() * 4 + (threadIdx.x >> 6);
   if (u >= n_units) return;
   const int lane = threadIdx.x & 63;
+  if (KIND == 3) {
+    const uint64_t w0 = ins0(u, hw);
+    const uint64_t w = (__popcll(w0 & (z >> 7)) & 1) ? w0 : (w0 | (1ull << hw));
+    const uint64_t s = w * 64 + lane;
+    const float4 v = ld_nt(st + s);
+    st_nt(st + s, make_float4(-v.x, -v.y, -v.z, -v.w));
+    return;
+  }
   const int xl = (int)((x >> 1) & 63), csw = (int)(x & 1);
   if (KIND == 2) {
     const uint64_t w0 = ins0(u, hw), w1 = w0 ^ (x >> 7);
@@ -263,7 +288,6 @@ __global__ __launch_bounds__(kThreads) void k_pauli_apply(float4* __restrict__ s
     return;
   }
   const uint64_t s = u * 64 + lane;
-  if (KIND == 0 && whole_window_sign && !(__popcll((u * 128) & z) & 1)) return;
   const bool ok = s < n4;
   const float4 v = ok ? ld_nt(st + s) : make_float4(0.f, 0.f, 0.f, 0.f);
   const float4 p = KIND == 1 ? shfl_xor4(v, xl) : v;
@@ -279,17 +303,20 @@ static int pauli_apply_impl(hipStream_t s, void* state, int n, uint64_t x, uint6
   const uint64_t n4 = (1ull << n) / 2;
   const uint64_t windows = ceil_div(n4, 64);
   const int y = __builtin_popcountll(x & z) & 3;
-  if (x == 0) {
-    const int wws = (z & 127) == 0;
+  if (x == 0 && (z & 127) == 0) {
+    const int hw = (63 - __builtin_clzll(z >> 7));
+    hipLaunchKernelGGL(k_pauli_apply<3>, dim3(ceil_div(windows / 2, 4)), dim3(kThreads), 0, s,
+                       (float4*)state, n4, windows / 2, x, z, y, hw);
+  } else if (x == 0) {
     hipLaunchKernelGGL(k_pauli_apply<0>, dim3(ceil_div(windows, 4)), dim3(kThreads), 0, s,
-                       (float4*)state, n4, windows, x, z, y, 0, wws);
+                       (float4*)state, n4, windows, x, z, y, 0);
   } else if (x < 128) {
     hipLaunchKernelGGL(k_pauli_apply<1>, dim3(ceil_div(windows, 4)), dim3(kThreads), 0, s,
-                       (float4*)state, n4, windows, x, z, y, 0, 0);
+                       (float4*)state, n4, windows, x, z, y, 0);
   } else {
     const int hw = (63 - __builtin_clzll(x)) - 7;
     hipLaunchKernelGGL(k_pauli_apply<2>, dim3(ceil_div(windows / 2, 4)), dim3(kThreads), 0, s,
-                       (float4*)state, n4, windows / 2, x, z, y, hw, 0);
+                       (float4*)state, n4, windows / 2, x, z, y, hw);
   }
   return launch_status("apply_pauli_string");
 }

@@ -196,6 +196,26 @@ def test_pauli_csr_device_matches_oracle(case):
     assert np.array_equal(v.cpu().numpy(), v0.astype(np.complex64))
 
 
+@pytest.mark.parametrize("case", ["random_10q", "heisenberg_12q", "single_1q"])
+def test_hamiltonian_expectation_matches_dense(case):
+    """<psi|H|psi> from the grouped FP64 Pauli expectations equals the dense
+    FP64 sandwich with the oracle's Kronecker-product matrix."""
+    from blackwell_backend.hamiltonian import SparsePauliOp, expectation
+    labels, coeffs, n = _csr_cases()[case]
+    if case != "single_1q":
+        coeffs = np.real(coeffs)                  # Hermitian: real total
+    rng = np.random.default_rng(5)
+    psi = (rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)).astype(np.complex64)
+    st = torch.from_numpy(psi).to("cuda:0")
+    got = expectation(st, SparsePauliOp(labels, coeffs))
+    p64 = psi.astype(np.complex128)
+    want = np.vdot(p64, OH.pauli_dense(labels, coeffs) @ p64)
+    if case == "single_1q":
+        assert isinstance(got, complex) and abs(got - want) < 1e-12 * max(1, abs(want))
+    else:
+        assert isinstance(got, float) and abs(got - want.real) < 1e-11 * max(1, abs(want))
+
+
 def test_hamiltonian_evolution_matches_expm(sim):
     from blackwell_backend import workloads as W
     from blackwell_backend.hamiltonian import HamiltonianCSR
