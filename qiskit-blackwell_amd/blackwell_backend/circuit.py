@@ -70,6 +70,8 @@ class QuantumCircuit:
         self.clbits = [Bit(i, "Clbit") for i in range(int(num_clbits))]
         self.name = name or f"circuit-{next(_ids)}"
         self.data = []
+        # e^{i global_phase} multiplies the whole state (Qiskit's attribute)
+        self.global_phase = 0.0
 
     @property
     def num_qubits(self):
@@ -160,6 +162,7 @@ class QuantumCircuit:
 
     def copy(self, name=None):
         c = QuantumCircuit(self.num_qubits, self.num_clbits, name or self.name)
+        c.global_phase = self.global_phase
         for ins in self.data:
             c.data.append(CircuitInstruction(
                 ins.operation,
@@ -184,7 +187,15 @@ def fingerprint(circuit):
     if not isinstance(circuit, QuantumCircuit):
         return None
     return (circuit.num_qubits, circuit.num_clbits, len(circuit.data),
-            hash(tuple(map(id, circuit.data))))
+            hash(tuple(map(id, circuit.data))), float(circuit.global_phase))
+
+
+def global_phase(circuit):
+    """The circuit's global phase in radians (0 when it has none)."""
+    try:
+        return float(getattr(circuit, "global_phase", 0.0) or 0.0)
+    except TypeError as e:
+        raise ValueError("the circuit's global phase has unbound parameters") from e
 
 
 def instructions(circuit):

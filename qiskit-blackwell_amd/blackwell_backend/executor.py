@@ -102,15 +102,28 @@ class _Dev:
 
 
 class CompiledCircuit:
-    def __init__(self, n_qubits, n_clbits, ops):
+    def __init__(self, n_qubits, n_clbits, ops, global_phase=0.0):
         self.n_qubits = n_qubits
         self.n_clbits = n_clbits
         self.ops = ops
+        # the circuit's global phase enters as the initial amplitude e^{i phi}
+        # (the reference drops it); 0 keeps the plain |0...0>
+        phi = float(global_phase)
+        self.phase0 = None if phi == 0.0 else complex(np.cos(phi), np.sin(phi))
         self._plans = {}
         self._blocks = None
         self._graphs = {}
         self._tuned = {}
         self.has_stochastic = any(o.kind in ("measure", "reset") for o in ops)
+
+    def init_state(self, state):
+        """|0...0> times e^{i global_phase} into `state` (device launches only,
+        so a graph can hold them)."""
+        call("qsim_state_init", _lib.stream_handle(state.device), ctypes.c_void_p(state.data_ptr()),
+             self.n_qubits)
+        if self.phase0 is not None:
+            state[:1].fill_(self.phase0)
+        return state
 
     # ---------------------------------------------------------------- graphs
     def graphed(self, mode, device):
@@ -120,7 +133,7 @@ class CompiledCircuit:
         key = (mode, str(device))
         if key not in self._graphs:
             plan = self.plan(mode, device)
-            self._graphs[key] = (GraphedRun(plan, self.n_qubits, device)
+            self._graphs[key] = (GraphedRun(plan, self, device)
                                  if not self.has_stochastic and plan.graph_safe() else None)
         return self._graphs[key]
 
@@ -400,9 +413,9 @@ class GraphedRun:
     allocation (workspaces, plan tables) exists before the graph records
     pointers."""
 
-    def __init__(self, plan, n_qubits, device):
-        self.n = n_qubits
-        self.buf = torch.empty(1 << n_qubits, dtype=torch.complex64, device=device)
+    def __init__(self, plan, cc, device):
+        self.cc = cc
+        self.buf = torch.empty(1 << cc.n_qubits, dtype=torch.complex64, device=device)
         side = torch.cuda.Stream(device)
         side.wait_stream(torch.cuda.current_stream(device))
         with torch.cuda.stream(side):
@@ -415,8 +428,7 @@ class GraphedRun:
         torch.cuda.synchronize(device)
 
     def _body(self, plan):
-        call("qsim_state_init", _lib.stream_handle(self.buf.device), ctypes.c_void_p(self.buf.data_ptr()),
-             self.n)
+        self.cc.init_state(self.buf)
         plan.run(self.buf)
 
     def replay(self):

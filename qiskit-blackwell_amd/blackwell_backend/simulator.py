@@ -14,6 +14,8 @@ Differences by design:
     mode="plain" runs one kernel per gate;
   * sampling uses FP64 uniforms and an FP64 inverse CDF; counts are tallied
     with numpy instead of a Python loop over shots;
+  * a circuit's global_phase multiplies the statevector (the reference
+    drops it), as the initial amplitude e^{i phase};
   * qiskit is optional: with it installed, circuits with non-native gates
     are first transpiled to the native basis at optimisation level 1 as the
     reference does (simulator.py:489-498), and qiskit Result objects are
@@ -29,7 +31,7 @@ import numpy as np
 import torch
 
 from . import compiler as C
-from .circuit import fingerprint, instructions
+from .circuit import fingerprint, global_phase, instructions
 from .executor import CompiledCircuit
 from .result import make_result
 
@@ -135,7 +137,7 @@ class BlackwellSimulator:
                 except (TypeError, ValueError):
                     pv.append(hash(str(p)))
             parts.append((name, tuple(pv), tuple(qs), tuple(cs)))
-        return (circuit.num_qubits, circuit.num_clbits, hash(tuple(parts)))
+        return (circuit.num_qubits, circuit.num_clbits, hash(tuple(parts)), global_phase(circuit))
 
     def _memo_entry(self, circuit):
         """Memo of one of this module's circuits while its content is unchanged."""
@@ -176,7 +178,7 @@ class BlackwellSimulator:
             circuit = pm.run(circuit)
             instrs = instructions(circuit)
         ops = C.compile_ops(instrs)
-        cc = CompiledCircuit(circuit.num_qubits, circuit.num_clbits, ops)
+        cc = CompiledCircuit(circuit.num_qubits, circuit.num_clbits, ops, global_phase(circuit))
         self._compile_cache[key] = cc
         return cc
 
@@ -199,7 +201,7 @@ class BlackwellSimulator:
                 torch.cuda.synchronize(self.device)
                 return state, [0] * cc.n_clbits, time.perf_counter() - t0
         plan = cc.plan(mode, self.device)
-        state = self.bk.state_init(cc.n_qubits, device=self.device)
+        state = cc.init_state(torch.empty(1 << cc.n_qubits, dtype=torch.complex64, device=self.device))
         clbits = [0] * cc.n_clbits
         torch.cuda.synchronize(self.device)
         t0 = time.perf_counter()
@@ -216,6 +218,7 @@ class BlackwellSimulator:
         QC = _qiskit_circuit_class()
         if QC is not None and type(circuit).__module__.startswith("qiskit"):
             new = QC(circuit.num_qubits, circuit.num_clbits, name=circuit.name)
+            new.global_phase = circuit.global_phase
             for ins in circuit.data:
                 if ins.operation.name.lower() not in ("measure", "reset"):
                     new.append(ins.operation, ins.qubits, ins.clbits)

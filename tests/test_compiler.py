@@ -252,6 +252,10 @@ def test_circuit_fingerprint_tracks_content():
     u[0, 0] = 5.0                                       # the caller's array is not aliased
     assert qc.data[2].operation.params[0][0, 0] == 1.0
     assert fingerprint(object()) is None
+    f2 = fingerprint(qc)
+    qc.global_phase = 0.25                              # the phase is part of the content
+    assert fingerprint(qc) != f2
+    assert qc.copy().global_phase == 0.25
 
 
 def test_plan_blocks_seed_lookahead_with_wide_ops():

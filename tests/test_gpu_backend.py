@@ -281,6 +281,35 @@ def test_graph_replays_match_eager():
     assert graphed.compile(W.qft(12)).graphed("plain", graphed.device) is not None
 
 
+def test_global_phase_multiplies_the_state():
+    """global_phase = phi gives e^{i phi} times the phase-free state in every
+    mode, eager and graphed; changing it on the same circuit object is seen."""
+    from blackwell_backend import BlackwellSimulator, workloads as W
+    from blackwell_backend.circuit import QuantumCircuit
+    phi = 0.7
+    ph = np.exp(1j * phi)
+    qc = QuantumCircuit(1)
+    qc.h(0)
+    qc.global_phase = phi
+    for graphs in (False, True):
+        sim = BlackwellSimulator("cuda:0", use_graphs=graphs)
+        sv, _ = sim.run_statevector(qc)
+        assert np.allclose(sv, ph * np.array([1, 1]) / np.sqrt(2), atol=1e-7), (graphs, sv)
+        base = W.su4_volume(12, 6, 8)
+        ref, _ = sim.run_statevector(base, mode="plain")
+        for mode in ("plain", "blocks", "tiles", "whole"):
+            c = base.copy()
+            c.global_phase = phi
+            got, _ = sim.run_statevector(c, mode=mode)
+            assert np.max(np.abs(got - ph * ref)) < 5e-6, (graphs, mode)
+        qc2 = QuantumCircuit(2)
+        qc2.x(1)
+        a, _ = sim.run_statevector(qc2)
+        qc2.global_phase = np.pi
+        b, _ = sim.run_statevector(qc2)
+        assert a[2] == 1 and abs(b[2] + 1) < 1e-7
+
+
 def test_tune_mode_picks_a_measured_mode(sim):
     from blackwell_backend import BlackwellSimulator, workloads as W
     tuned = BlackwellSimulator("cuda:0", mode="tune")
