@@ -175,6 +175,35 @@ def per_gate_sweep(n=30, reps=4):
     return out
 
 
+def stride_sweep(n=30, reps=6):
+    """GB/s of the dense 1Q kernel for every target and of the 2Q kernel for
+    (7, q) and (12, q) on an n-qubit state: the partner stream of target t
+    sits 2^(t+3) bytes away, and the rate depends on that distance."""
+    import blackwell_kernels as bk
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    st = bk.state_init(n, device=dev)
+    u2 = torch.tensor([[0.6, 0.8j], [0.8j, 0.6]], dtype=torch.complex64, device=dev)
+    u4 = torch.from_numpy(np.linalg.qr(np.random.default_rng(1).standard_normal((4, 4)) + 0j)[0]
+                          .astype(np.complex64)).to(dev)
+    N = 1 << n
+
+    def gbs(fn):
+        fn()
+        torch.cuda.synchronize()
+        with EventTimer(stream) as ev:
+            for _ in range(reps):
+                fn()
+        return round(16.0 * N / (ev.ms() / reps * 1e-3) / 1e9, 1)
+
+    out = {"1q": {t: gbs(lambda: bk.apply_gate(st, u2, t)) for t in range(n)},
+           "2q_7_q": {q: gbs(lambda: bk.apply_gate_2q(st, u4, 7, q)) for q in range(8, n)},
+           "2q_12_q": {q: gbs(lambda: bk.apply_gate_2q(st, u4, 12, q)) for q in range(13, n)}}
+    del st
+    torch.cuda.empty_cache()
+    return out
+
+
 def observables_sweep(n=30, reps=4):
     """GB/s of the Pauli-string expectation, Pauli application and noise
     channel kernels on an n-qubit state, plus the grouped energy of the
@@ -493,6 +522,7 @@ SECTIONS = {
     "c5": ("c5_chebyshev_22q", lambda: c5_bench()),
     "serving": ("serving", lambda: serving_bench()),
     "observables": ("observables_30q", lambda: observables_sweep(30)),
+    "stride": ("stride_sweep_30q", lambda: stride_sweep(30)),
 }
 
 
