@@ -298,6 +298,27 @@ typedef struct qsim_instance {
 int qsim_run_instances(void* stream, void* states_base, const void* inst_dev, int n_inst,
                        const void* ops_dev, const void* coefs_dev, void* shots_dev,
                        int max_qubits);
+
+/* One (instance, block) of a batched block step (new): block_rec.state is the
+ * device address of a complex64 state of n_qubits (10..32), .mat the device
+ * address of its 32x32 complex64 block (row-major, row bit j <-> q[j]), and
+ * .set_begin the prefix sum of 2^(n_qubits - 10) over the records before it
+ * in the same step.                                                       */
+typedef struct qsim_block_rec {
+  uint64_t state;
+  uint64_t mat;
+  uint64_t set_begin;
+  int32_t  n_qubits;
+  int8_t   q[5];
+  int8_t   pad[7];
+} qsim_block_rec;
+/* Apply one 5-qubit block to each of n_recs states in one launch (FP32 MFMA,
+ * the same arithmetic as qsim_apply_block_mfma, so each state's result is
+ * independent of the batch).  recs_host is validated on the host and must
+ * hold the same bytes as the device copy recs_dev, which the kernel reads.
+ * The states of one call must not overlap.  (new; the serving HBM lane)   */
+int qsim_apply_block_batch(void* stream, const qsim_block_rec* recs_host, const void* recs_dev,
+                           int64_t n_recs);
 /* Fixed-n spec paths kept for API parity: probs out, float32.
  * qv8: gate_matrices f32[C][G][4][4][2], gate_qubits i32[C][G][2], row index
  * r = 2*bit(q1) + bit(q0) with q0 the first listed qubit (qv8_sim_sm120.cu:88-90).

@@ -39,7 +39,7 @@ import numpy as np
 import torch
 
 from blackwell_kernels import _lib
-from blackwell_kernels._lib import call, OP_DTYPE, INSTANCE_DTYPE
+from blackwell_kernels._lib import call, BLOCK_REC_DTYPE, OP_DTYPE, INSTANCE_DTYPE
 from . import compiler as C
 from .circuit import instructions
 from .executor import CompiledCircuit, tile_ops
@@ -299,23 +299,104 @@ class _HbmJob:
         self.staging = None
 
 
+class _HbmBatch:
+    """HBM-lane requests with gate payloads, run together: their block plans
+    execute as batched block steps (one qsim_apply_block_batch launch per
+    step applies block s of every request that has one), then each state is
+    sampled with its own seeded device uniforms.  Block matrices and step
+    records go to the device in one pinned copy.  Each request's arithmetic
+    is that of qsim_apply_blocks, so its samples do not depend on the batch."""
+
+    def __init__(self, jobs, stream, pool):
+        self.jobs = jobs
+        self.reqs = [j.req for j in jobs]
+        self.stream = stream
+        self.pool = pool
+        self.staging = None
+        self.event = None
+
+    def launch(self):
+        from blackwell_kernels import sample_with_uniforms, uniforms_splitmix
+        from blackwell_kernels.cuquantum import block_step_records, run_block_steps
+        for j in self.jobs:
+            if j.prepared is not None:
+                j.prepared.result()
+            else:
+                j.prepare()
+        dev = self.jobs[0].sim.device
+        sh = ctypes.c_void_p(self.stream.cuda_stream)
+        ns = [r.n_qubits for r in self.reqs]
+        mats = [j.blocks[1] for j in self.jobs]
+        mat_bytes = [m.nbytes for m in mats]
+        moff = np.concatenate([[0], np.cumsum(mat_bytes)]).astype(np.int64)
+        n_rec = sum(len(m) for m in mats)
+        rec_size = BLOCK_REC_DTYPE.itemsize
+        total = int(moff[-1]) + n_rec * rec_size
+        with torch.cuda.stream(self.stream):
+            sizes = [1 << n for n in ns]
+            soff = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+            self.slab = torch.empty(int(soff[-1]), dtype=torch.complex64, device=dev)
+            self.d_buf = torch.empty(max(total, 1), dtype=torch.uint8, device=dev)
+            base = self.slab.data_ptr()
+            recs, off = block_step_records([base + 8 * int(o) for o in soff[:-1]], ns,
+                                           [self.d_buf.data_ptr() + int(o) for o in moff[:-1]],
+                                           [j.blocks[0] for j in self.jobs])
+            self.staging = self.pool.acquire(total)
+            host = self.staging.numpy()
+            for m, o in zip(mats, moff[:-1]):
+                host[int(o):int(o) + m.nbytes] = m.view(np.uint8).reshape(-1)
+            host[int(moff[-1]):total] = recs.view(np.uint8)
+            self.d_buf[:total].copy_(self.staging[:total], non_blocking=True)
+            self.recs = recs                       # validated host copy of the records
+            for n, o in zip(ns, soff[:-1]):
+                call("qsim_state_init", sh, ctypes.c_void_p(base + 8 * int(o)), n)
+            run_block_steps(sh, recs, self.d_buf[int(moff[-1]):], off)
+            shot_off = np.concatenate([[0], np.cumsum([r.shots for r in self.reqs])]).astype(np.int64)
+            self.h_out = torch.empty(max(int(shot_off[-1]), 1), dtype=torch.int64, pin_memory=True)
+            self.d_outs = []
+            for r, o, so in zip(self.reqs, soff[:-1], shot_off[:-1]):
+                st = self.slab[int(o):int(o) + (1 << r.n_qubits)]
+                u = uniforms_splitmix(r.seed, r.shots, dev)
+                d = sample_with_uniforms(st, u)
+                self.d_outs.append(d)
+                self.h_out[int(so):int(so) + r.shots].copy_(d, non_blocking=True)
+            self.shot_off = shot_off
+            self.event = torch.cuda.Event()
+            self.event.record(self.stream)
+
+    def done(self):
+        return self.event is not None and self.event.query()
+
+    def finish(self, now):
+        self.event.synchronize()
+        out = self.h_out.numpy()
+        for r, so in zip(self.reqs, self.shot_off[:-1]):
+            r.samples = out[int(so):int(so) + r.shots].copy()
+            r.t_done = now
+        self.slab = self.d_buf = self.d_outs = None
+        self.pool.release(self.staging)
+        self.staging = None
+
+
 # ---------------------------------------------------------------- server
 class Server:
     """Continuous-batching server.
 
     budget_bytes     HBM budget for resident states (None: 80 % of free memory)
     max_batch        most requests per LDS launch
-    max_hbm_jobs     HBM-lane circuits in flight at once
+    max_hbm_jobs     HBM-lane launch units (batches or single circuits) in flight
+    max_hbm_batch    most HBM-lane circuits per batch
     planner_threads  host threads that build HBM-lane block plans ahead
     plan_ahead       most queued HBM-lane circuits planned ahead of admission
     """
 
     def __init__(self, sim=None, budget_bytes=None, max_batch=2048, max_hbm_jobs=4,
                  device="cuda:0", lds_factory=None, hbm_factory=None, planner_threads=8,
-                 plan_ahead=64):
+                 plan_ahead=256, max_hbm_batch=64, hbm_batch_factory=None):
         """lds_factory(reqs) / hbm_factory(req) build lane jobs with launch(),
-        done() and finish(now); the defaults run on the GPU, tests inject
-        host-only fakes to drive the scheduling logic."""
+        done() and finish(now); hbm_batch_factory(jobs) runs several HBM-lane
+        jobs as one unit (None: one job per unit).  The defaults run on the
+        GPU; tests inject host-only fakes to drive the scheduling logic."""
         fake = lds_factory is not None and hbm_factory is not None
         if sim is None and not fake:
             from .simulator import BlackwellSimulator
@@ -330,6 +411,8 @@ class Server:
         self.max_hbm_jobs = int(max_hbm_jobs)
         self.planner_threads = int(planner_threads)
         self.plan_ahead = int(plan_ahead)
+        self.max_hbm_batch = int(max_hbm_batch)
+        self.hbm_batch_factory = hbm_batch_factory
         self.in_use = 0
         self.peak_in_use = 0
         if fake:
@@ -340,6 +423,8 @@ class Server:
             self.lds_factory = lambda reqs: _LdsBatch(reqs, self.device, self.lds_stream)
             self.pinned = _PinnedPool()
             self.hbm_factory = lambda req: _HbmJob(req, self.sim, self.hbm_stream, self.pinned)
+            if hbm_batch_factory is None:
+                self.hbm_batch_factory = lambda jobs: _HbmBatch(jobs, self.hbm_stream, self.pinned)
         self.rejected = []
         self.launches = 0
 
@@ -424,24 +509,39 @@ class Server:
                     self.peak_in_use = max(self.peak_in_use, self.in_use)
                     lds_inflight.append(b)
                     launched = True
-            # HBM lane
+            # HBM lane: ready jobs (planned, within budget) go out together;
+            # a job without a gate payload runs alone
             while queue_hbm and len(hbm_inflight) < self.max_hbm_jobs:
-                j = queue_hbm[0]
-                fut = getattr(j, "prepared", None)
-                if fut is not None and not fut.done():
-                    break                 # its plan is still being built
-                r = j.reqs[0]
-                need = self.admit_bytes(r)
-                if self.in_use + need > self.budget:
+                group = []
+                limit = self.max_hbm_batch if self.hbm_batch_factory is not None else 1
+                while queue_hbm and len(group) < limit:
+                    j = queue_hbm[0]
+                    fut = getattr(j, "prepared", None)
+                    if fut is not None and not fut.done():
+                        break             # its plan is still being built
+                    batchable = getattr(j.reqs[0], "gates", None) is not None
+                    if group and not batchable:
+                        break
+                    r = j.reqs[0]
+                    need = self.admit_bytes(r)
+                    if self.in_use + need > self.budget:
+                        break
+                    queue_hbm.popleft()
+                    n_planned = max(0, n_planned - 1)
+                    self.in_use += need
+                    r.t_admit = time.perf_counter() - t0
+                    group.append(j)
+                    if not batchable:
+                        break
+                if not group:
                     break
-                queue_hbm.popleft()
-                n_planned = max(0, n_planned - 1)
-                self.in_use += need
-                r.t_admit = time.perf_counter() - t0
-                j.launch()
+                unit = (self.hbm_batch_factory(group)
+                        if self.hbm_batch_factory is not None and
+                        getattr(group[0].reqs[0], "gates", None) is not None else group[0])
+                unit.launch()
                 self.launches += 1
                 self.peak_in_use = max(self.peak_in_use, self.in_use)
-                hbm_inflight.append(j)
+                hbm_inflight.append(unit)
                 launched = True
             if time.perf_counter() - t0 > timeout_s:
                 raise TimeoutError(f"serving did not finish within {timeout_s} s ({done}/{total} done)")

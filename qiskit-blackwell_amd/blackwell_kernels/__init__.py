@@ -12,7 +12,7 @@ from .cuquantum import (
     measure_qubit, measure_probs, sample, sample_with_uniforms, uniforms_splitmix, norm2,
     expectation_z, expectation_pauli, expectation_paulis, expectation_pauli_string,
     apply_pauli_string,
-    apply_mcgate, apply_mc2qgate, apply_mcx, apply_mcswap, apply_block, run_tile_program,
+    apply_mcgate, apply_mc2qgate, apply_mcx, apply_mcswap, apply_block, apply_blocks_batched, run_tile_program,
     swap_gate, cswap_gate,
     apply_depolarizing, apply_amplitude_damping, apply_dephasing, pauli_x_gate,
     apply_gate_batched, apply_gates_batched_fused, state_init_batched,

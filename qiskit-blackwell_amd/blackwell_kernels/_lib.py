@@ -52,6 +52,7 @@ PROTOTYPES = {
     "qsim_apply_block_mfma": (_i, [_vp, _vp, _i, _vp, _pint, _i]),
     "qsim_plan_blocks": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _pint]),
     "qsim_apply_blocks": (_i, [_vp, _vp, _i, _vp, _vp, _i]),
+    "qsim_apply_block_batch": (_i, [_vp, _vp, _vp, _i64]),
     "qsim_reduce_workspace_bytes": (_i64, [_i]),
     "qsim_norm2": (_i, [_vp, _vp, _i, _vp, _vp]),
     "qsim_measure_probs": (_i, [_vp, _vp, _i, _vp]),
@@ -85,13 +86,15 @@ PROTOTYPES = {
     "qsim_cheb_finish": (_i, [_vp, _vp, _vp, _vp, _i64]),
 }
 
-# numpy mirrors of the two structs of the C ABI
+# numpy mirrors of the structs of the C ABI
 OP_DTYPE = np.dtype([("kind", "u1"), ("nq", "u1"), ("q", "u1", (6,)),
                      ("cmask", "<u4"), ("coef", "<u4")])
 INSTANCE_DTYPE = np.dtype([("state_off", "<u8"), ("seed", "<u8"), ("shot_off", "<u8"),
                            ("op_off", "<u4"), ("n_ops", "<u4"), ("shots", "<u4"),
                            ("n_qubits", "u1"), ("init", "u1"), ("want", "u1"), ("pad", "u1")])
-assert OP_DTYPE.itemsize == 16 and INSTANCE_DTYPE.itemsize == 40
+BLOCK_REC_DTYPE = np.dtype([("state", "<u8"), ("mat", "<u8"), ("set_begin", "<u8"),
+                            ("n_qubits", "<i4"), ("q", "i1", (5,)), ("pad", "i1", (7,))])
+assert OP_DTYPE.itemsize == 16 and INSTANCE_DTYPE.itemsize == 40 and BLOCK_REC_DTYPE.itemsize == 40
 
 OP_M1, OP_M2, OP_D1, OP_D2, OP_PH, OP_X, OP_SWAP, OP_MK, OP_DK = range(1, 10)
 

@@ -215,6 +215,70 @@ def apply_block(state, block, qubits):
     return state
 
 
+def block_step_records(state_ptrs, n_qubits, mat_ptrs, qubits):
+    """Records of batched block steps (numpy BLOCK_REC_DTYPE) and the
+    offsets of each step in them.  state_ptrs[i] / n_qubits[i]: instance i's
+    state; mat_ptrs[i]: device address of its complex64 [B_i, 32, 32]
+    blocks; qubits[i]: host int array [B_i, >= 5].  Step s holds block s of
+    every instance with s < B_i, in instance order."""
+    counts = np.array([len(q) for q in qubits], dtype=np.int64)
+    steps = int(counts.max()) if counts.size else 0
+    inst = np.repeat(np.arange(counts.size), counts)
+    step = np.concatenate([np.arange(c) for c in counts]) if counts.size else np.zeros(0, np.int64)
+    order = np.lexsort((inst, step))             # by step, then instance
+    inst, step = inst[order], step[order]
+    qcat = np.concatenate([np.asarray(q, dtype=np.int64)[:, :5] for q in qubits]) \
+        if counts.size else np.zeros((0, 5), np.int64)
+    qoff = np.concatenate([[0], np.cumsum(counts)[:-1]]) if counts.size else np.zeros(0, np.int64)
+    nq = np.asarray(n_qubits, dtype=np.int64)
+    recs = np.zeros(inst.size, dtype=_lib.BLOCK_REC_DTYPE)
+    recs["state"] = np.asarray(state_ptrs, dtype=np.uint64)[inst]
+    recs["mat"] = np.asarray(mat_ptrs, dtype=np.uint64)[inst] + step.astype(np.uint64) * np.uint64(32 * 32 * 8)
+    recs["n_qubits"] = nq[inst]
+    recs["q"] = qcat[qoff[inst] + step]
+    sets = np.left_shift(np.int64(1), nq[inst] - 10)
+    off = np.zeros(steps + 1, dtype=np.int64)
+    off[1:] = np.cumsum(np.bincount(step, minlength=steps))
+    csum = np.cumsum(sets) - sets                 # exclusive prefix over all records
+    recs["set_begin"] = (csum - csum[off[step]]).astype(np.uint64)
+    return recs, off
+
+
+def run_block_steps(stream, recs, recs_dev, off):
+    """One qsim_apply_block_batch launch per step (recs: host records,
+    recs_dev: a device tensor holding the same bytes)."""
+    base_h = recs.ctypes.data
+    base_d = recs_dev.data_ptr()
+    size = _lib.BLOCK_REC_DTYPE.itemsize
+    for s in range(len(off) - 1):
+        a, b = int(off[s]), int(off[s + 1])
+        call("qsim_apply_block_batch", stream, _lib.ctypes.c_void_p(base_h + a * size),
+             _lib.ctypes.c_void_p(base_d + a * size), b - a)
+
+
+def apply_blocks_batched(states, mats, qubits):
+    """Apply each instance's 5-qubit block list (mats[i]: complex64 device
+    [B_i, 32, 32], qubits[i]: host int [B_i, >= 5]) to states[i] (10..32
+    qubits), one launch per block step for all instances together."""
+    if not states:
+        return states
+    for st in states:
+        _check_state(st)
+    dev = states[0].device
+    ns = [n_qubits_of(st) for st in states]
+    for m, q in zip(mats, qubits):
+        if m.dtype != torch.complex64 or not m.is_cuda or not m.is_contiguous() or \
+                tuple(m.shape[1:]) != (32, 32) or m.shape[0] != len(q):
+            raise RuntimeError("mats[i] must be a contiguous CUDA complex64 [B_i, 32, 32] tensor")
+    recs, off = block_step_records([st.data_ptr() for st in states], ns,
+                                   [m.data_ptr() for m in mats], qubits)
+    if recs.size == 0:
+        return states
+    recs_dev = torch.from_numpy(recs.view(np.uint8).copy()).to(dev)
+    run_block_steps(stream_handle(dev), recs, recs_dev, off)
+    return states
+
+
 def _tile_mask(n, must, bits=None):
     """Tile mask holding `must` plus the lowest other qubits (qubit 0 always)."""
     bits = min(n, bits or 10)

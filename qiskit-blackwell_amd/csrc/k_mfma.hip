@@ -109,6 +109,91 @@ __global__ __launch_bounds__(64 * W) void k_block5(float2* __restrict__ st, uint
   }
 }
 
+// Batched block step (the serving HBM lane): one launch applies one block
+// to each of many instance states.  Record r (qsim_block_rec) names the
+// state, the block matrix and the qubits, and covers the launch's sets
+// [set_begin_r, set_begin_r + 2^(n_r - 10)) of 32 groups each.  Wave w takes
+// sets w*spw .. w*spw + spw - 1; each set's record is found by a binary
+// search over set_begin (the same for the whole wave), and its groups go
+// through exactly the MFMA sequence of k_block5, so an instance's result
+// does not depend on what else is in the batch.  Row offsets are 32-bit
+// (n <= 32, checked on the host).
+__global__ __launch_bounds__(256) void k_block5_batch(const qsim_block_rec* __restrict__ recs,
+                                                      int n_recs, uint64_t total_sets, int spw) {
+  const int lane = threadIdx.x & 63;
+  const int h = lane >> 5, c = lane & 31;
+  const uint64_t wave = (uint64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int cur = -1;
+  float ur[16], ui[16];
+  uint32_t rof[16];
+  int qv[5], qs[5];
+  float2* st = nullptr;
+  uint64_t sb = 0;
+  for (int k = 0; k < spw; ++k) {
+    const uint64_t set = wave * spw + k;
+    if (set >= total_sets) break;
+    int lo = 0, hi = n_recs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (recs[mid].set_begin <= set) lo = mid; else hi = mid - 1;
+    }
+    if (lo != cur) {
+      cur = lo;
+      const qsim_block_rec& r = recs[lo];
+      st = reinterpret_cast<float2*>(r.state);
+      sb = r.set_begin;
+      const float2* U = reinterpret_cast<const float2*>(r.mat);
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const float2 u = U[c * 32 + 2 * s + h];
+        ur[s] = u.x;
+        ui[s] = u.y;
+      }
+#pragma unroll
+      for (int j = 0; j < 5; ++j) qv[j] = r.q[j];
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const int row = 2 * s + h;
+        uint32_t o = 0;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) o |= (uint32_t)((row >> j) & 1) << qv[j];
+        rof[s] = o;
+      }
+      // ascending block positions, for inserting zeros into the group index
+#pragma unroll
+      for (int j = 0; j < 5; ++j) qs[j] = qv[j];
+#pragma unroll
+      for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j < 4 - i; ++j)
+          if (qs[j] > qs[j + 1]) { const int t = qs[j]; qs[j] = qs[j + 1]; qs[j + 1] = t; }
+    }
+    uint64_t base = (set - sb) * 32 + c;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) base = ins0(base, qs[j]);
+    float2 x[16];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) x[s] = st[base | rof[s]];
+    f32x16 accr = {0.f}, acci = {0.f};
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      accr = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[s], x[s].x, accr, 0, 0, 0);
+      accr = __builtin_amdgcn_mfma_f32_32x32x2f32(-ui[s], x[s].y, accr, 0, 0, 0);
+      acci = __builtin_amdgcn_mfma_f32_32x32x2f32(ui[s], x[s].x, acci, 0, 0, 0);
+      acci = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[s], x[s].y, acci, 0, 0, 0);
+    }
+    // register r of lane (h, c) holds row i = 8 (r / 4) + 4 h + r % 4 of group c
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int i = 8 * (r >> 2) + 4 * h + (r & 3);
+      uint32_t o = 0;
+#pragma unroll
+      for (int j = 0; j < 5; ++j) o |= (uint32_t)((i >> j) & 1) << qv[j];
+      st[base | o] = make_float2(accr[r], acci[r]);
+    }
+  }
+}
+
 // Wide variant (n >= 11, n <= 32): 16-byte lanes over sets of 64 groups.
 //   Q0 = 0 (qubit 0 outside the block): lane (h, c) holds groups 2c and 2c+1
 //     of row 2s+h as one float4 (they are adjacent amplitudes); the even and
@@ -421,6 +506,40 @@ int qsim_apply_block_mfma(void* stream, void* state, int n_qubits, const void* b
   hipLaunchKernelGGL((k_block5<W>), dim3(blocks), dim3(64 * W), 0, as_stream(stream),
                      (float2*)state, n_sets, a, (const float2*)block_dev);
   return launch_status("apply_block_mfma");
+}
+
+int qsim_apply_block_batch(void* stream, const qsim_block_rec* recs_host, const void* recs_dev,
+                           int64_t n_recs) {
+  clear_error();
+  if (n_recs < 0 || n_recs > (1 << 24)) return fail(QSIM_ERANGE, "n_recs = %lld out of range", (long long)n_recs);
+  if (n_recs == 0) return QSIM_OK;
+  if (!recs_host || !recs_dev) return fail(QSIM_EINVAL, "NULL pointer argument");
+  uint64_t total = 0;
+  for (int64_t i = 0; i < n_recs; ++i) {
+    const qsim_block_rec& r = recs_host[i];
+    if (r.n_qubits < 10 || r.n_qubits > 32)
+      return fail(QSIM_ERANGE, "record %lld: batched blocks need 10 <= n_qubits <= 32 (got %d)",
+                  (long long)i, r.n_qubits);
+    if (!r.state || !r.mat) return fail(QSIM_EINVAL, "record %lld: NULL state or matrix", (long long)i);
+    uint64_t seen = 0;
+    for (int j = 0; j < 5; ++j) {
+      if (r.q[j] < 0 || r.q[j] >= r.n_qubits)
+        return fail(QSIM_EINVAL, "record %lld: block qubit %d out of range", (long long)i, r.q[j]);
+      if (seen >> r.q[j] & 1) return fail(QSIM_EINVAL, "record %lld: block qubits must differ", (long long)i);
+      seen |= 1ull << r.q[j];
+    }
+    if (r.set_begin != total)
+      return fail(QSIM_EINVAL, "record %lld: set_begin %llu, expected %llu (prefix of 2^(n-10))",
+                  (long long)i, (unsigned long long)r.set_begin, (unsigned long long)total);
+    total += 1ull << (r.n_qubits - 10);
+  }
+  const int spw = total < 8192 ? 1 : (total < 65536 ? 2 : 4);
+  const uint64_t waves = ceil_div(total, spw);
+  if (ceil_div(waves, 4) > 0x7fffffffull) return fail(QSIM_ERANGE, "batch too large (%llu sets)",
+                                                     (unsigned long long)total);
+  hipLaunchKernelGGL(k_block5_batch, dim3((unsigned)ceil_div(waves, 4)), dim3(256), 0, as_stream(stream),
+                     (const qsim_block_rec*)recs_dev, (int)n_recs, total, spw);
+  return launch_status("apply_block_batch");
 }
 
 }  // extern "C"

@@ -57,6 +57,9 @@ int main(void) {
   printf("%zu %zu %zu %zu %zu %zu\n", sizeof(qsim_instance), offsetof(qsim_instance, op_off),
          offsetof(qsim_instance, shots), offsetof(qsim_instance, n_qubits),
          offsetof(qsim_instance, want), offsetof(qsim_instance, shot_off));
+  printf("%zu %zu %zu %zu %zu\n", sizeof(qsim_block_rec), offsetof(qsim_block_rec, mat),
+         offsetof(qsim_block_rec, set_begin), offsetof(qsim_block_rec, n_qubits),
+         offsetof(qsim_block_rec, q));
   return 0;
 }'''
     with tempfile.TemporaryDirectory() as d:
@@ -64,9 +67,13 @@ int main(void) {
         exe = os.path.join(d, "probe")
         open(c, "w").write(prog)
         subprocess.run(["gcc", "-I", os.path.dirname(HEADER), c, "-o", exe], check=True)
-        l1, l2 = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")[:2]
+        l1, l2, l3 = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")[:3]
     op = [int(x) for x in l1.split()]
     ins = [int(x) for x in l2.split()]
+    rec = [int(x) for x in l3.split()]
+    bd = _lib.BLOCK_REC_DTYPE
+    assert rec == [bd.itemsize, bd.fields["mat"][1], bd.fields["set_begin"][1], bd.fields["n_qubits"][1],
+                   bd.fields["q"][1]]
     od, idt = _lib.OP_DTYPE, _lib.INSTANCE_DTYPE
     assert op == [od.itemsize, od.fields["cmask"][1], od.fields["coef"][1], od.fields["q"][1]]
     assert ins == [idt.itemsize, idt.fields["op_off"][1], idt.fields["shots"][1],
@@ -113,6 +120,20 @@ def test_meta_and_validation_without_device():
     assert lib.qsim_apply_dephasing(None, v8, 4, 4, 0.1, 0.5, v8, ctypes.byref(b)) == 1
     assert b"target_qubit" in lib.qsim_last_error()
     assert lib.qsim_expectation_workspace_bytes(20) >= 8 * 1024 * 8
+    import numpy as np
+    recs = np.zeros(2, dtype=_lib.BLOCK_REC_DTYPE)
+    recs["state"], recs["mat"], recs["n_qubits"] = 8, 8, 12
+    recs["q"] = [0, 1, 2, 3, 4]
+    recs[1]["set_begin"] = 3                                           # prefix must be 4
+    rp = recs.ctypes.data_as(ctypes.c_void_p)
+    assert lib.qsim_apply_block_batch(None, rp, v8, 2) == 1
+    assert b"set_begin" in lib.qsim_last_error()
+    recs[1]["set_begin"] = 4
+    recs[1]["q"] = [0, 1, 2, 3, 3]
+    assert lib.qsim_apply_block_batch(None, rp, v8, 2) == 1
+    recs[1]["q"] = [0, 1, 2, 3, 4]
+    recs[1]["n_qubits"] = 9
+    assert lib.qsim_apply_block_batch(None, rp, v8, 2) == 2
 
 
 def test_python_layer_raises_runtime_error():

@@ -464,3 +464,39 @@ def test_spmv_and_chebyshev_kernels(bk, dev):
     t2 = 2 * x - y
     assert np.allclose(tn.cpu().numpy(), t2, atol=1e-6)
     assert np.allclose(acc.cpu().numpy(), y + (0.2 + 0.1j) * t2, atol=1e-5)
+
+
+def test_block_batch_equals_per_instance_blocks(bk, dev):
+    """Batched block steps (qsim_apply_block_batch) give bitwise the results
+    of applying each instance's blocks alone (qsim_apply_block_mfma, whose
+    variant depends on the qubits), for instances of different sizes and
+    block counts, including qubit 0 and three or more low block qubits."""
+    rng = np.random.default_rng(77)
+    ns = [10, 11, 12, 15, 13]
+    counts = [3, 1, 4, 2, 5]
+    states, mats, qubits, refs = [], [], [], []
+    for i, (n, b) in enumerate(zip(ns, counts)):
+        psi = rand_state(n, 500 + i)
+        qs = []
+        for k in range(b):
+            if k % 3 == 0:
+                q = [0, 1, 2, int(n - 1), int(n - 2)]          # three low qubits, with qubit 0
+            else:
+                q = [int(x) for x in rng.choice(n, 5, replace=False)]
+            qs.append(q + [-1, -1, -1])
+        us = np.stack([rand_unitary(32, 900 + 10 * i + k) for k in range(b)]).astype(np.complex64)
+        ref = to_dev(psi, dev)
+        for k in range(b):
+            bk.apply_block(ref, torch.from_numpy(us[k]).to(dev), qs[k][:5])
+        refs.append(ref)
+        states.append(to_dev(psi, dev))
+        mats.append(torch.from_numpy(us).to(dev))
+        qubits.append(np.array(qs, dtype=np.int32))
+    bk.apply_blocks_batched(states, mats, qubits)
+    for i, (a, b) in enumerate(zip(states, refs)):
+        assert torch.equal(a.view(torch.int32), b.view(torch.int32)), (i, ns[i])
+    # against FP64 too, for one instance
+    psi = rand_state(ns[2], 502).astype(np.complex128)
+    for k in range(counts[2]):
+        psi = O.apply_matrix(psi, mats[2][k].cpu().numpy().astype(np.complex128), list(qubits[2][k][:5]))
+    assert _rel_err(psi, states[2].cpu().numpy()) < 8e-6 * counts[2]

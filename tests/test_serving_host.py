@@ -94,3 +94,36 @@ def test_su4_program_matches_circuit_program():
     ops, cf = S.su4_program(9, 9, 4)
     ops2, cf2 = S.circuit_program(W.su4_volume(9, 9, 4))
     assert np.array_equal(ops, ops2) and np.allclose(cf, cf2)
+
+
+def test_hbm_lane_batches_payload_requests_within_budget():
+    """HBM-lane requests that carry gate payloads go out in batches (at most
+    max_hbm_batch, within the budget); a request without a payload runs
+    alone; every request is answered once."""
+    from blackwell_backend.serving import Request, Server
+
+    class FakeHbm(FakeJob):
+        def __init__(self, req):
+            super().__init__(req, 0.003)
+            self.req, self.prepared = req, None
+
+    batches = []
+
+    def batch_factory(jobs):
+        batches.append([j.req.rid for j in jobs])
+        return FakeJob([j.req for j in jobs], 0.003)
+
+    def one(req):
+        return FakeHbm(req)
+
+    budget = 3 * ((1 << 19) * 8 + 8 * 16 + (1 << 20))      # three 19-qubit requests at a time
+    s = Server(budget_bytes=budget, lds_factory=lambda rs: FakeJob(rs, 0.001), hbm_factory=one,
+               hbm_batch_factory=batch_factory, max_hbm_batch=8, planner_threads=0)
+    rs = [Request(i, 0.0, 19, 16, 7 + i, gates=([(0, 1)], [None])) for i in range(10)]
+    rs.append(Request(10, 0.0, 19, 16, 99, program=("su4", 19, 19, 1)))   # no payload
+    st = s.serve(rs)
+    assert st["served"] == 11
+    assert all(1 <= len(b) <= 3 for b in batches)          # the budget caps each batch
+    assert sorted(sum(batches, [])) == list(range(10))
+    assert s.peak_in_use <= budget
+    assert rs[10].samples is not None and 10 not in sum(batches, [])
