@@ -315,7 +315,7 @@ __global__ __launch_bounds__(kThreads) void k_leaf_prefix(const double* __restri
   __shared__ double sh[4];
   const uint64_t beg = (uint64_t)blockIdx.x * kScanSeg;
   double carry = seg[blockIdx.x];
-  for (int b = 0; b < kScanSeg; b += kThreads) {
+  for (int b = 0; b < kScanSeg && beg + b < nl; b += kThreads) {
     const uint64_t j = beg + b + threadIdx.x;
     const double v = j < nl ? tot[j] : 0.0;
     const double incl = block_incl_scan(v, sh);
@@ -326,6 +326,32 @@ __global__ __launch_bounds__(kThreads) void k_leaf_prefix(const double* __restri
     carry += last;
     __syncthreads();
   }
+}
+
+// States of at most kScanSeg leaves (n <= 20): the three scan passes in one
+// workgroup, with the same operations in the same order (the total is the
+// segment sum of k_seg_sum; the single segment's exclusive offset is 0).
+__global__ __launch_bounds__(kThreads) void k_scan_small(const double* __restrict__ tot, uint64_t nl,
+                                                         double* __restrict__ pre,
+                                                         double* __restrict__ total) {
+  __shared__ double sh[4];
+  double s = 0.0;
+  for (int k = threadIdx.x; k < kScanSeg; k += kThreads)
+    if ((uint64_t)k < nl) s += tot[k];
+  const double r = block_sum(s, sh);
+  double carry = 0.0;
+  for (int b = 0; b < kScanSeg && (uint64_t)b < nl; b += kThreads) {
+    const uint64_t j = b + threadIdx.x;
+    const double v = j < nl ? tot[j] : 0.0;
+    const double incl = block_incl_scan(v, sh);
+    if (j < nl) pre[j] = carry + (incl - v);
+    __shared__ double last;
+    if (threadIdx.x == kThreads - 1) last = incl;
+    __syncthreads();
+    carry += last;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = r;
 }
 
 // Last amplitude of the leaf with nonzero probability (-1 if none).
@@ -490,12 +516,17 @@ int qsim_sample(void* stream, const void* state, int n_qubits, const void* unifo
   hipLaunchKernelGGL(k_leaf_totals, dim3(ceil_div(p.nl, 4)), dim3(kThreads), 0, s,
                      (const float4*)state, p.n_amps, p.nl, tot);
   if (int rc = launch_status("sample: leaf totals")) return rc;
-  hipLaunchKernelGGL(k_seg_sum, dim3(p.ns), dim3(kThreads), 0, s, tot, p.nl, seg);
-  if (int rc = launch_status("sample: segment sums")) return rc;
-  hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(kThreads), 0, s, seg, p.ns, total);
-  if (int rc = launch_status("sample: segment scan")) return rc;
-  hipLaunchKernelGGL(k_leaf_prefix, dim3(p.ns), dim3(kThreads), 0, s, tot, p.nl, seg, pre);
-  if (int rc = launch_status("sample: leaf prefix")) return rc;
+  if (p.ns == 1) {
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kThreads), 0, s, tot, p.nl, pre, total);
+    if (int rc = launch_status("sample: leaf prefix")) return rc;
+  } else {
+    hipLaunchKernelGGL(k_seg_sum, dim3(p.ns), dim3(kThreads), 0, s, tot, p.nl, seg);
+    if (int rc = launch_status("sample: segment sums")) return rc;
+    hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(kThreads), 0, s, seg, p.ns, total);
+    if (int rc = launch_status("sample: segment scan")) return rc;
+    hipLaunchKernelGGL(k_leaf_prefix, dim3(p.ns), dim3(kThreads), 0, s, tot, p.nl, seg, pre);
+    if (int rc = launch_status("sample: leaf prefix")) return rc;
+  }
   hipLaunchKernelGGL(k_sample, dim3(ceil_div((uint64_t)n_shots, 4)), dim3(kThreads), 0, s,
                      (const float4*)state, p.n_amps, p.nl, pre, tot, total,
                      (const double*)uniforms_dev,
