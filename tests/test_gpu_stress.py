@@ -365,3 +365,23 @@ def test_repeated_gate_stress(sim, mode):
     b.h(0).rz(10.0, 0)
     f = O.fidelity(sim.run_statevector(a, mode=mode)[0], sim.run_statevector(b, mode=mode)[0])
     assert f > 1 - 1e-6, f                                         # the reference asks 0.99
+
+
+@pytest.mark.parametrize("family", ["GHZ", "QFT", "EfficientSU2", "DeepRz", "Toffoli", "QV"])
+def test_ab_suite_accuracy(sim, family):
+    """The reference's A/B accuracy phase (tests/ab_test.py:240-285 of the
+    reference): each circuit family at 4-20 qubits against the FP64 oracle
+    (Aer CPU there), fidelity >= 1 - 1e-7; at 4 and 8 qubits the 8192-shot
+    counts of run() also pass chi-square against the FP64 distribution."""
+    from blackwell_backend import workloads as W
+    for n in (4, 8, 12, 16, 20):
+        qc = W.AB_SUITE[family](n)
+        for mode in ("auto", "plain", "blocks") if n >= 12 else ("auto", "plain"):
+            check(sim, qc, mode)
+        if n <= 8:
+            m = qc.copy()
+            m.measure_all()
+            counts = sim.run(m, shots=8192, seed=n).get_counts()
+            idx = np.array([int(k, 16) for k in counts for _ in range(counts[k])])
+            p, _, _ = O.chi_square_p(idx, O.probs(ref_sv(qc)), 8192)
+            assert p > 1e-3, (family, n, p)
