@@ -14,12 +14,13 @@ Modes:
   auto   - tiles with the whole state in one tile when n <= 13, else blocks.
 """
 import ctypes
+import importlib
 
 import numpy as np
 import torch
 
 from blackwell_kernels import _lib
-from blackwell_kernels._lib import call, OP_DTYPE
+from blackwell_kernels._lib import OP_DTYPE
 from . import compiler as C
 
 WHOLE_TILE_MAX = 13     # n <= 13: the whole state is one LDS tile (64 KiB)
@@ -102,7 +103,10 @@ class _Dev:
 
 
 class CompiledCircuit:
-    def __init__(self, n_qubits, n_clbits, ops, global_phase=0.0):
+    def __init__(self, n_qubits, n_clbits, ops, global_phase=0.0, bk=None):
+        """bk: the blackwell_kernels module the plans launch through (the
+        simulator's, which its kernel_path selects); None = the one imported."""
+        self.bk = bk if bk is not None else importlib.import_module("blackwell_kernels")
         self.n_qubits = n_qubits
         self.n_clbits = n_clbits
         self.ops = ops
@@ -119,8 +123,9 @@ class CompiledCircuit:
     def init_state(self, state):
         """|0...0> times e^{i global_phase} into `state` (device launches only,
         so a graph can hold them)."""
-        call("qsim_state_init", _lib.stream_handle(state.device), ctypes.c_void_p(state.data_ptr()),
-             self.n_qubits)
+        kl = self.bk._lib
+        kl.call("qsim_state_init", kl.stream_handle(state.device), ctypes.c_void_p(state.data_ptr()),
+                self.n_qubits)
         if self.phase0 is not None:
             state[:1].fill_(self.phase0)
         return state
@@ -160,7 +165,7 @@ class CompiledCircuit:
             plan = self.plan(mode, device)
             times = []
             for r in range(TUNE_REPS + 1):
-                call("qsim_state_init", _lib.stream_handle(device), ctypes.c_void_p(state.data_ptr()), n)
+                self.init_state(state)
                 a = torch.cuda.Event(enable_timing=True)
                 b = torch.cuda.Event(enable_timing=True)
                 a.record(stream)
@@ -355,9 +360,10 @@ class _Plan:
         when given, returns a context manager entered around each step (for
         per-kernel-class timing with events on the launch stream)."""
         n = self.cc.n_qubits
-        s = _lib.stream_handle(state.device)
+        kl = self.cc.bk._lib
+        s = kl.stream_handle(state.device)
         sp = ctypes.c_void_p(state.data_ptr())
-        lib = _lib.load()
+        lib = kl.load()
         for kind, a in self.steps:
             if step_timer is not None and kind != "sub":
                 with step_timer(kind, a):
@@ -367,6 +373,8 @@ class _Plan:
         return state
 
     def _step(self, kind, a, n, s, sp, lib, state, rand, clbits, step_timer):
+        bk = self.cc.bk
+        call = bk._lib.call
         if kind.startswith("qsim_"):
             rc = getattr(lib, kind)(s, sp, *a)
             if rc:
@@ -387,20 +395,18 @@ class _Plan:
             idx, qs = a
             k = len(qs)
             blk = self.dev.mats[self.mat_offsets[idx]:self.mat_offsets[idx] + (1 << (2 * k))]
-            from blackwell_kernels import apply_block
-            apply_block(state, blk.reshape(1 << k, 1 << k), qs)
+            bk.apply_block(state, blk.reshape(1 << k, 1 << k), qs)
         elif kind == "sub":
             a.run(state, rand, clbits, step_timer)
         elif kind in ("measure", "reset"):
-            from blackwell_kernels import measure_qubit, apply_mcx
             op = a
             u = rand() if rand is not None else float(torch.rand(1).item())
-            o = measure_qubit(state, op.qubits[0], u)
+            o = bk.measure_qubit(state, op.qubits[0], u)
             if kind == "measure":
                 if clbits is not None and op.clbit is not None and op.clbit < len(clbits):
                     clbits[op.clbit] = o
             elif o == 1:
-                apply_mcx(state, [], op.qubits[0])
+                bk.apply_mcx(state, [], op.qubits[0])
         else:
             raise RuntimeError(f"unknown step {kind}")
 

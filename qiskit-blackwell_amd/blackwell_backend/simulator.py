@@ -41,14 +41,20 @@ _DEFAULT_KERNEL_PATH = _PKG_ROOT
 
 def _load_kernels(kernel_path=None):
     """Import blackwell_kernels from kernel_path | $BLACKWELL_KERNEL_PATH |
-    this package's directory (simulator.py:46-63)."""
-    path = kernel_path or os.environ.get("BLACKWELL_KERNEL_PATH", _DEFAULT_KERNEL_PATH)
-    if path not in sys.path:
-        sys.path.insert(0, path)
-    if "blackwell_kernels" in sys.modules:
-        existing = sys.modules["blackwell_kernels"]
-        ex_path = (getattr(existing, "__path__", [None]) or [None])[0] or ""
-        if os.path.abspath(path) not in os.path.abspath(ex_path):
+    this package's directory (simulator.py:46-63).  The requested directory
+    goes to the front of sys.path, and a module already imported from another
+    directory is replaced, so two builds can be loaded one after the other
+    (the reference keeps an existing sys.path entry where it is, so a path
+    listed behind another build's did not win, and compares directories as
+    substrings)."""
+    path = os.path.abspath(kernel_path or os.environ.get("BLACKWELL_KERNEL_PATH", _DEFAULT_KERNEL_PATH))
+    while path in sys.path:
+        sys.path.remove(path)
+    sys.path.insert(0, path)
+    existing = sys.modules.get("blackwell_kernels")
+    if existing is not None:
+        ex_dir = (getattr(existing, "__path__", [None]) or [None])[0] or ""
+        if os.path.dirname(os.path.abspath(ex_dir)) != path:
             for key in list(sys.modules):
                 if key == "blackwell_kernels" or key.startswith("blackwell_kernels."):
                     del sys.modules[key]
@@ -178,7 +184,7 @@ class BlackwellSimulator:
             circuit = pm.run(circuit)
             instrs = instructions(circuit)
         ops = C.compile_ops(instrs)
-        cc = CompiledCircuit(circuit.num_qubits, circuit.num_clbits, ops, global_phase(circuit))
+        cc = CompiledCircuit(circuit.num_qubits, circuit.num_clbits, ops, global_phase(circuit), bk=self.bk)
         self._compile_cache[key] = cc
         return cc
 

@@ -145,3 +145,30 @@ def test_python_layer_raises_runtime_error():
         bk.expectation_z(torch.zeros(4, dtype=torch.complex64), 1)
     with pytest.raises(RuntimeError):
         bk.apply_depolarizing(torch.zeros(4, dtype=torch.complex64), 0, 0.1, 0.5)
+
+
+def test_kernel_path_loading(tmp_path, monkeypatch):
+    """The simulator finds its kernel module by kernel_path, then
+    $BLACKWELL_KERNEL_PATH, then its own package (simulator.py:46-63 of the
+    reference); a second build at another path replaces the loaded one, as
+    the reference's kernel A/B harness does (tests/kernel_ab.py:183-230)."""
+    import shutil
+    import sys
+    from blackwell_backend.simulator import _load_kernels
+    pkg = os.path.join(ROOT, "qiskit-blackwell_amd")
+    bk = _load_kernels(pkg)
+    assert os.path.abspath(bk.__file__).startswith(os.path.abspath(pkg))
+    other = tmp_path / "build_b"
+    shutil.copytree(os.path.join(pkg, "blackwell_kernels"), other / "blackwell_kernels",
+                    ignore=shutil.ignore_patterns("__pycache__"))
+    try:
+        bk_b = _load_kernels(str(other))
+        assert os.path.abspath(bk_b.__file__).startswith(str(other))
+        assert bk_b._lib.load().qsim_abi_version() == 1
+        monkeypatch.setenv("BLACKWELL_KERNEL_PATH", pkg)
+        bk_env = _load_kernels(None)
+        assert os.path.abspath(bk_env.__file__).startswith(os.path.abspath(pkg))
+    finally:
+        if str(other) in sys.path:
+            sys.path.remove(str(other))
+        _load_kernels(pkg)

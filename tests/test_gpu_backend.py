@@ -321,3 +321,32 @@ def test_tune_mode_picks_a_measured_mode(sim):
     assert mode in ("plain", "whole", "blocks")
     res = tuned.run(qc, shots=1000, seed=3)
     assert sum(res.get_counts().values()) == 1000
+
+
+def test_two_kernel_builds_side_by_side(tmp_path):
+    """The reference's kernel A/B harness (tests/kernel_ab.py:100-230) runs
+    two simulators on two kernel builds: here a copy of this build at another
+    path; each simulator's plans launch through its own library, and the two
+    agree bit for bit."""
+    import os
+    import shutil
+    import sys
+    from blackwell_backend import BlackwellSimulator, workloads as W
+    pkg = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "qiskit-blackwell_amd")
+    other = tmp_path / "build_b"
+    shutil.copytree(os.path.join(pkg, "blackwell_kernels"), other / "blackwell_kernels",
+                    ignore=shutil.ignore_patterns("__pycache__"))
+    try:
+        sim_a = BlackwellSimulator("cuda:0", kernel_path=pkg)
+        sim_b = BlackwellSimulator("cuda:0", kernel_path=str(other))
+        assert sim_b.bk._lib.LIB_PATH.startswith(str(other)) and sim_a.bk is not sim_b.bk
+        for qc in (W.qft(10), W.su4_volume(14, 8, 3), W.c3_rotation_mcx(12, 2, 5)):
+            assert sim_b.compile(qc).bk is sim_b.bk
+            a, _ = sim_a.run_statevector(qc)
+            b, _ = sim_b.run_statevector(qc)
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), qc.name
+    finally:
+        if str(other) in sys.path:
+            sys.path.remove(str(other))
+        from blackwell_backend.simulator import _load_kernels
+        _load_kernels(pkg)
