@@ -217,6 +217,29 @@ int qsim_sample(void* stream, const void* state, int n_qubits, const void* unifo
  * (new)                                                                  */
 int qsim_uniforms_splitmix(void* stream, uint64_t seed, int64_t count, void* uniforms_dev);
 
+/* One state of a batched sampling call (new): .state / .out are device
+ * addresses of a complex64 state of n_qubits (8..20) and of its int64[shots]
+ * output, .seed its SplitMix64 seed (as qsim_uniforms_splitmix), and
+ * .leaf_off / .shot_off the prefix sums of 2^(n_qubits - 8) and of shots
+ * over the records before it.                                            */
+typedef struct qsim_sample_rec {
+  uint64_t state;
+  uint64_t out;
+  uint64_t seed;
+  uint64_t leaf_off;
+  uint64_t shot_off;
+  int32_t  n_qubits;
+  uint32_t shots;
+} qsim_sample_rec;
+int64_t qsim_sample_batch_workspace_bytes(int64_t n_leaves, int64_t n_states);
+/* qsim_sample on each of n_states states with its seeded uniforms, in three
+ * launches for all of them; every state's samples are bitwise those of
+ * qsim_uniforms_splitmix + qsim_sample on it alone.  recs_host is validated
+ * and must hold the bytes of the device copy recs_dev.  (new; the serving
+ * HBM lane)                                                              */
+int qsim_sample_batch(void* stream, const qsim_sample_rec* recs_host, const void* recs_dev,
+                      int64_t n_states, void* workspace);
+
 /* ---- Pauli strings, expectation values, noise channels ------------------ */
 /* A Pauli string is (x_mask, z_mask): qubit q carries X if only x bit q is
  * set, Z if only z bit q, Y if both; P = i^popcount(x&z) X^x Z^z.

@@ -39,13 +39,15 @@ import numpy as np
 import torch
 
 from blackwell_kernels import _lib
-from blackwell_kernels._lib import call, BLOCK_REC_DTYPE, OP_DTYPE, INSTANCE_DTYPE
+from blackwell_kernels._lib import call, BLOCK_REC_DTYPE, OP_DTYPE, INSTANCE_DTYPE, SAMPLE_REC_DTYPE
 from . import compiler as C
 from .circuit import instructions
 from .executor import CompiledCircuit, tile_ops
 from . import workloads as W
 
 LDS_MAX_QUBITS = 14
+# HBM-lane states up to this size are sampled together (qsim_sample_batch)
+SAMPLE_BATCH_MAX_QUBITS = 20
 GOLDEN = 0x9E3779B97F4A7C15
 
 
@@ -317,7 +319,8 @@ class _HbmBatch:
 
     def launch(self):
         from blackwell_kernels import sample_with_uniforms, uniforms_splitmix
-        from blackwell_kernels.cuquantum import block_step_records, run_block_steps
+        from blackwell_kernels.cuquantum import (block_step_records, run_block_steps, run_sample_batch,
+                                                 sample_records)
         for j in self.jobs:
             if j.prepared is not None:
                 j.prepared.result()
@@ -327,39 +330,47 @@ class _HbmBatch:
         sh = ctypes.c_void_p(self.stream.cuda_stream)
         ns = [r.n_qubits for r in self.reqs]
         mats = [j.blocks[1] for j in self.jobs]
-        mat_bytes = [m.nbytes for m in mats]
-        moff = np.concatenate([[0], np.cumsum(mat_bytes)]).astype(np.int64)
+        moff = np.concatenate([[0], np.cumsum([m.nbytes for m in mats])]).astype(np.int64)
         n_rec = sum(len(m) for m in mats)
-        rec_size = BLOCK_REC_DTYPE.itemsize
-        total = int(moff[-1]) + n_rec * rec_size
+        small = [i for i, n in enumerate(ns) if n <= SAMPLE_BATCH_MAX_QUBITS]
+        rec_at = int(moff[-1])
+        srec_at = rec_at + n_rec * BLOCK_REC_DTYPE.itemsize
+        total = srec_at + len(small) * SAMPLE_REC_DTYPE.itemsize
         with torch.cuda.stream(self.stream):
-            sizes = [1 << n for n in ns]
-            soff = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+            soff = np.concatenate([[0], np.cumsum([1 << n for n in ns])]).astype(np.int64)
+            shot_off = np.concatenate([[0], np.cumsum([r.shots for r in self.reqs])]).astype(np.int64)
             self.slab = torch.empty(int(soff[-1]), dtype=torch.complex64, device=dev)
             self.d_buf = torch.empty(max(total, 1), dtype=torch.uint8, device=dev)
-            base = self.slab.data_ptr()
+            self.d_out = torch.empty(max(int(shot_off[-1]), 1), dtype=torch.int64, device=dev)
+            base, obase = self.slab.data_ptr(), self.d_out.data_ptr()
             recs, off = block_step_records([base + 8 * int(o) for o in soff[:-1]], ns,
                                            [self.d_buf.data_ptr() + int(o) for o in moff[:-1]],
                                            [j.blocks[0] for j in self.jobs])
+            srecs, leaves = sample_records([base + 8 * int(soff[i]) for i in small], [ns[i] for i in small],
+                                           [obase + 8 * int(shot_off[i]) for i in small],
+                                           [self.reqs[i].seed for i in small],
+                                           [self.reqs[i].shots for i in small])
             self.staging = self.pool.acquire(total)
             host = self.staging.numpy()
             for m, o in zip(mats, moff[:-1]):
                 host[int(o):int(o) + m.nbytes] = m.view(np.uint8).reshape(-1)
-            host[int(moff[-1]):total] = recs.view(np.uint8)
+            host[rec_at:srec_at] = recs.view(np.uint8)
+            host[srec_at:total] = srecs.view(np.uint8)
             self.d_buf[:total].copy_(self.staging[:total], non_blocking=True)
-            self.recs = recs                       # validated host copy of the records
+            self.recs, self.srecs = recs, srecs     # validated host copies of the records
             for n, o in zip(ns, soff[:-1]):
                 call("qsim_state_init", sh, ctypes.c_void_p(base + 8 * int(o)), n)
-            run_block_steps(sh, recs, self.d_buf[int(moff[-1]):], off)
-            shot_off = np.concatenate([[0], np.cumsum([r.shots for r in self.reqs])]).astype(np.int64)
-            self.h_out = torch.empty(max(int(shot_off[-1]), 1), dtype=torch.int64, pin_memory=True)
-            self.d_outs = []
-            for r, o, so in zip(self.reqs, soff[:-1], shot_off[:-1]):
-                st = self.slab[int(o):int(o) + (1 << r.n_qubits)]
-                u = uniforms_splitmix(r.seed, r.shots, dev)
-                d = sample_with_uniforms(st, u)
-                self.d_outs.append(d)
-                self.h_out[int(so):int(so) + r.shots].copy_(d, non_blocking=True)
+            run_block_steps(sh, recs, self.d_buf[rec_at:], off)
+            if small:
+                run_sample_batch(sh, srecs, self.d_buf[srec_at:], leaves, dev)
+            for i, r in enumerate(self.reqs):
+                if r.n_qubits <= SAMPLE_BATCH_MAX_QUBITS:
+                    continue
+                st = self.slab[int(soff[i]):int(soff[i]) + (1 << r.n_qubits)]
+                d = sample_with_uniforms(st, uniforms_splitmix(r.seed, r.shots, dev))
+                self.d_out[int(shot_off[i]):int(shot_off[i]) + r.shots].copy_(d)
+            self.h_out = torch.empty(self.d_out.numel(), dtype=torch.int64, pin_memory=True)
+            self.h_out.copy_(self.d_out, non_blocking=True)
             self.shot_off = shot_off
             self.event = torch.cuda.Event()
             self.event.record(self.stream)
@@ -373,7 +384,7 @@ class _HbmBatch:
         for r, so in zip(self.reqs, self.shot_off[:-1]):
             r.samples = out[int(so):int(so) + r.shots].copy()
             r.t_done = now
-        self.slab = self.d_buf = self.d_outs = None
+        self.slab = self.d_buf = self.d_out = None
         self.pool.release(self.staging)
         self.staging = None
 

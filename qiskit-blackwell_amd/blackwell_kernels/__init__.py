@@ -9,7 +9,7 @@ __version__ = "0.1.0"
 from .cuquantum import (
     apply_gate, apply_gate_2q, apply_gates_fused, apply_gates_reg_tiled,
     apply_diagonal, apply_diagonal_phase, apply_diagonal_2q, state_init,
-    measure_qubit, measure_probs, sample, sample_with_uniforms, uniforms_splitmix, norm2,
+    measure_qubit, measure_probs, sample, sample_with_uniforms, sample_batch, uniforms_splitmix, norm2,
     expectation_z, expectation_pauli, expectation_paulis, expectation_pauli_string,
     apply_pauli_string,
     apply_mcgate, apply_mc2qgate, apply_mcx, apply_mcswap, apply_block, apply_blocks_batched, run_tile_program,

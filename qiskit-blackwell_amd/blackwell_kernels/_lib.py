@@ -61,6 +61,8 @@ PROTOTYPES = {
     "qsim_sample_workspace_bytes": (_i64, [_i]),
     "qsim_sample": (_i, [_vp, _vp, _i, _vp, _i64, _vp, _vp]),
     "qsim_uniforms_splitmix": (_i, [_vp, _u64, _i64, _vp]),
+    "qsim_sample_batch_workspace_bytes": (_i64, [_i64, _i64]),
+    "qsim_sample_batch": (_i, [_vp, _vp, _vp, _i64, _vp]),
     "qsim_expectation_workspace_bytes": (_i64, [_i]),
     "qsim_expectation_paulis": (_i, [_vp, _vp, _i, _i64, _vp, _vp, _vp, _vp]),
     "qsim_expectation_z": (_i, [_vp, _vp, _i, _u64, _vp, _vp]),
@@ -94,6 +96,9 @@ INSTANCE_DTYPE = np.dtype([("state_off", "<u8"), ("seed", "<u8"), ("shot_off", "
                            ("n_qubits", "u1"), ("init", "u1"), ("want", "u1"), ("pad", "u1")])
 BLOCK_REC_DTYPE = np.dtype([("state", "<u8"), ("mat", "<u8"), ("set_begin", "<u8"),
                             ("n_qubits", "<i4"), ("q", "i1", (5,)), ("pad", "i1", (7,))])
+SAMPLE_REC_DTYPE = np.dtype([("state", "<u8"), ("out", "<u8"), ("seed", "<u8"), ("leaf_off", "<u8"),
+                             ("shot_off", "<u8"), ("n_qubits", "<i4"), ("shots", "<u4")])
+assert SAMPLE_REC_DTYPE.itemsize == 48
 assert OP_DTYPE.itemsize == 16 and INSTANCE_DTYPE.itemsize == 40 and BLOCK_REC_DTYPE.itemsize == 40
 
 OP_M1, OP_M2, OP_D1, OP_D2, OP_PH, OP_X, OP_SWAP, OP_MK, OP_DK = range(1, 10)

@@ -401,6 +401,50 @@ def uniforms_splitmix(seed, count, device="cuda"):
     return out
 
 
+def sample_records(state_ptrs, n_qubits, out_ptrs, seeds, shots):
+    """Records (numpy SAMPLE_REC_DTYPE) of one batched sampling call and the
+    number of leaves its workspace must hold."""
+    nq = np.asarray(n_qubits, dtype=np.int64)
+    sh = np.asarray(shots, dtype=np.int64)
+    recs = np.zeros(nq.size, dtype=_lib.SAMPLE_REC_DTYPE)
+    recs["state"] = np.asarray(state_ptrs, dtype=np.uint64)
+    recs["out"] = np.asarray(out_ptrs, dtype=np.uint64)
+    recs["seed"] = np.asarray([int(x) & 0xFFFFFFFFFFFFFFFF for x in seeds], dtype=np.uint64)
+    leaves = np.left_shift(np.int64(1), nq - 8)
+    recs["leaf_off"] = (np.cumsum(leaves) - leaves).astype(np.uint64)
+    recs["shot_off"] = (np.cumsum(sh) - sh).astype(np.uint64)
+    recs["n_qubits"] = nq
+    recs["shots"] = sh.astype(np.uint32)
+    return recs, int(leaves.sum())
+
+
+def run_sample_batch(stream, recs, recs_dev, n_leaves, device):
+    """qsim_sample_batch over host records `recs` (device copy recs_dev)."""
+    ws = _lib.workspace("sample_batch", _lib.load().qsim_sample_batch_workspace_bytes(n_leaves, len(recs)),
+                        device)
+    call("qsim_sample_batch", stream, _lib.ctypes.c_void_p(recs.ctypes.data),
+         _lib.ctypes.c_void_p(recs_dev.data_ptr()), len(recs), ptr(ws))
+
+
+def sample_batch(states, seeds, shots):
+    """int64 device tensors of samples for each state (8..20 qubits), drawn
+    with the seeded SplitMix64 uniforms of uniforms_splitmix(seed, shots):
+    bitwise sample_with_uniforms(state, uniforms_splitmix(seed, shots)) for
+    every state, in three launches for all of them."""
+    if not states:
+        return []
+    for st in states:
+        _check_state(st)
+    dev = states[0].device
+    ns = [n_qubits_of(st) for st in states]
+    outs = [torch.empty(int(k), dtype=torch.int64, device=dev) for k in shots]
+    recs, leaves = sample_records([st.data_ptr() for st in states], ns,
+                                  [o.data_ptr() for o in outs], seeds, shots)
+    recs_dev = torch.from_numpy(recs.view(np.uint8).copy()).to(dev)
+    run_sample_batch(stream_handle(dev), recs, recs_dev, leaves, dev)
+    return outs
+
+
 def sample(state, n_shots):
     """n_shots indices from |state|^2 (cuquantum.py:283-294); uniforms from
     torch's generator on the state's device, in FP64."""

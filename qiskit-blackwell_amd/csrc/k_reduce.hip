@@ -331,10 +331,9 @@ __global__ __launch_bounds__(kThreads) void k_leaf_prefix(const double* __restri
 // States of at most kScanSeg leaves (n <= 20): the three scan passes in one
 // workgroup, with the same operations in the same order (the total is the
 // segment sum of k_seg_sum; the single segment's exclusive offset is 0).
-__global__ __launch_bounds__(kThreads) void k_scan_small(const double* __restrict__ tot, uint64_t nl,
-                                                         double* __restrict__ pre,
-                                                         double* __restrict__ total) {
-  __shared__ double sh[4];
+__device__ __forceinline__ void scan_small(const double* __restrict__ tot, uint64_t nl,
+                                           double* __restrict__ pre, double* __restrict__ total,
+                                           double* sh) {
   double s = 0.0;
   for (int k = threadIdx.x; k < kScanSeg; k += kThreads)
     if ((uint64_t)k < nl) s += tot[k];
@@ -354,6 +353,13 @@ __global__ __launch_bounds__(kThreads) void k_scan_small(const double* __restric
   if (threadIdx.x == 0) *total = r;
 }
 
+__global__ __launch_bounds__(kThreads) void k_scan_small(const double* __restrict__ tot, uint64_t nl,
+                                                         double* __restrict__ pre,
+                                                         double* __restrict__ total) {
+  __shared__ double sh[4];
+  scan_small(tot, nl, pre, total, sh);
+}
+
 // Last amplitude of the leaf with nonzero probability (-1 if none).
 __device__ __forceinline__ int64_t leaf_last_nonzero(const LeafScan& L, int lane) {
   const uint64_t hi = __ballot(L.q[2] > 0.0 || L.q[3] > 0.0);
@@ -367,20 +373,14 @@ __device__ __forceinline__ int64_t leaf_last_nonzero(const LeafScan& L, int lane
   return 2 * fl + (__shfl(L.q[1] > 0.0 ? 1 : 0, fl, 64));
 }
 
-// One wave per shot.  Leaf j = largest j with pre[j] <= x; a leaf with zero
-// total reached by rounding steps back to the previous nonzero leaf, and an
-// x past the leaf's own total (rounding again) takes the leaf's last nonzero
-// amplitude, so a zero-probability index is never returned.
-__global__ __launch_bounds__(kThreads) void k_sample(const float4* __restrict__ st, uint64_t n_amps,
-                                                     uint64_t nl, const double* __restrict__ pre,
-                                                     const double* __restrict__ tot,
-                                                     const double* __restrict__ total,
-                                                     const double* __restrict__ u, uint64_t shots,
-                                                     int64_t* __restrict__ out) {
-  const int lane = threadIdx.x & 63;
-  const uint64_t shot = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (shot >= shots) return;
-  const double x = u[shot] * total[0];
+// Index sampled for x = u * total by one wave.  Leaf j = largest j with
+// pre[j] <= x; a leaf with zero total reached by rounding steps back to the
+// previous nonzero leaf, and an x past the leaf's own total (rounding again)
+// takes the leaf's last nonzero amplitude, so a zero-probability index is
+// never returned.
+__device__ __forceinline__ int64_t sample_one(const float4* __restrict__ st, uint64_t n_amps,
+                                              uint64_t nl, const double* __restrict__ pre,
+                                              const double* __restrict__ tot, double x, int lane) {
   uint64_t lo = 0, hi = nl - 1;
   while (lo < hi) {
     const uint64_t mid = (lo + hi + 1) >> 1;
@@ -408,7 +408,81 @@ __global__ __launch_bounds__(kThreads) void k_sample(const float4* __restrict__ 
     local = leaf_last_nonzero(L, lane);
     if (local < 0) local = 0;  // all-zero state: index 0 (as the reference)
   }
-  if (lane == 0) out[shot] = (int64_t)(leaf * kLeaf) + local;
+  return (int64_t)(leaf * kLeaf) + local;
+}
+
+// One wave per shot.
+__global__ __launch_bounds__(kThreads) void k_sample(const float4* __restrict__ st, uint64_t n_amps,
+                                                     uint64_t nl, const double* __restrict__ pre,
+                                                     const double* __restrict__ tot,
+                                                     const double* __restrict__ total,
+                                                     const double* __restrict__ u, uint64_t shots,
+                                                     int64_t* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t shot = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (shot >= shots) return;
+  const int64_t idx = sample_one(st, n_amps, nl, pre, tot, u[shot] * total[0], lane);
+  if (lane == 0) out[shot] = idx;
+}
+
+// ---------------------------------------------------------------- batched sampling
+// Many states at once (the serving HBM lane): qsim_sample_rec r names a
+// state of 8..20 qubits, its output and seed, and the prefix offsets of its
+// leaves and shots.  Each state's leaf totals, scan and searches are exactly
+// those of qsim_sample on it alone, with u_k = splitmix_u(seed, k) drawn in
+// the kernel as qsim_uniforms_splitmix draws them.
+__device__ __forceinline__ int find_rec(const qsim_sample_rec* __restrict__ recs, int n,
+                                        uint64_t v, bool by_leaf) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const uint64_t key = by_leaf ? recs[mid].leaf_off : recs[mid].shot_off;
+    if (key <= v) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kThreads) void k_leaf_totals_batch(const qsim_sample_rec* __restrict__ recs,
+                                                                int n_recs, uint64_t n_leaves,
+                                                                double* __restrict__ tot) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t leaf = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (leaf >= n_leaves) return;
+  const qsim_sample_rec& r = recs[find_rec(recs, n_recs, leaf, true)];
+  const float4* lf = reinterpret_cast<const float4*>(r.state) + (leaf - r.leaf_off) * (kLeaf / 2);
+  const float4 a = ld_nt(lf + lane), b = ld_nt(lf + 64 + lane);
+  double v = p2(lo2(a)) + p2(hi2(a));
+  v += p2(lo2(b)) + p2(hi2(b));
+  v = wave_sum(v);
+  if (lane == 0) tot[leaf] = v;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_small_batch(const qsim_sample_rec* __restrict__ recs,
+                                                               const double* __restrict__ tot,
+                                                               double* __restrict__ pre,
+                                                               double* __restrict__ totals) {
+  __shared__ double sh[4];
+  const qsim_sample_rec& r = recs[blockIdx.x];
+  const uint64_t nl = 1ull << (r.n_qubits - 8);
+  scan_small(tot + r.leaf_off, nl, pre + r.leaf_off, totals + blockIdx.x, sh);
+}
+
+__global__ __launch_bounds__(kThreads) void k_sample_batch(const qsim_sample_rec* __restrict__ recs,
+                                                           int n_recs, uint64_t n_shots,
+                                                           const double* __restrict__ tot,
+                                                           const double* __restrict__ pre,
+                                                           const double* __restrict__ totals) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t shot = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (shot >= n_shots) return;
+  const int ri = find_rec(recs, n_recs, shot, false);
+  const qsim_sample_rec& r = recs[ri];
+  const uint64_t k = shot - r.shot_off;
+  const double x = splitmix_u(r.seed, k) * totals[ri];
+  const uint64_t n_amps = 1ull << r.n_qubits;
+  const int64_t idx = sample_one(reinterpret_cast<const float4*>(r.state), n_amps, n_amps / kLeaf,
+                                 pre + r.leaf_off, tot + r.leaf_off, x, lane);
+  if (lane == 0) reinterpret_cast<int64_t*>(r.out)[k] = idx;
 }
 
 struct SamplePlan {
@@ -532,6 +606,50 @@ int qsim_sample(void* stream, const void* state, int n_qubits, const void* unifo
                      (const double*)uniforms_dev,
                      (uint64_t)n_shots, (int64_t*)out_dev);
   return launch_status("sample: search");
+}
+
+// Layout (doubles): tot[n_leaves] | pre[n_leaves] | totals[n_states]
+int64_t qsim_sample_batch_workspace_bytes(int64_t n_leaves, int64_t n_states) {
+  if (n_leaves < 0 || n_states < 0) return -1;
+  return (2 * n_leaves + n_states + 1) * (int64_t)sizeof(double);
+}
+
+int qsim_sample_batch(void* stream, const qsim_sample_rec* recs_host, const void* recs_dev,
+                      int64_t n_states, void* workspace) {
+  clear_error();
+  if (n_states < 0 || n_states > (1 << 24)) return fail(QSIM_ERANGE, "n_states = %lld out of range",
+                                                      (long long)n_states);
+  if (n_states == 0) return QSIM_OK;
+  if (!recs_host || !recs_dev || !workspace) return fail(QSIM_EINVAL, "NULL pointer argument");
+  uint64_t leaves = 0, shots = 0;
+  for (int64_t i = 0; i < n_states; ++i) {
+    const qsim_sample_rec& r = recs_host[i];
+    if (r.n_qubits < 8 || r.n_qubits > 20)
+      return fail(QSIM_ERANGE, "record %lld: batched sampling needs 8 <= n_qubits <= 20 (got %d)",
+                  (long long)i, r.n_qubits);
+    if (!r.state || (r.shots && !r.out)) return fail(QSIM_EINVAL, "record %lld: NULL state or output",
+                                                     (long long)i);
+    if (r.leaf_off != leaves || r.shot_off != shots)
+      return fail(QSIM_EINVAL, "record %lld: leaf_off/shot_off must be the prefix sums of "
+                  "2^(n_qubits-8) and shots", (long long)i);
+    leaves += 1ull << (r.n_qubits - 8);
+    shots += r.shots;
+  }
+  hipStream_t s = as_stream(stream);
+  const qsim_sample_rec* rd = (const qsim_sample_rec*)recs_dev;
+  double* tot = (double*)workspace;
+  double* pre = tot + leaves;
+  double* totals = pre + leaves;
+  hipLaunchKernelGGL(k_leaf_totals_batch, dim3((unsigned)ceil_div(leaves, 4)), dim3(kThreads), 0, s, rd,
+                     (int)n_states, leaves, tot);
+  if (int rc = launch_status("sample_batch: leaf totals")) return rc;
+  hipLaunchKernelGGL(k_scan_small_batch, dim3((unsigned)n_states), dim3(kThreads), 0, s, rd, tot, pre,
+                     totals);
+  if (int rc = launch_status("sample_batch: scans")) return rc;
+  if (shots == 0) return QSIM_OK;
+  hipLaunchKernelGGL(k_sample_batch, dim3((unsigned)ceil_div(shots, 4)), dim3(kThreads), 0, s, rd,
+                     (int)n_states, shots, tot, pre, totals);
+  return launch_status("sample_batch: search");
 }
 
 int qsim_uniforms_splitmix(void* stream, uint64_t seed, int64_t count, void* uniforms_dev) {

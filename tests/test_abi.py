@@ -60,6 +60,9 @@ int main(void) {
   printf("%zu %zu %zu %zu %zu\n", sizeof(qsim_block_rec), offsetof(qsim_block_rec, mat),
          offsetof(qsim_block_rec, set_begin), offsetof(qsim_block_rec, n_qubits),
          offsetof(qsim_block_rec, q));
+  printf("%zu %zu %zu %zu %zu\n", sizeof(qsim_sample_rec), offsetof(qsim_sample_rec, seed),
+         offsetof(qsim_sample_rec, shot_off), offsetof(qsim_sample_rec, n_qubits),
+         offsetof(qsim_sample_rec, shots));
   return 0;
 }'''
     with tempfile.TemporaryDirectory() as d:
@@ -67,10 +70,13 @@ int main(void) {
         exe = os.path.join(d, "probe")
         open(c, "w").write(prog)
         subprocess.run(["gcc", "-I", os.path.dirname(HEADER), c, "-o", exe], check=True)
-        l1, l2, l3 = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")[:3]
+        l1, l2, l3, l4 = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")[:4]
     op = [int(x) for x in l1.split()]
     ins = [int(x) for x in l2.split()]
     rec = [int(x) for x in l3.split()]
+    sd = _lib.SAMPLE_REC_DTYPE
+    assert [int(x) for x in l4.split()] == [sd.itemsize, sd.fields["seed"][1], sd.fields["shot_off"][1],
+                                            sd.fields["n_qubits"][1], sd.fields["shots"][1]]
     bd = _lib.BLOCK_REC_DTYPE
     assert rec == [bd.itemsize, bd.fields["mat"][1], bd.fields["set_begin"][1], bd.fields["n_qubits"][1],
                    bd.fields["q"][1]]
@@ -134,6 +140,15 @@ def test_meta_and_validation_without_device():
     recs[1]["q"] = [0, 1, 2, 3, 4]
     recs[1]["n_qubits"] = 9
     assert lib.qsim_apply_block_batch(None, rp, v8, 2) == 2
+    from blackwell_kernels.cuquantum import sample_records
+    srecs, leaves = sample_records([8, 8], [10, 12], [8, 8], [1, 2], [16, 0])
+    assert leaves == 4 + 16 and list(srecs["leaf_off"]) == [0, 4] and list(srecs["shot_off"]) == [0, 16]
+    sp_ = srecs.ctypes.data_as(ctypes.c_void_p)
+    srecs[1]["leaf_off"] = 3
+    assert lib.qsim_sample_batch(None, sp_, v8, 2, v8) == 1
+    srecs[1]["leaf_off"] = 4
+    srecs[1]["n_qubits"] = 21
+    assert lib.qsim_sample_batch(None, sp_, v8, 2, v8) == 2
 
 
 def test_python_layer_raises_runtime_error():

@@ -500,3 +500,23 @@ def test_block_batch_equals_per_instance_blocks(bk, dev):
     for k in range(counts[2]):
         psi = O.apply_matrix(psi, mats[2][k].cpu().numpy().astype(np.complex128), list(qubits[2][k][:5]))
     assert _rel_err(psi, states[2].cpu().numpy()) < 8e-6 * counts[2]
+
+
+def test_sample_batch_equals_per_state_sampling(bk, dev):
+    """qsim_sample_batch gives, for every state, bitwise the samples of
+    uniforms_splitmix + sample_with_uniforms on that state alone, including
+    zero-shot records and states with zero-probability leaves."""
+    ns = [8, 12, 10, 20, 15, 9]
+    shots = [1000, 0, 4096, 777, 2048, 1]
+    seeds = [3, 5, 7, 11, 13, 2 ** 63 + 17]
+    states = []
+    for i, n in enumerate(ns):
+        psi = rand_state(n, 800 + i)
+        if i == 2:
+            psi[: 1 << (n - 1)] = 0            # first half of the leaves all zero
+            psi /= np.linalg.norm(psi)
+        states.append(to_dev(psi.astype(np.complex64), dev))
+    got = bk.sample_batch(states, seeds, shots)
+    for st, sd, k, g in zip(states, seeds, shots, got):
+        ref = bk.sample_with_uniforms(st, bk.uniforms_splitmix(sd, k, dev))
+        assert g.numel() == k and torch.equal(g, ref)
