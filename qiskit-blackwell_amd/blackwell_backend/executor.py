@@ -34,6 +34,39 @@ BLOCK_US = 5.0
 # device-resident arguments only (no host uploads, no host-side decisions)
 GRAPH_SAFE_STEPS = ("blocks", "tile", "mcgate")
 TUNE_REPS = 3
+# Tile programs above 13 qubits: a tile of m qubits is one workgroup, so m
+# trades ops fused per HBM pass against workgroups per pass (2^(n-m)).
+# Measured on MI355X (A/B circuit families, 16 and 20 qubits): 10-qubit
+# tiles beat 13-qubit ones by 2-3x there (13-qubit tiles give 8-128
+# workgroups); larger states can afford larger tiles.
+AUTO_TILES_MAX_QUBITS = 20   # auto considers tile programs up to this size
+# auto-mode cost model for 14 <= n <= 20 (same measurements): a tile pass
+# ~5 us plus ~0.85 us per op at 16 qubits (x1.13 per qubit; a dense 4x4 op
+# counts 1.4, a wider dense op 2); a fused block ~8.2 us at 16 qubits (x1.07
+# per qubit)
+TILE_PASS_US = 5.0
+TILE_OP_US_16Q = 0.85
+TILE_OP_WEIGHT = {"m2": 1.4, "mk": 2.0}
+BLOCK_US_16Q = 8.2
+
+
+def default_tile_shape(n):
+    """(tile qubits, always-included low qubits) of "tiles" for n qubits."""
+    if n <= WHOLE_TILE_MAX:
+        return n, min(4, n)
+    m = 10 if n <= 20 else (11 if n <= 22 else (12 if n <= 24 else 13))
+    return m, 4
+
+
+def tune_tile_shapes(n):
+    """Tile shapes the tuner times for n > 13 qubits."""
+    m, low = default_tile_shape(n)
+    shapes = [(m, low), (m, 3), (m - 1, low), (min(m + 1, WHOLE_TILE_MAX), low), (WHOLE_TILE_MAX, 4)]
+    out = []
+    for sh in shapes:
+        if sh not in out and 1 <= sh[1] <= sh[0] <= WHOLE_TILE_MAX:
+            out.append(sh)
+    return out
 
 
 def _d8(m):
@@ -116,6 +149,7 @@ class CompiledCircuit:
         self.phase0 = None if phi == 0.0 else complex(np.cos(phi), np.sin(phi))
         self._plans = {}
         self._blocks = None
+        self._tiles = None
         self._graphs = {}
         self._tuned = {}
         self.has_stochastic = any(o.kind in ("measure", "reset") for o in ops)
@@ -146,7 +180,9 @@ class CompiledCircuit:
     def tuned_mode(self, device):
         """The fastest execution mode for this circuit on `device`, measured
         once: every applicable mode runs once to warm up and TUNE_REPS times
-        timed with HIP events on the launch stream, from |0...0>."""
+        timed with HIP events on the launch stream, from |0...0>.  Above 13
+        qubits the candidates include LDS tile programs of several tile
+        shapes ("tiles/<tile qubits>/<low qubits>")."""
         key = str(device)
         if key in self._tuned:
             return self._tuned[key]
@@ -155,7 +191,7 @@ class CompiledCircuit:
         if n <= WHOLE_TILE_MAX:
             cands.append("whole")
         else:
-            cands.append("tiles")
+            cands += [f"tiles/{m}/{low}" for m, low in tune_tile_shapes(n)]
         if n >= BLOCK_MIN_QUBITS:
             cands.append("blocks")
         state = torch.empty(1 << n, dtype=torch.complex64, device=device)
@@ -195,11 +231,19 @@ class CompiledCircuit:
             self._plans[key] = p
         return p
 
+    def tile_entries(self):
+        """plan_tiles of the ops at the default tile shape (computed once)."""
+        if self._tiles is None:
+            m, low = default_tile_shape(self.n_qubits)
+            self._tiles = C.plan_tiles(self.ops, self.n_qubits, m=m, low=low)
+        return self._tiles
+
     def resolve_mode(self, mode, device=None):
-        """auto: the whole state as one LDS program below 10 qubits, MFMA
-        blocks above 13, and between the two whichever the cost model says
-        is cheaper (deep circuits favour blocks: one block absorbs many ops).
-        tune: the mode tuned_mode measured fastest on `device`."""
+        """auto: the whole state as one LDS program below 10 qubits; between
+        10 and 13 qubits that or MFMA blocks, and between 14 and 20 qubits
+        tile programs or MFMA blocks, whichever the cost model says is cheaper
+        (deep circuits favour blocks: one block absorbs many ops); MFMA blocks
+        above.  tune: the mode tuned_mode measured fastest on `device`."""
         if mode == "tune":
             if device is None:
                 raise ValueError("mode 'tune' needs the device to measure on")
@@ -209,8 +253,16 @@ class CompiledCircuit:
         n = self.n_qubits
         if n < BLOCK_MIN_QUBITS:
             return "whole"
-        if n > WHOLE_TILE_MAX:
+        if n > AUTO_TILES_MAX_QUBITS:
             return "blocks"
+        if n > WHOLE_TILE_MAX:
+            passes = sum(1 for e in self.tile_entries() if isinstance(e, C.TilePass))
+            others = len(self.tile_entries()) - passes
+            w_ops = sum(TILE_OP_WEIGHT.get(o.kind, 1.0) for o in self.ops)
+            t_tiles = (passes * TILE_PASS_US + w_ops * TILE_OP_US_16Q * 1.13 ** (n - 16)
+                       + others * BLOCK_US_16Q)
+            t_blocks = len(self.block_entries()) * BLOCK_US_16Q * 1.07 ** (n - 16)
+            return "tiles" if t_tiles < t_blocks else "blocks"
         t_whole = len(self.ops) * WHOLE_OP_US_12Q * 2.0 ** (n - 12)
         t_blocks = len(self.block_entries()) * BLOCK_US
         return "blocks" if t_blocks < t_whole else "whole"
@@ -226,7 +278,15 @@ class CompiledCircuit:
                 return self._build_tiles([C.TilePass(mask, self.ops)] if self.ops else [], device)
             mode = "tiles"
         if mode == "tiles":
-            return self._build_tiles(C.plan_tiles(self.ops, n, m=min(WHOLE_TILE_MAX, n)), device)
+            return self._build_tiles(self.tile_entries(), device)
+        if mode.startswith("tiles/"):
+            try:
+                m, low = (int(x) for x in mode.split("/")[1:])
+            except ValueError:
+                raise ValueError(f"tile mode must be 'tiles/<m>/<low>' (got {mode!r})") from None
+            if not (1 <= low <= m <= WHOLE_TILE_MAX):
+                raise ValueError(f"tile shape needs 1 <= low <= m <= {WHOLE_TILE_MAX} (got {mode!r})")
+            return self._build_tiles(C.plan_tiles(self.ops, n, m=min(m, n), low=low), device)
         if mode == "blocks":
             if n < 10:
                 return self._build_tiles(C.plan_tiles(self.ops, n, m=n), device)

@@ -103,8 +103,8 @@ class BlackwellSimulator:
     GRAPH_MAX_QUBITS = 20
 
     def __init__(self, device="cuda", kernel_path=None, mode="auto", use_graphs=True):
-        if mode not in self.MODES:
-            raise ValueError(f"mode must be one of {self.MODES}")
+        if mode not in self.MODES and not str(mode).startswith("tiles/"):
+            raise ValueError(f"mode must be one of {self.MODES} or 'tiles/<m>/<low>'")
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())

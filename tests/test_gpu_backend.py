@@ -321,6 +321,16 @@ def test_tune_mode_picks_a_measured_mode(sim):
     assert mode in ("plain", "whole", "blocks")
     res = tuned.run(qc, shots=1000, seed=3)
     assert sum(res.get_counts().values()) == 1000
+    # above 13 qubits the tuner also times tile shapes; every shape is exact
+    qc = W.qft(15)
+    ref = oracle_sv(qc)
+    mode = tuned.compile(qc).tuned_mode(tuned.device)
+    assert mode == "plain" or mode == "blocks" or mode.startswith("tiles/")
+    for m, low in ((13, 3), (12, 4), (11, 5)):
+        sv, _ = tuned.run_statevector(qc, mode=f"tiles/{m}/{low}")
+        check(ref, sv)
+    with pytest.raises(ValueError):
+        tuned.run_statevector(qc, mode="tiles/14/4")
 
 
 def test_two_kernel_builds_side_by_side(tmp_path):
