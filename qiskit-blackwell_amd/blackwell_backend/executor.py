@@ -150,6 +150,7 @@ class CompiledCircuit:
         self._plans = {}
         self._blocks = None
         self._tiles = None
+        self._auto = None
         self._graphs = {}
         self._tuned = {}
         self.has_stochastic = any(o.kind in ("measure", "reset") for o in ops)
@@ -250,6 +251,11 @@ class CompiledCircuit:
             return self.tuned_mode(device)
         if mode != "auto":
             return mode
+        if self._auto is None:
+            self._auto = self._auto_mode()
+        return self._auto
+
+    def _auto_mode(self):
         n = self.n_qubits
         if n < BLOCK_MIN_QUBITS:
             return "whole"
