@@ -305,17 +305,17 @@ static int pauli_apply_impl(hipStream_t s, void* state, int n, uint64_t x, uint6
   const int y = __builtin_popcountll(x & z) & 3;
   if (x == 0 && (z & 127) == 0) {
     const int hw = (63 - __builtin_clzll(z >> 7));
-    hipLaunchKernelGGL(k_pauli_apply<3>, dim3(ceil_div(windows / 2, 4)), dim3(kThreads), 0, s,
+    hipLaunchKernelGGL(k_pauli_apply<3>, grid1d(ceil_div(windows / 2, 4)), dim3(kThreads), 0, s,
                        (float4*)state, n4, windows / 2, x, z, y, hw);
   } else if (x == 0) {
-    hipLaunchKernelGGL(k_pauli_apply<0>, dim3(ceil_div(windows, 4)), dim3(kThreads), 0, s,
+    hipLaunchKernelGGL(k_pauli_apply<0>, grid1d(ceil_div(windows, 4)), dim3(kThreads), 0, s,
                        (float4*)state, n4, windows, x, z, y, 0);
   } else if (x < 128) {
-    hipLaunchKernelGGL(k_pauli_apply<1>, dim3(ceil_div(windows, 4)), dim3(kThreads), 0, s,
+    hipLaunchKernelGGL(k_pauli_apply<1>, grid1d(ceil_div(windows, 4)), dim3(kThreads), 0, s,
                        (float4*)state, n4, windows, x, z, y, 0);
   } else {
     const int hw = (63 - __builtin_clzll(x)) - 7;
-    hipLaunchKernelGGL(k_pauli_apply<2>, dim3(ceil_div(windows / 2, 4)), dim3(kThreads), 0, s,
+    hipLaunchKernelGGL(k_pauli_apply<2>, grid1d(ceil_div(windows / 2, 4)), dim3(kThreads), 0, s,
                        (float4*)state, n4, windows / 2, x, z, y, hw);
   }
   return launch_status("apply_pauli_string");

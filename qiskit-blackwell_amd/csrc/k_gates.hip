@@ -87,7 +87,7 @@ int build_mc(McSpec& s, int n, const int64_t* controls, int n_controls, const in
 // ============================================================ state init
 template <typename I>
 __global__ __launch_bounds__(kThreads) void k_init(float4* __restrict__ st, I n4, I per_state4) {
-  const I f = (I)blockIdx.x * kThreads + threadIdx.x;
+  const I f = gblock<I>() * kThreads + threadIdx.x;
   if (f >= n4) return;
   const bool first = (f & (per_state4 - 1)) == 0;
   st_nt(st + f, make_float4(first ? 1.f : 0.f, 0.f, 0.f, 0.f));
@@ -101,10 +101,10 @@ static int init_impl(void* stream, void* states, uint64_t n_states, int n) {
   const uint64_t n4 = per4 * n_states;
   const uint64_t blocks = ceil_div(n4, kThreads);
   if (n4 <= 0xffffffffull)
-    hipLaunchKernelGGL(k_init<uint32_t>, dim3(blocks), dim3(kThreads), 0, as_stream(stream),
+    hipLaunchKernelGGL(k_init<uint32_t>, grid1d(blocks), dim3(kThreads), 0, as_stream(stream),
                        (float4*)states, (uint32_t)n4, (uint32_t)per4);
   else
-    hipLaunchKernelGGL(k_init<uint64_t>, dim3(blocks), dim3(kThreads), 0, as_stream(stream),
+    hipLaunchKernelGGL(k_init<uint64_t>, grid1d(blocks), dim3(kThreads), 0, as_stream(stream),
                        (float4*)states, n4, per4);
   return launch_status("state_init");
 }
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(kThreads) void k_gate1q(float4* __restrict__ st, I 
                                                      const float2* __restrict__ gdev, M2 gv) {
   float2 g[4];
   load_m2(gdev, gv, g);
-  const I w0 = (MODE == 2 ? xcd_block<I>() : (I)blockIdx.x) * kThreads + threadIdx.x;
+  const I w0 = (MODE == 2 ? xcd_block<I>() : gblock<I>()) * kThreads + threadIdx.x;
   const bool live = w0 < n_work;
   const I w = live ? w0 : n_work - 1;
   if (MODE == 0) {
@@ -191,7 +191,7 @@ int gate1q_impl(void* stream, void* states, uint64_t n_states, int n, const floa
   const uint64_t slots = (1ull << n) / 2 * n_states;
   const int mode = t == 0 ? 0 : (t < 7 ? 1 : 2);
   const uint64_t work = mode == 2 ? slots / 2 : slots;
-  const dim3 grid(ceil_div(work, kThreads)), block(kThreads);
+  const dim3 grid = grid1d(ceil_div(work, kThreads)), block(kThreads);
   if (slots <= 0xffffffffull) {
     const uint32_t w = (uint32_t)work;
     if (mode == 0) hipLaunchKernelGGL((k_gate1q<uint32_t, 0>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I 
                                                      M4 gv) {
   float2 g[16];
   load_m4(gdev, gv, g);
-  const I w0 = (S > 0 ? xcd_block<I>() : (I)blockIdx.x) * kThreads + threadIdx.x;
+  const I w0 = (S > 0 ? xcd_block<I>() : gblock<I>()) * kThreads + threadIdx.x;
   const bool live = w0 < n_work;
   const I w = live ? w0 : n_work - 1;
   const int lane = threadIdx.x & 63;
@@ -334,7 +334,7 @@ int gate2q_impl(void* stream, void* state, int n, const float2* gdev, const M4& 
   hipStream_t s = as_stream(stream);
   const int S = (lo >= 7) + (hi >= 7);
   const uint64_t work = ((1ull << n) / 2) >> S;
-  const dim3 grid(ceil_div(work, kThreads)), block(kThreads);
+  const dim3 grid = grid1d(ceil_div(work, kThreads)), block(kThreads);
   if (n <= 32) {
     const uint32_t w = (uint32_t)work;
     if (S == 0) hipLaunchKernelGGL((k_gate2q<uint32_t, 0>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(kThreads) void k_diag(float4* __restrict__ st, I n4
   } else {
     d0 = dv.g[0]; d1 = dv.g[1]; d2 = dv.g[2]; d3 = dv.g[3];
   }
-  const I base = (I)blockIdx.x * (kThreads * U) + threadIdx.x;
+  const I base = gblock<I>() * (kThreads * U) + threadIdx.x;
   float4 v[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) {
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(kThreads) void k_diag(float4* __restrict__ st, I n4
 template <typename I, int U>
 __global__ __launch_bounds__(kThreads) void k_phase(float4* __restrict__ st, I n_items, int t,
                                                     float2 ph) {
-  const I base = (I)blockIdx.x * (kThreads * U) + threadIdx.x;
+  const I base = gblock<I>() * (kThreads * U) + threadIdx.x;
   const int s = t - 1;
   I f[U];
   float4 v[U];
@@ -429,17 +429,17 @@ static int diag_impl(void* stream, void* state, int n, int kind, int q0, int q1,
   hipStream_t s = as_stream(stream);
   if (n <= 32) {
     if (kind == 0)
-      hipLaunchKernelGGL((k_diag<uint32_t, kUd, 0>), dim3(blocks), dim3(kThreads), 0, s,
+      hipLaunchKernelGGL((k_diag<uint32_t, kUd, 0>), grid1d(blocks), dim3(kThreads), 0, s,
                          (float4*)state, (uint32_t)n4, t, hi, ddev, dv);
     else
-      hipLaunchKernelGGL((k_diag<uint32_t, kUd, 1>), dim3(blocks), dim3(kThreads), 0, s,
+      hipLaunchKernelGGL((k_diag<uint32_t, kUd, 1>), grid1d(blocks), dim3(kThreads), 0, s,
                          (float4*)state, (uint32_t)n4, t, hi, ddev, dv);
   } else {
     if (kind == 0)
-      hipLaunchKernelGGL((k_diag<uint64_t, kUd, 0>), dim3(blocks), dim3(kThreads), 0, s,
+      hipLaunchKernelGGL((k_diag<uint64_t, kUd, 0>), grid1d(blocks), dim3(kThreads), 0, s,
                          (float4*)state, n4, t, hi, ddev, dv);
     else
-      hipLaunchKernelGGL((k_diag<uint64_t, kUd, 1>), dim3(blocks), dim3(kThreads), 0, s,
+      hipLaunchKernelGGL((k_diag<uint64_t, kUd, 1>), grid1d(blocks), dim3(kThreads), 0, s,
                          (float4*)state, n4, t, hi, ddev, dv);
   }
   return launch_status("apply_diagonal");
@@ -452,10 +452,10 @@ static int phase_impl(void* stream, void* state, int n, float2 ph, int t) {
   const uint64_t items = t > 0 ? (1ull << n) / 4 : (1ull << n) / 2;
   const uint64_t blocks = ceil_div(items, (uint64_t)kThreads * kUd);
   if (n <= 32)
-    hipLaunchKernelGGL((k_phase<uint32_t, kUd>), dim3(blocks), dim3(kThreads), 0,
+    hipLaunchKernelGGL((k_phase<uint32_t, kUd>), grid1d(blocks), dim3(kThreads), 0,
                        as_stream(stream), (float4*)state, (uint32_t)items, t, ph);
   else
-    hipLaunchKernelGGL((k_phase<uint64_t, kUd>), dim3(blocks), dim3(kThreads), 0,
+    hipLaunchKernelGGL((k_phase<uint64_t, kUd>), grid1d(blocks), dim3(kThreads), 0,
                        as_stream(stream), (float4*)state, items, t, ph);
   return launch_status("apply_diagonal_phase");
 }
@@ -493,7 +493,7 @@ __global__ __launch_bounds__(kThreads) void k_mc(float2* __restrict__ st, I n_it
 #pragma unroll
     for (int k = 0; k < 16; ++k) g[k] = gdev[k];
   }
-  const I base = (I)blockIdx.x * (kThreads * U) + threadIdx.x;
+  const I base = gblock<I>() * (kThreads * U) + threadIdx.x;
   const I m0 = I(1) << a.t0;
   const I m1 = (a.op >= 2) ? (I(1) << a.t1) : I(0);
 #pragma unroll
@@ -536,7 +536,7 @@ __global__ __launch_bounds__(kThreads) void k_mc4(float4* __restrict__ st, I n_i
 #pragma unroll
     for (int k = 0; k < 16; ++k) g[k] = gdev[k];
   }
-  const I base = (I)blockIdx.x * (kThreads * U) + threadIdx.x;
+  const I base = gblock<I>() * (kThreads * U) + threadIdx.x;
   const I m0 = I(1) << a.t0;
   const I m1 = (a.op >= 2) ? (I(1) << a.t1) : I(0);
 #pragma unroll
@@ -656,7 +656,7 @@ __global__ __launch_bounds__(kThreads) void k_mc1(float4* __restrict__ st, I n_i
 template <typename I, int MODE, int OP>
 static void launch_mc1(hipStream_t s, void* state, uint64_t items, const Mc1Args& a, bool full,
                        const float2* gdev, const M2& gv) {
-  const dim3 grid(ceil_div(items, kThreads / 64)), block(kThreads);
+  const dim3 grid = grid1d(ceil_div(items, kThreads / 64)), block(kThreads);
   if (full)
     hipLaunchKernelGGL((k_mc1<I, MODE, OP, 1>), grid, block, 0, s, (float4*)state, (I)items, a, gdev, gv);
   else
@@ -738,19 +738,19 @@ static int mc_impl(void* stream, void* state, int n, const float2* gdev, const i
     a.t1 = op >= 2 ? a.t1 - 1 : 0;
     const uint64_t blocks = ceil_div(items, (uint64_t)kThreads * kUm);
     if (!wide)
-      hipLaunchKernelGGL((k_mc4<uint32_t, kUm>), dim3(blocks), dim3(kThreads), 0, s,
+      hipLaunchKernelGGL((k_mc4<uint32_t, kUm>), grid1d(blocks), dim3(kThreads), 0, s,
                          (float4*)state, (uint32_t)items, a, gdev);
     else
-      hipLaunchKernelGGL((k_mc4<uint64_t, kUm>), dim3(blocks), dim3(kThreads), 0, s,
+      hipLaunchKernelGGL((k_mc4<uint64_t, kUm>), grid1d(blocks), dim3(kThreads), 0, s,
                          (float4*)state, items, a, gdev);
   } else {
     const uint64_t items = 1ull << (n - spec.n_pos);
     const uint64_t blocks = ceil_div(items, (uint64_t)kThreads * kUm);
     if (!wide)
-      hipLaunchKernelGGL((k_mc<uint32_t, kUm>), dim3(blocks), dim3(kThreads), 0, s,
+      hipLaunchKernelGGL((k_mc<uint32_t, kUm>), grid1d(blocks), dim3(kThreads), 0, s,
                          (float2*)state, (uint32_t)items, a, gdev);
     else
-      hipLaunchKernelGGL((k_mc<uint64_t, kUm>), dim3(blocks), dim3(kThreads), 0, s,
+      hipLaunchKernelGGL((k_mc<uint64_t, kUm>), grid1d(blocks), dim3(kThreads), 0, s,
                          (float2*)state, items, a, gdev);
   }
   return launch_status("apply_mc");

@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kThreads) void k_collapse(float4* __restrict__ st, 
   const int o = (u * tot < p0) ? 0 : 1;
   const double po = o == 0 ? p0 : tot - p0;
   const double sc = po > 0.0 ? 1.0 / sqrt(po) : 0.0;
-  const uint64_t f = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  const uint64_t f = gblock<uint64_t>() * kThreads + threadIdx.x;
   if (f == 0) *outcome = o;
   if (f >= n4) return;
   float4 v = ld_nt(st + f);
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kThreads) void k_scale(float4* __restrict__ st, uin
                                                     const double* __restrict__ res) {
   const double tot = res[0];
   const double sc = tot > 0.0 ? 1.0 / sqrt(tot) : 0.0;
-  const uint64_t f = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  const uint64_t f = gblock<uint64_t>() * kThreads + threadIdx.x;
   if (f >= n4) return;
   float4 v = ld_nt(st + f);
   v.x = (float)(v.x * sc); v.y = (float)(v.y * sc);
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(kThreads) void k_scale(float4* __restrict__ st, uin
 
 __global__ __launch_bounds__(kThreads) void k_probs(const float4* __restrict__ st, uint64_t n4,
                                                     float2* __restrict__ probs) {
-  const uint64_t f = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  const uint64_t f = gblock<uint64_t>() * kThreads + threadIdx.x;
   if (f >= n4) return;
   const float4 v = ld_nt(st + f);
   probs[f] = make_float2((float)p2(lo2(v)), (float)p2(hi2(v)));
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(kThreads) void k_leaf_totals(const float4* __restri
                                                           uint64_t n_amps, uint64_t n_leaves,
                                                           double* __restrict__ tot) {
   const int lane = threadIdx.x & 63;
-  const uint64_t leaf = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint64_t leaf = gblock<uint64_t>() * 4 + (threadIdx.x >> 6);
   if (leaf >= n_leaves) return;
   const int nl = (int)(n_amps < (uint64_t)kLeaf ? n_amps : kLeaf);
   const float4* lf = st + leaf * (kLeaf / 2);
@@ -529,7 +529,7 @@ int qsim_measure_probs(void* stream, const void* state, int n_qubits, void* prob
   if (int rc = check_n(n_qubits)) return rc;
   if (!state || !probs_dev) return fail(QSIM_EINVAL, "NULL pointer argument");
   const uint64_t n4 = (1ull << n_qubits) / 2;
-  hipLaunchKernelGGL(k_probs, dim3(ceil_div(n4, kThreads)), dim3(kThreads), 0, as_stream(stream),
+  hipLaunchKernelGGL(k_probs, grid1d(ceil_div(n4, kThreads)), dim3(kThreads), 0, as_stream(stream),
                      (const float4*)state, n4, (float2*)probs_dev);
   return launch_status("measure_probs");
 }
@@ -544,7 +544,7 @@ int qsim_measure_qubit(void* stream, void* state, int n_qubits, int target, doub
   if (int rc = reduce_impl(s, state, n_qubits, 1, target, ws)) return rc;
   const uint64_t n4 = (1ull << n_qubits) / 2;
   int* dev_out = (int*)(ws + 2 * kRedBlocks + 4);
-  hipLaunchKernelGGL(k_collapse, dim3(ceil_div(n4, kThreads)), dim3(kThreads), 0, s, (float4*)state,
+  hipLaunchKernelGGL(k_collapse, grid1d(ceil_div(n4, kThreads)), dim3(kThreads), 0, s, (float4*)state,
                      n4, target, random_val, ws + 2 * kRedBlocks, dev_out);
   if (int rc = launch_status("measure_qubit collapse")) return rc;
   int host = -1;
@@ -562,7 +562,7 @@ int qsim_renormalize(void* stream, void* state, int n_qubits, void* workspace) {
   double* ws = (double*)workspace;
   if (int rc = reduce_impl(s, state, n_qubits, 0, 0, ws)) return rc;
   const uint64_t n4 = (1ull << n_qubits) / 2;
-  hipLaunchKernelGGL(k_scale, dim3(ceil_div(n4, kThreads)), dim3(kThreads), 0, s, (float4*)state,
+  hipLaunchKernelGGL(k_scale, grid1d(ceil_div(n4, kThreads)), dim3(kThreads), 0, s, (float4*)state,
                      n4, ws + 2 * kRedBlocks);
   return launch_status("renormalize");
 }
@@ -587,7 +587,7 @@ int qsim_sample(void* stream, const void* state, int n_qubits, const void* unifo
   double* pre = tot + p.nl;
   double* seg = pre + p.nl;
   double* total = seg + p.ns;
-  hipLaunchKernelGGL(k_leaf_totals, dim3(ceil_div(p.nl, 4)), dim3(kThreads), 0, s,
+  hipLaunchKernelGGL(k_leaf_totals, grid1d(ceil_div(p.nl, 4)), dim3(kThreads), 0, s,
                      (const float4*)state, p.n_amps, p.nl, tot);
   if (int rc = launch_status("sample: leaf totals")) return rc;
   if (p.ns == 1) {

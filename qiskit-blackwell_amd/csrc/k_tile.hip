@@ -287,7 +287,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_program(float4* __restric
                                                               int n_ops,
                                                               const float2* __restrict__ cf) {
   extern __shared__ float2 sh[];
-  const uint64_t base = tile_base(blockIdx.x, g);
+  if (gblock<uint64_t>() >> g.nc) return;    // whole workgroup: past the last tile
+  const uint64_t base = tile_base(gblock<uint64_t>(), g);
   tile_load(sh, st, base, g);
   __syncthreads();
   run_ops(sh, g.m, n_ops, [&](int k) { return ops[k]; }, cf);
@@ -304,7 +305,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_fused1q(float4* __restric
                                                               Fused1Q f,
                                                               const float2* __restrict__ gates) {
   extern __shared__ float2 sh[];
-  const uint64_t base = tile_base(blockIdx.x, g);
+  if (gblock<uint64_t>() >> g.nc) return;    // whole workgroup: past the last tile
+  const uint64_t base = tile_base(gblock<uint64_t>(), g);
   tile_load(sh, st, base, g);
   __syncthreads();
   run_ops(sh, g.m, f.count, [&](int k) {
@@ -392,7 +394,7 @@ __global__ __launch_bounds__(256) void k_reg_fused(float4* __restrict__ st, uint
                                                    const int32_t* __restrict__ targets,
                                                    int n_gates, int max_t) {
   const int lane = threadIdx.x & 63;
-  const uint64_t wave = ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  const uint64_t wave = (gblock<uint64_t>() * 256 + threadIdx.x) >> 6;
   const uint64_t sa = wave * 128 + lane, sb = sa + 64;
   float4 A = sa < n_slots ? st[sa] : make_float4(0.f, 0.f, 0.f, 0.f);
   float4 B = sb < n_slots ? st[sb] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -419,7 +421,7 @@ static int reg_fused_impl(hipStream_t s, void* states, uint64_t n_states, int n,
   const uint64_t n_slots = n_states * ((1ull << n) / 2);
   const uint64_t waves = ceil_div(n_slots, 128);
   const int max_t = std::min(n, 8) - 1;
-  hipLaunchKernelGGL(k_reg_fused, dim3(ceil_div(waves, 4)), dim3(256), 0, s, (float4*)states, n_slots,
+  hipLaunchKernelGGL(k_reg_fused, grid1d(ceil_div(waves, 4)), dim3(256), 0, s, (float4*)states, n_slots,
                      (const float2*)gates, (const int32_t*)targets, n_gates, max_t);
   return launch_status("apply_gates_reg_tiled");
 }
@@ -597,7 +599,7 @@ int qsim_tile_program(void* stream, void* state, int n_qubits, uint64_t tile_mas
   if (int rc = make_geom(g, n_qubits, tile_mask)) return rc;
   const uint64_t tiles = 1ull << g.nc;
   const size_t lds = sizeof(float2) << g.m;
-  hipLaunchKernelGGL(k_tile_program, dim3(tiles), dim3(kTileThreads), lds, as_stream(stream),
+  hipLaunchKernelGGL(k_tile_program, grid1d(tiles), dim3(kTileThreads), lds, as_stream(stream),
                      (float4*)state, g, (const qsim_op*)ops_dev, n_ops, (const float2*)coefs_dev);
   return launch_status("tile_program");
 }
@@ -639,7 +641,7 @@ int qsim_apply_gates_fused(void* stream, void* state, int n_qubits, const void* 
       f.q[j] = (uint8_t)__builtin_popcountll(mask & ((1ull << t) - 1));  // local position
     }
     const size_t lds = sizeof(float2) << g.m;
-    hipLaunchKernelGGL(k_tile_fused1q, dim3(1ull << g.nc), dim3(kTileThreads), lds, s,
+    hipLaunchKernelGGL(k_tile_fused1q, grid1d(1ull << g.nc), dim3(kTileThreads), lds, s,
                        (float4*)state, g, f, (const float2*)gates_dev);
     if (int rc = launch_status("apply_gates_fused")) return rc;
     k += cnt;

@@ -48,6 +48,15 @@ int gate1q_impl(void* stream, void* states, uint64_t n_states, int n, const floa
 
 inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
+// Grid of `blocks` workgroups: one row while it fits 2^23 blocks (2^31
+// work-items of 256), else rows of 2^23 blocks (kernels index blocks with
+// gblock() and guard the tail of the last row).
+inline dim3 grid1d(uint64_t blocks) {
+  constexpr uint64_t kRow = 1ull << 23;
+  if (blocks <= kRow) return dim3((unsigned)(blocks ? blocks : 1));
+  return dim3((unsigned)kRow, (unsigned)ceil_div(blocks, kRow));
+}
+
 // Sorted affected-qubit list plus control mask for the multi-controlled kernels.
 struct McSpec {
   int      pos[10];   // ascending positions of all affected qubits
@@ -155,13 +164,21 @@ __device__ __forceinline__ float4 shfl_xor4(float4 v, int m) {
   return r;
 }
 
+// Flat block index of a 1D grid laid out as grid1d() does (gridDim.y rows of
+// gridDim.x blocks): an AMD dispatch holds at most 2^32 - 1 work-items per
+// dimension, so launches over states of more than 2^31 slots use rows.
+template <typename I>
+__device__ __forceinline__ I gblock() {
+  return (I)blockIdx.y * (I)gridDim.x + (I)blockIdx.x;
+}
+
 // XCD-aware block order.  Workgroups are dispatched round-robin over the 8
 // XCDs (block b -> XCD b % 8); renumbering so that XCD x walks the x-th
 // eighth of the grid keeps each XCD's L2 and DRAM pages on one contiguous
 // region instead of 4 KiB stripes shared by all eight.
 template <typename I>
 __device__ __forceinline__ I xcd_block() {
-  const I nb = gridDim.x, b = blockIdx.x;
+  const I nb = (I)gridDim.x * (I)gridDim.y, b = gblock<I>();
   if (nb & 7) return b;
   return (b & 7) * (nb >> 3) + (b >> 3);
 }

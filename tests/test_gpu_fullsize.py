@@ -152,3 +152,40 @@ def test_pauli_conjugation_30q():
     scale = float(1 << n)   # |<P>| <= ||psi||^2 ~ 2^n for this unnormalised state
     assert same
     assert np.max(np.abs(e1 - sign * e0)) <= 1e-12 * scale, (e0, e1, sign)
+
+
+def test_33_qubit_state_64bit_paths():
+    """A 33-qubit state (64 GiB, beyond 32-bit amplitude indices) through the
+    64-bit index variants: init, 1Q on the top qubit, controlled gates across
+    bit 32, a dense 2Q gate and its inverse, diagonal and phase gates, the
+    FP64 norm, a Z expectation and the sampler."""
+    import blackwell_kernels as bk
+    n = 33
+    print("[fullsize] 33-qubit state, 64-bit index paths", flush=True)
+    dev = "cuda:0"
+    st = bk.state_init(n, device=dev)
+    assert abs(bk.norm2(st) - 1) < 1e-12 and complex(st[0].item()) == 1
+    h = bk.hadamard_gate(dev)
+    u4 = torch.from_numpy(np.linalg.qr(np.random.default_rng(5).standard_normal((4, 4))
+                                       + 1j * np.random.default_rng(6).standard_normal((4, 4)))[0]
+                          .astype(np.complex64)).to(dev)
+    bk.apply_gate(st, h, 32)                                   # (|0> + |2^32>) / sqrt 2
+    assert abs(bk.expectation_z(st, 1 << 32)) < 1e-6
+    u = bk.uniforms_splitmix(9, 1 << 16, dev)
+    smp = bk.sample_with_uniforms(st, u)
+    vals, cnt = torch.unique(smp, return_counts=True)
+    assert set(vals.tolist()) == {0, 1 << 32} and abs(cnt[0].item() / (1 << 16) - 0.5) < 0.01
+    bk.apply_mcx(st, [32], 0)                                  # CNOT across bit 32
+    bk.apply_gate_2q(st, u4, 5, 32)
+    bk.apply_gate_2q(st, u4.conj().T.contiguous(), 5, 32)      # and its inverse
+    bk.apply_diagonal(st, bk.rz_gate_diag(0.7, dev), 32)
+    bk.apply_diagonal(st, bk.rz_gate_diag(-0.7, dev), 32)
+    bk.apply_diagonal_phase(st, 1j, 32)
+    bk.apply_diagonal_phase(st, -1j, 32)
+    bk.apply_mcx(st, [32], 0)
+    bk.apply_gate(st, h, 32)
+    p0 = abs(complex(st[0].item())) ** 2
+    nrm = bk.norm2(st)
+    del st
+    torch.cuda.empty_cache()
+    assert p0 >= 1 - 1e-6 and abs(nrm - 1) <= 1e-6, (p0, nrm)
