@@ -158,7 +158,7 @@ def test_33_qubit_state_64bit_paths():
     """A 33-qubit state (64 GiB, beyond 32-bit amplitude indices) through the
     64-bit index variants: init, 1Q on the top qubit, controlled gates across
     bit 32, a dense 2Q gate and its inverse, diagonal and phase gates, the
-    FP64 norm, a Z expectation and the sampler."""
+    FP64 norm, a Z expectation, the sampler and a 5-qubit MFMA block."""
     import blackwell_kernels as bk
     n = 33
     print("[fullsize] 33-qubit state, 64-bit index paths", flush=True)
@@ -182,6 +182,11 @@ def test_33_qubit_state_64bit_paths():
     bk.apply_diagonal(st, bk.rz_gate_diag(-0.7, dev), 32)
     bk.apply_diagonal_phase(st, 1j, 32)
     bk.apply_diagonal_phase(st, -1j, 32)
+    from helpers import rand_unitary
+    ub = rand_unitary(32, 33).astype(np.complex64)
+    qs = [0, 7, 15, 24, 32]                                    # MFMA block (LDS-staged above 32 qubits)
+    bk.apply_block(st, torch.from_numpy(ub).to(dev), qs)
+    bk.apply_block(st, torch.from_numpy(ub.conj().T.copy()).to(dev), qs)
     bk.apply_mcx(st, [32], 0)
     bk.apply_gate(st, h, 32)
     p0 = abs(complex(st[0].item())) ** 2
