@@ -575,9 +575,10 @@ __global__ __launch_bounds__(kThreads) void k_mc4(float4* __restrict__ st, I n_i
 //     at those bits (and at the target in MODE 2) and then has them set, so
 //     only windows with every high control set are visited;
 //   * controls 1..6 are slot bits inside the window.  On slot bits 3..5 a
-//     control switches whole 128 B lines on or off, so only active lanes load
-//     and store.  On slot bits 0..2 (FULL) it splits every line, and the wave
-//     rewrites whole lines instead (inactive lanes store what they loaded):
+//     control switches whole 128 B lines on or off, and on slot bit 2 whole
+//     64 B halves, so only active lanes load and store.  On slot bits 0..1
+//     (FULL) it splits every line into 16-32 B runs, and the wave rewrites
+//     whole lines instead (inactive lanes store what they loaded): such
 //     partial-line traffic costs more than the untouched half;
 //   * a control on qubit 0 restricts the update to the odd amplitude (.zw)
 //     of each slot.
@@ -690,7 +691,10 @@ static bool mc1_launch(hipStream_t s, void* state, int n, const McSpec& spec, in
   }
   if (n < 8 || n - 7 - a.nh < 0) return false;
   const uint64_t items = 1ull << (n - 7 - a.nh);
-  const bool full = (a.lmask & 7u) != 0;
+  // controls on qubits 1..2 leave active runs of 16-32 B: the wave rewrites
+  // whole lines; a control on qubit 3 leaves 64 B runs, which the active
+  // lanes read and write alone (measured 11 % faster than whole lines)
+  const bool full = (a.lmask & 3u) != 0;
   const bool wide = (1ull << (n - 1)) > 0xffffffffull;
   M2 gv = {};
   if (op == 0) {
