@@ -384,7 +384,13 @@ __global__ __launch_bounds__(kThreads) void k_diag(float4* __restrict__ st, I n4
 
 // Phase on bit t = 1 only: t >= 1 touches slot ins0(w, t-1) | 2^(t-1); t = 0
 // touches the .hi half of every slot (the cache line is read either way).
-template <typename I, int U>
+// Phase on the bit-t = 1 half.  MODE 0: t = 0, every slot (its odd
+// amplitude changes).  MODE 1: t = 1..3, every slot: the slots with bit t set
+// come in runs of 16-64 B, so whole lines are rewritten (the untouched slots
+// store what they loaded; measured faster than touching the runs alone).
+// MODE 2: t >= 4, only the slots with bit t set, in whole 128 B lines.  All
+// accesses are nontemporal.
+template <typename I, int U, int MODE>
 __global__ __launch_bounds__(kThreads) void k_phase(float4* __restrict__ st, I n_items, int t,
                                                     float2 ph) {
   const I base = gblock<I>() * (kThreads * U) + threadIdx.x;
@@ -395,16 +401,17 @@ __global__ __launch_bounds__(kThreads) void k_phase(float4* __restrict__ st, I n
   for (int u = 0; u < U; ++u) {
     I w = base + (I)u * kThreads;
     if (w >= n_items) w = n_items - 1;
-    f[u] = t > 0 ? (ins0(w, s) | (I(1) << s)) : w;
-    v[u] = st[f[u]];
+    f[u] = MODE == 2 ? (ins0(w, s) | (I(1) << s)) : w;
+    v[u] = ld_nt(st + f[u]);
   }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const I w = base + (I)u * kThreads;
     float4 r;
-    if (t > 0) r = cat4(cmulc(lo2(v[u]), ph), cmulc(hi2(v[u]), ph));
-    else r = cat4(lo2(v[u]), cmulc(hi2(v[u]), ph));
-    if (w < n_items) st[f[u]] = r;
+    if (MODE == 0) r = cat4(lo2(v[u]), cmulc(hi2(v[u]), ph));
+    else if (MODE == 1 && !((f[u] >> s) & 1)) r = v[u];
+    else r = cat4(cmulc(lo2(v[u]), ph), cmulc(hi2(v[u]), ph));
+    if (w < n_items) st_nt(st + f[u], r);
   }
 }
 
@@ -445,18 +452,28 @@ static int diag_impl(void* stream, void* state, int n, int kind, int q0, int q1,
   return launch_status("apply_diagonal");
 }
 
+template <typename I, int MODE>
+static void launch_phase(hipStream_t s, void* state, uint64_t items, int t, float2 ph) {
+  hipLaunchKernelGGL((k_phase<I, kUd, MODE>), grid1d(ceil_div(items, (uint64_t)kThreads * kUd)),
+                     dim3(kThreads), 0, s, (float4*)state, (I)items, t, ph);
+}
+
 static int phase_impl(void* stream, void* state, int n, float2 ph, int t) {
   if (int rc = check_n(n)) return rc;
   if (int rc = check_q(t, n, "target_qubit")) return rc;
   if (!state) return fail(QSIM_EINVAL, "state is NULL");
-  const uint64_t items = t > 0 ? (1ull << n) / 4 : (1ull << n) / 2;
-  const uint64_t blocks = ceil_div(items, (uint64_t)kThreads * kUd);
-  if (n <= 32)
-    hipLaunchKernelGGL((k_phase<uint32_t, kUd>), grid1d(blocks), dim3(kThreads), 0,
-                       as_stream(stream), (float4*)state, (uint32_t)items, t, ph);
-  else
-    hipLaunchKernelGGL((k_phase<uint64_t, kUd>), grid1d(blocks), dim3(kThreads), 0,
-                       as_stream(stream), (float4*)state, items, t, ph);
+  const int mode = t == 0 ? 0 : (t < 4 ? 1 : 2);
+  const uint64_t items = mode == 2 ? (1ull << n) / 4 : (1ull << n) / 2;
+  hipStream_t s = as_stream(stream);
+  if (n <= 32) {
+    if (mode == 0) launch_phase<uint32_t, 0>(s, state, items, t, ph);
+    if (mode == 1) launch_phase<uint32_t, 1>(s, state, items, t, ph);
+    if (mode == 2) launch_phase<uint32_t, 2>(s, state, items, t, ph);
+  } else {
+    if (mode == 0) launch_phase<uint64_t, 0>(s, state, items, t, ph);
+    if (mode == 1) launch_phase<uint64_t, 1>(s, state, items, t, ph);
+    if (mode == 2) launch_phase<uint64_t, 2>(s, state, items, t, ph);
+  }
   return launch_status("apply_diagonal_phase");
 }
 
