@@ -230,13 +230,34 @@ __device__ __forceinline__ void mat4(const float2 g[16], float2& a0, float2& a1,
   a0 = b0; a1 = b1; a2 = b2; a3 = b3;
 }
 
-// Row r of g selected without dynamic register indexing.
-__device__ __forceinline__ void pick_row(const float2 g[16], bool b1, bool b0, float2 out[4]) {
+// Lane-permuted rows, selected without dynamic register indexing.  A lane
+// whose amplitudes sit at matrix row r keeps its inputs in lane-relative
+// order x_k (column k ^ r), so it needs h[k] = g[r][k ^ r].  The compensated
+// dot sums the columns as ((c0 + c1) + (c2 + c3)), a tree that an XOR of the
+// column index maps onto itself, so the permuted form rounds identically.
+// perm_row: r = 2*b1 + b0.
+__device__ __forceinline__ void perm_row(const float2 g[16], bool b1, bool b0, float2 h[4]) {
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const float2 r0 = b0 ? g[4 + c] : g[c];
-    const float2 r1 = b0 ? g[12 + c] : g[8 + c];
-    out[c] = b1 ? r1 : r0;
+  for (int k = 0; k < 4; ++k) {
+    const float2 r0 = b0 ? g[4 + (k ^ 1)] : g[k];                 // rows 1 / 0
+    const float2 r1 = b0 ? g[12 + (k ^ 3)] : g[8 + (k ^ 2)];      // rows 3 / 2
+    h[k] = b1 ? r1 : r0;
+  }
+}
+// perm_rows2: rows b0 and 2 + b0 with columns k ^ b0.
+__device__ __forceinline__ void perm_rows2(const float2 g[16], bool b0, float2 ha[4], float2 hb[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    ha[k] = b0 ? g[4 + (k ^ 1)] : g[k];
+    hb[k] = b0 ? g[12 + (k ^ 1)] : g[8 + k];
+  }
+}
+// perm_rows0: rows 2*b1 and 2*b1 + 1 with columns k ^ 2*b1.
+__device__ __forceinline__ void perm_rows0(const float2 g[16], bool b1, float2 ha[4], float2 hb[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    ha[k] = b1 ? g[8 + (k ^ 2)] : g[k];
+    hb[k] = b1 ? g[12 + (k ^ 2)] : g[4 + k];
   }
 }
 
@@ -273,20 +294,21 @@ __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I 
       o0 = cat4(a0, a1);
       o1 = cat4(a2, a3);
     } else {
+      // Inputs in lane-relative order: x_k has matrix column k ^ bl (self,
+      // lo partner, then the same two with bit hi set); the rows are
+      // permuted to match once per lane, so no per-amplitude select remains.
       const int m = 1 << (lo - 1);
       const bool bl = (lane & m) != 0;
       const float4 p0 = shfl_xor4(v0, m), p1 = shfl_xor4(v1, m);
-      float2 ga[4], gb[4];
-      pick_row(g, false, bl, ga);   // rows of the amplitudes held here: r = bl, 2 + bl
-      pick_row(g, true, bl, gb);
+      float2 ha[4], hb[4];
+      perm_rows2(g, bl, ha, hb);
       float2 y[2][2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const float2 s0 = h ? hi2(v0) : lo2(v0), s1 = h ? hi2(v1) : lo2(v1);
-        const float2 q0 = h ? hi2(p0) : lo2(p0), q1 = h ? hi2(p1) : lo2(p1);
-        const float2 a0 = bl ? q0 : s0, a1 = bl ? s0 : q0, a2 = bl ? q1 : s1, a3 = bl ? s1 : q1;
-        y[h][0] = crow4(ga, a0, a1, a2, a3);
-        y[h][1] = crow4(gb, a0, a1, a2, a3);
+        const float2 x0 = h ? hi2(v0) : lo2(v0), x1 = h ? hi2(p0) : lo2(p0);
+        const float2 x2 = h ? hi2(v1) : lo2(v1), x3 = h ? hi2(p1) : lo2(p1);
+        y[h][0] = crow4(ha, x0, x1, x2, x3);
+        y[h][1] = crow4(hb, x0, x1, x2, x3);
       }
       o0 = cat4(y[0][0], y[1][0]);
       o1 = cat4(y[0][1], y[1][1]);
@@ -298,26 +320,22 @@ __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I 
     const bool bh = (lane & mh) != 0;
     float4 o;
     if (lo == 0) {
+      // x = (lo2(v), hi2(v), lo2(p), hi2(p)) have columns k ^ 2*bh
       const float4 p = shfl_xor4(v, mh);
-      const float2 a0 = bh ? lo2(p) : lo2(v), a1 = bh ? hi2(p) : hi2(v);
-      const float2 a2 = bh ? lo2(v) : lo2(p), a3 = bh ? hi2(v) : hi2(p);
-      float2 ga[4], gb[4];
-      pick_row(g, bh, false, ga);
-      pick_row(g, bh, true, gb);
-      o = cat4(crow4(ga, a0, a1, a2, a3), crow4(gb, a0, a1, a2, a3));
+      float2 ha[4], hb[4];
+      perm_rows0(g, bh, ha, hb);
+      o = cat4(crow4(ha, lo2(v), hi2(v), lo2(p), hi2(p)),
+               crow4(hb, lo2(v), hi2(v), lo2(p), hi2(p)));
     } else {
+      // x = (self, lo partner, hi partner, both) have columns k ^ r,
+      // r = 2*bh + bl the row of the amplitudes this lane holds
       const int ml = 1 << (lo - 1);
       const bool bl = (lane & ml) != 0;
       const float4 pl = shfl_xor4(v, ml), ph = shfl_xor4(v, mh), pb = shfl_xor4(v, ml | mh);
-      // the slot with (bit lo, bit hi) = (x, y) lives in the lane with those bits
-#define QSIM_SRC(x, y) ((x) == bl ? ((y) == bh ? v : ph) : ((y) == bh ? pl : pb))
-      const float4 A0 = QSIM_SRC(false, false), A1 = QSIM_SRC(true, false);
-      const float4 A2 = QSIM_SRC(false, true), A3 = QSIM_SRC(true, true);
-#undef QSIM_SRC
-      float2 gr[4];
-      pick_row(g, bh, bl, gr);
-      o = cat4(crow4(gr, lo2(A0), lo2(A1), lo2(A2), lo2(A3)),
-               crow4(gr, hi2(A0), hi2(A1), hi2(A2), hi2(A3)));
+      float2 h[4];
+      perm_row(g, bh, bl, h);
+      o = cat4(crow4(h, lo2(v), lo2(pl), lo2(ph), lo2(pb)),
+               crow4(h, hi2(v), hi2(pl), hi2(ph), hi2(pb)));
     }
     if (live) st_nt(st + w, o);
   }
