@@ -55,7 +55,7 @@ __device__ __forceinline__ double wsum64(double v) {
 // whole wave (computed on the scalar unit).  Per slot and term that leaves
 // t = v_0 k_0 + v_1 k_1 and acc += ws * t, with v_c = |a_c|^2 (KIND 0) or
 // Re / Im of conj(a_c) * partner_c.
-template <int NT, int KIND>
+template <int NT, int KIND, bool NTL>
 __global__ __launch_bounds__(kThreads) void k_expect(const float4* __restrict__ st, uint64_t n4,
                                                      uint64_t n_units, uint64_t chunk, uint64_t x,
                                                      int hw, ExpTerms a, double* __restrict__ part) {
@@ -88,8 +88,8 @@ __global__ __launch_bounds__(kThreads) void k_expect(const float4* __restrict__ 
       wk[k] = KIND == 2 ? ins0(u, hw) : u;
       const uint64_t sl = wk[k] * 64 + lane;
       const bool ok = u < end && sl < n4;
-      v[k] = ok ? ld_nt(st + sl) : zero;
-      if (KIND == 2) p[k] = ok ? ld_nt(st + ((wk[k] ^ xw) * 64 + lane)) : zero;
+      v[k] = ok ? ld4<NTL>(st + sl) : zero;
+      if (KIND == 2) p[k] = ok ? ld4<NTL>(st + ((wk[k] ^ xw) * 64 + lane)) : zero;
     }
 #pragma unroll
     for (int k = 0; k < U; ++k) {
@@ -170,8 +170,13 @@ static void launch_expect(hipStream_t s, const void* state, int n, uint64_t x, c
   const uint64_t want = ceil_div(n_units, 16);   // 4 waves x 4 units per workgroup
   const int blocks = (int)(want < (uint64_t)kRedBlocks ? (want ? want : 1) : kRedBlocks);
   const uint64_t chunk = ceil_div(n_units, blocks);
-  hipLaunchKernelGGL((k_expect<NT, KIND>), dim3(blocks), dim3(kThreads), 0, s,
-                     (const float4*)state, n4, n_units, chunk, x, hw, a, part);
+  // NTL: nontemporal loads above the MALL size
+  if (nt_for_qubits(n))
+    hipLaunchKernelGGL((k_expect<NT, KIND, true>), dim3(blocks), dim3(kThreads), 0, s,
+                       (const float4*)state, n4, n_units, chunk, x, hw, a, part);
+  else
+    hipLaunchKernelGGL((k_expect<NT, KIND, false>), dim3(blocks), dim3(kThreads), 0, s,
+                       (const float4*)state, n4, n_units, chunk, x, hw, a, part);
   hipLaunchKernelGGL((k_expect_final<NT>), dim3(1), dim3(kThreads), 0, s, part, blocks, a, out);
 }
 
@@ -262,7 +267,7 @@ __device__ __forceinline__ float4 pauli_slot(float4 p, uint64_t s, uint64_t x, u
   return cat4(r[0], r[1]);
 }
 
-template <int KIND>
+template <int KIND, bool NT>
 __global__ __launch_bounds__(kThreads) void k_pauli_apply(float4* __restrict__ st, uint64_t n4,
                                                           uint64_t n_units, uint64_t x, uint64_t z,
                                                           int y, int hw) {
@@ -273,25 +278,45 @@ __global__ __launch_bounds__(kThreads) void k_pauli_apply(float4* __restrict__ s
     const uint64_t w0 = ins0(u, hw);
     const uint64_t w = (__popcll(w0 & (z >> 7)) & 1) ? w0 : (w0 | (1ull << hw));
     const uint64_t s = w * 64 + lane;
-    const float4 v = ld_nt(st + s);
-    st_nt(st + s, make_float4(-v.x, -v.y, -v.z, -v.w));
+    const float4 v = ld4<NT>(st + s);
+    st4<NT>(st + s, make_float4(-v.x, -v.y, -v.z, -v.w));
     return;
   }
   const int xl = (int)((x >> 1) & 63), csw = (int)(x & 1);
   if (KIND == 2) {
     const uint64_t w0 = ins0(u, hw), w1 = w0 ^ (x >> 7);
     const uint64_t s0 = w0 * 64 + lane, s1 = w1 * 64 + lane;
-    const float4 v0 = ld_nt(st + s0), v1 = ld_nt(st + s1);
+    const float4 v0 = ld4<NT>(st + s0), v1 = ld4<NT>(st + s1);
     const float4 p0 = shfl_xor4(v1, xl), p1 = shfl_xor4(v0, xl);
-    st_nt(st + s0, pauli_slot(p0, s0, x, z, y, csw));
-    st_nt(st + s1, pauli_slot(p1, s1, x, z, y, csw));
+    st4<NT>(st + s0, pauli_slot(p0, s0, x, z, y, csw));
+    st4<NT>(st + s1, pauli_slot(p1, s1, x, z, y, csw));
     return;
   }
   const uint64_t s = u * 64 + lane;
   const bool ok = s < n4;
-  const float4 v = ok ? ld_nt(st + s) : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 v = ok ? ld4<NT>(st + s) : make_float4(0.f, 0.f, 0.f, 0.f);
   const float4 p = KIND == 1 ? shfl_xor4(v, xl) : v;
-  if (ok) st_nt(st + s, pauli_slot(p, s, x, z, y, csw));
+  if (ok) st4<NT>(st + s, pauli_slot(p, s, x, z, y, csw));
+}
+
+template <bool NT>
+static void launch_pauli(hipStream_t s, void* state, uint64_t n4, uint64_t windows, uint64_t x,
+                         uint64_t z, int y) {
+  if (x == 0 && (z & 127) == 0) {
+    const int hw = (63 - __builtin_clzll(z >> 7));
+    hipLaunchKernelGGL((k_pauli_apply<3, NT>), grid1d(ceil_div(windows / 2, 4)), dim3(kThreads), 0, s,
+                       (float4*)state, n4, windows / 2, x, z, y, hw);
+  } else if (x == 0) {
+    hipLaunchKernelGGL((k_pauli_apply<0, NT>), grid1d(ceil_div(windows, 4)), dim3(kThreads), 0, s,
+                       (float4*)state, n4, windows, x, z, y, 0);
+  } else if (x < 128) {
+    hipLaunchKernelGGL((k_pauli_apply<1, NT>), grid1d(ceil_div(windows, 4)), dim3(kThreads), 0, s,
+                       (float4*)state, n4, windows, x, z, y, 0);
+  } else {
+    const int hw = (63 - __builtin_clzll(x)) - 7;
+    hipLaunchKernelGGL((k_pauli_apply<2, NT>), grid1d(ceil_div(windows / 2, 4)), dim3(kThreads), 0, s,
+                       (float4*)state, n4, windows / 2, x, z, y, hw);
+  }
 }
 
 static int pauli_apply_impl(hipStream_t s, void* state, int n, uint64_t x, uint64_t z) {
@@ -303,21 +328,8 @@ static int pauli_apply_impl(hipStream_t s, void* state, int n, uint64_t x, uint6
   const uint64_t n4 = (1ull << n) / 2;
   const uint64_t windows = ceil_div(n4, 64);
   const int y = __builtin_popcountll(x & z) & 3;
-  if (x == 0 && (z & 127) == 0) {
-    const int hw = (63 - __builtin_clzll(z >> 7));
-    hipLaunchKernelGGL(k_pauli_apply<3>, grid1d(ceil_div(windows / 2, 4)), dim3(kThreads), 0, s,
-                       (float4*)state, n4, windows / 2, x, z, y, hw);
-  } else if (x == 0) {
-    hipLaunchKernelGGL(k_pauli_apply<0>, grid1d(ceil_div(windows, 4)), dim3(kThreads), 0, s,
-                       (float4*)state, n4, windows, x, z, y, 0);
-  } else if (x < 128) {
-    hipLaunchKernelGGL(k_pauli_apply<1>, grid1d(ceil_div(windows, 4)), dim3(kThreads), 0, s,
-                       (float4*)state, n4, windows, x, z, y, 0);
-  } else {
-    const int hw = (63 - __builtin_clzll(x)) - 7;
-    hipLaunchKernelGGL(k_pauli_apply<2>, grid1d(ceil_div(windows / 2, 4)), dim3(kThreads), 0, s,
-                       (float4*)state, n4, windows / 2, x, z, y, hw);
-  }
+  if (nt_for_qubits(n)) launch_pauli<true>(s, state, n4, windows, x, z, y);
+  else launch_pauli<false>(s, state, n4, windows, x, z, y);
   return launch_status("apply_pauli_string");
 }
 

@@ -85,12 +85,12 @@ int build_mc(McSpec& s, int n, const int64_t* controls, int n_controls, const in
 }
 
 // ============================================================ state init
-template <typename I>
+template <typename I, bool NT>
 __global__ __launch_bounds__(kThreads) void k_init(float4* __restrict__ st, I n4, I per_state4) {
   const I f = gblock<I>() * kThreads + threadIdx.x;
   if (f >= n4) return;
   const bool first = (f & (per_state4 - 1)) == 0;
-  st_nt(st + f, make_float4(first ? 1.f : 0.f, 0.f, 0.f, 0.f));
+  st4<NT>(st + f, make_float4(first ? 1.f : 0.f, 0.f, 0.f, 0.f));
 }
 
 static int init_impl(void* stream, void* states, uint64_t n_states, int n) {
@@ -100,12 +100,15 @@ static int init_impl(void* stream, void* states, uint64_t n_states, int n) {
   const uint64_t per4 = (1ull << n) / 2;
   const uint64_t n4 = per4 * n_states;
   const uint64_t blocks = ceil_div(n4, kThreads);
-  if (n4 <= 0xffffffffull)
-    hipLaunchKernelGGL(k_init<uint32_t>, grid1d(blocks), dim3(kThreads), 0, as_stream(stream),
+  if (n4 > 0xffffffffull)
+    hipLaunchKernelGGL((k_init<uint64_t, true>), grid1d(blocks), dim3(kThreads), 0, as_stream(stream),
+                       (float4*)states, n4, per4);
+  else if (nt_for_bytes(n4 * 16))
+    hipLaunchKernelGGL((k_init<uint32_t, true>), grid1d(blocks), dim3(kThreads), 0, as_stream(stream),
                        (float4*)states, (uint32_t)n4, (uint32_t)per4);
   else
-    hipLaunchKernelGGL(k_init<uint64_t>, grid1d(blocks), dim3(kThreads), 0, as_stream(stream),
-                       (float4*)states, n4, per4);
+    hipLaunchKernelGGL((k_init<uint32_t, false>), grid1d(blocks), dim3(kThreads), 0, as_stream(stream),
+                       (float4*)states, (uint32_t)n4, (uint32_t)per4);
   return launch_status("state_init");
 }
 
@@ -126,7 +129,7 @@ __device__ __forceinline__ void load_m2(const float2* gdev, const M2& gv, float2
 // n_work = float4 slots (modes 0, 1) or slot pairs (mode 2).  Batched states
 // need no extra index math: the flat slot index has the state number above
 // bit n-2 and t < n keeps every pair inside one state.
-template <typename I, int MODE>
+template <typename I, int MODE, bool NT>
 __global__ __launch_bounds__(kThreads) void k_gate1q(float4* __restrict__ st, I n_work, int t,
                                                      const float2* __restrict__ gdev, M2 gv) {
   float2 g[4];
@@ -135,27 +138,27 @@ __global__ __launch_bounds__(kThreads) void k_gate1q(float4* __restrict__ st, I 
   const bool live = w0 < n_work;
   const I w = live ? w0 : n_work - 1;
   if (MODE == 0) {
-    const float4 v = ld_nt(st + w);
+    const float4 v = ld4<NT>(st + w);
     const float4 r = cat4(crow2(g[0], lo2(v), g[1], hi2(v)), crow2(g[2], lo2(v), g[3], hi2(v)));
-    if (live) st_nt(st + w, r);
+    if (live) st4<NT>(st + w, r);
   } else if (MODE == 1) {
     // lane l holds slot w with w & 63 == l; bit (t-1) of the slot index is
     // bit t of the amplitude index.  Partner lanes of live lanes are live
     // (n_work is a power of two larger than 2^(t-1)).
-    const float4 v = ld_nt(st + w);
+    const float4 v = ld4<NT>(st + w);
     const int m = 1 << (t - 1);
     const float4 p = shfl_xor4(v, m);
     const bool low = (threadIdx.x & m) == 0;
     const float2 ga = low ? g[0] : g[3], gb = low ? g[1] : g[2];
     const float4 r = cat4(crow2(ga, lo2(v), gb, lo2(p)), crow2(ga, hi2(v), gb, hi2(p)));
-    if (live) st_nt(st + w, r);
+    if (live) st4<NT>(st + w, r);
   } else {
     const int s = t - 1;
     const I f0 = ins0(w, s), f1 = f0 + (I(1) << s);
-    const float4 v0 = ld_nt(st + f0), v1 = ld_nt(st + f1);
+    const float4 v0 = ld4<NT>(st + f0), v1 = ld4<NT>(st + f1);
     const float4 r0 = cat4(crow2(g[0], lo2(v0), g[1], lo2(v1)), crow2(g[0], hi2(v0), g[1], hi2(v1)));
     const float4 r1 = cat4(crow2(g[2], lo2(v0), g[3], lo2(v1)), crow2(g[2], hi2(v0), g[3], hi2(v1)));
-    if (live) { st_nt(st + f0, r0); st_nt(st + f1, r1); }
+    if (live) { st4<NT>(st + f0, r0); st4<NT>(st + f1, r1); }
   }
 }
 
@@ -192,15 +195,20 @@ int gate1q_impl(void* stream, void* states, uint64_t n_states, int n, const floa
   const int mode = t == 0 ? 0 : (t < 7 ? 1 : 2);
   const uint64_t work = mode == 2 ? slots / 2 : slots;
   const dim3 grid = grid1d(ceil_div(work, kThreads)), block(kThreads);
-  if (slots <= 0xffffffffull) {
+  if (slots > 0xffffffffull) {
+    if (mode == 0) hipLaunchKernelGGL((k_gate1q<uint64_t, 0, true>), grid, block, 0, s, (float4*)states, work, t, gdev, gv);
+    if (mode == 1) hipLaunchKernelGGL((k_gate1q<uint64_t, 1, true>), grid, block, 0, s, (float4*)states, work, t, gdev, gv);
+    if (mode == 2) hipLaunchKernelGGL((k_gate1q<uint64_t, 2, true>), grid, block, 0, s, (float4*)states, work, t, gdev, gv);
+  } else if (nt_for_bytes(slots * 16)) {
     const uint32_t w = (uint32_t)work;
-    if (mode == 0) hipLaunchKernelGGL((k_gate1q<uint32_t, 0>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
-    if (mode == 1) hipLaunchKernelGGL((k_gate1q<uint32_t, 1>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
-    if (mode == 2) hipLaunchKernelGGL((k_gate1q<uint32_t, 2>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
+    if (mode == 0) hipLaunchKernelGGL((k_gate1q<uint32_t, 0, true>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
+    if (mode == 1) hipLaunchKernelGGL((k_gate1q<uint32_t, 1, true>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
+    if (mode == 2) hipLaunchKernelGGL((k_gate1q<uint32_t, 2, true>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
   } else {
-    if (mode == 0) hipLaunchKernelGGL((k_gate1q<uint64_t, 0>), grid, block, 0, s, (float4*)states, work, t, gdev, gv);
-    if (mode == 1) hipLaunchKernelGGL((k_gate1q<uint64_t, 1>), grid, block, 0, s, (float4*)states, work, t, gdev, gv);
-    if (mode == 2) hipLaunchKernelGGL((k_gate1q<uint64_t, 2>), grid, block, 0, s, (float4*)states, work, t, gdev, gv);
+    const uint32_t w = (uint32_t)work;
+    if (mode == 0) hipLaunchKernelGGL((k_gate1q<uint32_t, 0, false>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
+    if (mode == 1) hipLaunchKernelGGL((k_gate1q<uint32_t, 1, false>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
+    if (mode == 2) hipLaunchKernelGGL((k_gate1q<uint32_t, 2, false>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
   }
   return launch_status("apply_gate");
 }
@@ -261,7 +269,7 @@ __device__ __forceinline__ void perm_rows0(const float2 g[16], bool b1, float2 h
   }
 }
 
-template <typename I, int S>
+template <typename I, int S, bool NT>
 __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I n_work, int lo,
                                                      int hi, const float2* __restrict__ gdev,
                                                      M4 gv) {
@@ -274,19 +282,19 @@ __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I 
   if (S == 2) {
     const I ml = I(1) << (lo - 1), mh = I(1) << (hi - 1);
     const I f0 = ins0(ins0(w, lo - 1), hi - 1), f1 = f0 + ml, f2 = f0 + mh, f3 = f1 + mh;
-    const float4 v0 = ld_nt(st + f0), v1 = ld_nt(st + f1), v2 = ld_nt(st + f2), v3 = ld_nt(st + f3);
+    const float4 v0 = ld4<NT>(st + f0), v1 = ld4<NT>(st + f1), v2 = ld4<NT>(st + f2), v3 = ld4<NT>(st + f3);
     float2 a0 = lo2(v0), a1 = lo2(v1), a2 = lo2(v2), a3 = lo2(v3);
     float2 b0 = hi2(v0), b1 = hi2(v1), b2 = hi2(v2), b3 = hi2(v3);
     mat4(g, a0, a1, a2, a3);
     mat4(g, b0, b1, b2, b3);
     if (live) {
-      st_nt(st + f0, cat4(a0, b0)); st_nt(st + f1, cat4(a1, b1));
-      st_nt(st + f2, cat4(a2, b2)); st_nt(st + f3, cat4(a3, b3));
+      st4<NT>(st + f0, cat4(a0, b0)); st4<NT>(st + f1, cat4(a1, b1));
+      st4<NT>(st + f2, cat4(a2, b2)); st4<NT>(st + f3, cat4(a3, b3));
     }
   } else if (S == 1) {
     // hi is a stream: slot f0 has bit(hi) = 0, slot f1 bit(hi) = 1
     const I f0 = ins0(w, hi - 1), f1 = f0 + (I(1) << (hi - 1));
-    const float4 v0 = ld_nt(st + f0), v1 = ld_nt(st + f1);
+    const float4 v0 = ld4<NT>(st + f0), v1 = ld4<NT>(st + f1);
     float4 o0, o1;
     if (lo == 0) {
       float2 a0 = lo2(v0), a1 = hi2(v0), a2 = lo2(v1), a3 = hi2(v1);
@@ -313,9 +321,9 @@ __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I 
       o0 = cat4(y[0][0], y[1][0]);
       o1 = cat4(y[0][1], y[1][1]);
     }
-    if (live) { st_nt(st + f0, o0); st_nt(st + f1, o1); }
+    if (live) { st4<NT>(st + f0, o0); st4<NT>(st + f1, o1); }
   } else {
-    const float4 v = ld_nt(st + w);
+    const float4 v = ld4<NT>(st + w);
     const int mh = 1 << (hi - 1);
     const bool bh = (lane & mh) != 0;
     float4 o;
@@ -337,7 +345,7 @@ __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I 
       o = cat4(crow4(h, lo2(v), lo2(pl), lo2(ph), lo2(pb)),
                crow4(h, hi2(v), hi2(pl), hi2(ph), hi2(pb)));
     }
-    if (live) st_nt(st + w, o);
+    if (live) st4<NT>(st + w, o);
   }
 }
 
@@ -353,22 +361,27 @@ int gate2q_impl(void* stream, void* state, int n, const float2* gdev, const M4& 
   const int S = (lo >= 7) + (hi >= 7);
   const uint64_t work = ((1ull << n) / 2) >> S;
   const dim3 grid = grid1d(ceil_div(work, kThreads)), block(kThreads);
-  if (n <= 32) {
+  if (n > 32) {
+    if (S == 0) hipLaunchKernelGGL((k_gate2q<uint64_t, 0, true>), grid, block, 0, s, (float4*)state, work, lo, hi, gdev, gv);
+    if (S == 1) hipLaunchKernelGGL((k_gate2q<uint64_t, 1, true>), grid, block, 0, s, (float4*)state, work, lo, hi, gdev, gv);
+    if (S == 2) hipLaunchKernelGGL((k_gate2q<uint64_t, 2, true>), grid, block, 0, s, (float4*)state, work, lo, hi, gdev, gv);
+  } else if (nt_for_qubits(n)) {
     const uint32_t w = (uint32_t)work;
-    if (S == 0) hipLaunchKernelGGL((k_gate2q<uint32_t, 0>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
-    if (S == 1) hipLaunchKernelGGL((k_gate2q<uint32_t, 1>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
-    if (S == 2) hipLaunchKernelGGL((k_gate2q<uint32_t, 2>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
+    if (S == 0) hipLaunchKernelGGL((k_gate2q<uint32_t, 0, true>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
+    if (S == 1) hipLaunchKernelGGL((k_gate2q<uint32_t, 1, true>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
+    if (S == 2) hipLaunchKernelGGL((k_gate2q<uint32_t, 2, true>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
   } else {
-    if (S == 0) hipLaunchKernelGGL((k_gate2q<uint64_t, 0>), grid, block, 0, s, (float4*)state, work, lo, hi, gdev, gv);
-    if (S == 1) hipLaunchKernelGGL((k_gate2q<uint64_t, 1>), grid, block, 0, s, (float4*)state, work, lo, hi, gdev, gv);
-    if (S == 2) hipLaunchKernelGGL((k_gate2q<uint64_t, 2>), grid, block, 0, s, (float4*)state, work, lo, hi, gdev, gv);
+    const uint32_t w = (uint32_t)work;
+    if (S == 0) hipLaunchKernelGGL((k_gate2q<uint32_t, 0, false>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
+    if (S == 1) hipLaunchKernelGGL((k_gate2q<uint32_t, 1, false>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
+    if (S == 2) hipLaunchKernelGGL((k_gate2q<uint32_t, 2, false>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
   }
   return launch_status("apply_gate_2q");
 }
 
 // ============================================================ diagonals
 // kind 0: diag(d0, d1) on bit t;  kind 1: diag(d00, d01, d10, d11) on (lo, hi).
-template <typename I, int U, int KIND>
+template <typename I, int U, int KIND, bool NT>
 __global__ __launch_bounds__(kThreads) void k_diag(float4* __restrict__ st, I n4, int t, int hi,
                                                    const float2* __restrict__ ddev, M2 dv) {
   constexpr int kind = KIND;
@@ -384,7 +397,7 @@ __global__ __launch_bounds__(kThreads) void k_diag(float4* __restrict__ st, I n4
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     I f = base + (I)u * kThreads;
-    v[u] = ld_nt(st + (f < n4 ? f : n4 - 1));
+    v[u] = ld4<NT>(st + (f < n4 ? f : n4 - 1));
   }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
@@ -396,7 +409,7 @@ __global__ __launch_bounds__(kThreads) void k_diag(float4* __restrict__ st, I n4
     const float2 e0 = h0 ? (b0 ? d3 : d2) : (b0 ? d1 : d0);
     const float2 e1 = h1 ? (b1 ? d3 : d2) : (b1 ? d1 : d0);
     const float4 r = cat4(cmulc(lo2(v[u]), e0), cmulc(hi2(v[u]), e1));
-    if (f < n4) st_nt(st + f, r);
+    if (f < n4) st4<NT>(st + f, r);
   }
 }
 
@@ -408,7 +421,7 @@ __global__ __launch_bounds__(kThreads) void k_diag(float4* __restrict__ st, I n4
 // store what they loaded; measured faster than touching the runs alone).
 // MODE 2: t >= 4, only the slots with bit t set, in whole 128 B lines.  All
 // accesses are nontemporal.
-template <typename I, int U, int MODE>
+template <typename I, int U, int MODE, bool NT>
 __global__ __launch_bounds__(kThreads) void k_phase(float4* __restrict__ st, I n_items, int t,
                                                     float2 ph) {
   const I base = gblock<I>() * (kThreads * U) + threadIdx.x;
@@ -420,7 +433,7 @@ __global__ __launch_bounds__(kThreads) void k_phase(float4* __restrict__ st, I n
     I w = base + (I)u * kThreads;
     if (w >= n_items) w = n_items - 1;
     f[u] = MODE == 2 ? (ins0(w, s) | (I(1) << s)) : w;
-    v[u] = ld_nt(st + f[u]);
+    v[u] = ld4<NT>(st + f[u]);
   }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
@@ -429,11 +442,18 @@ __global__ __launch_bounds__(kThreads) void k_phase(float4* __restrict__ st, I n
     if (MODE == 0) r = cat4(lo2(v[u]), cmulc(hi2(v[u]), ph));
     else if (MODE == 1 && !((f[u] >> s) & 1)) r = v[u];
     else r = cat4(cmulc(lo2(v[u]), ph), cmulc(hi2(v[u]), ph));
-    if (w < n_items) st_nt(st + f[u], r);
+    if (w < n_items) st4<NT>(st + f[u], r);
   }
 }
 
 constexpr int kUd = 1;
+
+template <typename I, int KIND, bool NT>
+static void launch_diag(hipStream_t s, uint64_t blocks, void* state, uint64_t n4, int t, int hi,
+                        const float2* ddev, const M2& dv) {
+  hipLaunchKernelGGL((k_diag<I, kUd, KIND, NT>), grid1d(blocks), dim3(kThreads), 0, s,
+                     (float4*)state, (I)n4, t, hi, ddev, dv);
+}
 
 static int diag_impl(void* stream, void* state, int n, int kind, int q0, int q1,
                      const float2* ddev, const M2& dv) {
@@ -452,28 +472,28 @@ static int diag_impl(void* stream, void* state, int n, int kind, int q0, int q1,
   const uint64_t n4 = (1ull << n) / 2;
   const uint64_t blocks = ceil_div(n4, (uint64_t)kThreads * kUd);
   hipStream_t s = as_stream(stream);
-  if (n <= 32) {
-    if (kind == 0)
-      hipLaunchKernelGGL((k_diag<uint32_t, kUd, 0>), grid1d(blocks), dim3(kThreads), 0, s,
-                         (float4*)state, (uint32_t)n4, t, hi, ddev, dv);
-    else
-      hipLaunchKernelGGL((k_diag<uint32_t, kUd, 1>), grid1d(blocks), dim3(kThreads), 0, s,
-                         (float4*)state, (uint32_t)n4, t, hi, ddev, dv);
+  if (n > 32) {
+    if (kind == 0) launch_diag<uint64_t, 0, true>(s, blocks, state, n4, t, hi, ddev, dv);
+    else launch_diag<uint64_t, 1, true>(s, blocks, state, n4, t, hi, ddev, dv);
+  } else if (nt_for_qubits(n)) {
+    if (kind == 0) launch_diag<uint32_t, 0, true>(s, blocks, state, n4, t, hi, ddev, dv);
+    else launch_diag<uint32_t, 1, true>(s, blocks, state, n4, t, hi, ddev, dv);
   } else {
-    if (kind == 0)
-      hipLaunchKernelGGL((k_diag<uint64_t, kUd, 0>), grid1d(blocks), dim3(kThreads), 0, s,
-                         (float4*)state, n4, t, hi, ddev, dv);
-    else
-      hipLaunchKernelGGL((k_diag<uint64_t, kUd, 1>), grid1d(blocks), dim3(kThreads), 0, s,
-                         (float4*)state, n4, t, hi, ddev, dv);
+    if (kind == 0) launch_diag<uint32_t, 0, false>(s, blocks, state, n4, t, hi, ddev, dv);
+    else launch_diag<uint32_t, 1, false>(s, blocks, state, n4, t, hi, ddev, dv);
   }
   return launch_status("apply_diagonal");
 }
 
 template <typename I, int MODE>
-static void launch_phase(hipStream_t s, void* state, uint64_t items, int t, float2 ph) {
-  hipLaunchKernelGGL((k_phase<I, kUd, MODE>), grid1d(ceil_div(items, (uint64_t)kThreads * kUd)),
-                     dim3(kThreads), 0, s, (float4*)state, (I)items, t, ph);
+static void launch_phase(hipStream_t s, void* state, uint64_t items, int t, float2 ph, bool nt) {
+  const dim3 grid = grid1d(ceil_div(items, (uint64_t)kThreads * kUd));
+  if (nt)
+    hipLaunchKernelGGL((k_phase<I, kUd, MODE, true>), grid, dim3(kThreads), 0, s, (float4*)state,
+                       (I)items, t, ph);
+  else
+    hipLaunchKernelGGL((k_phase<I, kUd, MODE, false>), grid, dim3(kThreads), 0, s, (float4*)state,
+                       (I)items, t, ph);
 }
 
 static int phase_impl(void* stream, void* state, int n, float2 ph, int t) {
@@ -483,14 +503,15 @@ static int phase_impl(void* stream, void* state, int n, float2 ph, int t) {
   const int mode = t == 0 ? 0 : (t < 4 ? 1 : 2);
   const uint64_t items = mode == 2 ? (1ull << n) / 4 : (1ull << n) / 2;
   hipStream_t s = as_stream(stream);
+  const bool nt = nt_for_qubits(n);
   if (n <= 32) {
-    if (mode == 0) launch_phase<uint32_t, 0>(s, state, items, t, ph);
-    if (mode == 1) launch_phase<uint32_t, 1>(s, state, items, t, ph);
-    if (mode == 2) launch_phase<uint32_t, 2>(s, state, items, t, ph);
+    if (mode == 0) launch_phase<uint32_t, 0>(s, state, items, t, ph, nt);
+    if (mode == 1) launch_phase<uint32_t, 1>(s, state, items, t, ph, nt);
+    if (mode == 2) launch_phase<uint32_t, 2>(s, state, items, t, ph, nt);
   } else {
-    if (mode == 0) launch_phase<uint64_t, 0>(s, state, items, t, ph);
-    if (mode == 1) launch_phase<uint64_t, 1>(s, state, items, t, ph);
-    if (mode == 2) launch_phase<uint64_t, 2>(s, state, items, t, ph);
+    if (mode == 0) launch_phase<uint64_t, 0>(s, state, items, t, ph, nt);
+    if (mode == 1) launch_phase<uint64_t, 1>(s, state, items, t, ph, nt);
+    if (mode == 2) launch_phase<uint64_t, 2>(s, state, items, t, ph, nt);
   }
   return launch_status("apply_diagonal_phase");
 }
@@ -561,20 +582,10 @@ __global__ __launch_bounds__(kThreads) void k_mc(float2* __restrict__ st, I n_it
 // float4 path: valid when the lowest affected qubit is >= 1, so items 2w and
 // 2w+1 are adjacent amplitudes and share one 16-byte slot.  The host passes
 // positions, targets and control mask already shifted to float4 units.
-// NT: every access run is a whole 128 B line (lowest affected qubit >= 4),
-// so the accesses bypass the caches; shorter runs keep cached accesses, which
-// merge the partial lines (nontemporal 32 B runs measured 30-40 % slower).
-template <bool NT>
-__device__ __forceinline__ float4 mc_ld(const float4* p) {
-  if constexpr (NT) return ld_nt(p);
-  else return *p;
-}
-template <bool NT>
-__device__ __forceinline__ void mc_st(float4* p, float4 v) {
-  if constexpr (NT) st_nt(p, v);
-  else *p = v;
-}
-
+// NT: nontemporal accesses, used when every access run is a whole 128 B line
+// (lowest affected qubit >= 4) and the state is larger than the MALL;
+// shorter runs keep cached accesses, which merge the partial lines
+// (nontemporal 32 B runs measured 30-40 % slower).
 template <typename I, int U, bool NT>
 __global__ __launch_bounds__(kThreads) void k_mc4(float4* __restrict__ st, I n_items, McArgs a,
                                                   const float2* __restrict__ gdev) {
@@ -595,26 +606,26 @@ __global__ __launch_bounds__(kThreads) void k_mc4(float4* __restrict__ st, I n_i
     if (w >= n_items) return;
     const I f0 = mc_base(w, a);
     if (a.op < 2) {
-      const float4 x0 = mc_ld<NT>(st + (f0)), x1 = mc_ld<NT>(st + (f0 | m0));
+      const float4 x0 = ld4<NT>(st + (f0)), x1 = ld4<NT>(st + (f0 | m0));
       if (a.op == 1) {
-        mc_st<NT>(st + (f0), x1); mc_st<NT>(st + (f0 | m0), x0);
+        st4<NT>(st + (f0), x1); st4<NT>(st + (f0 | m0), x0);
       } else {
-        mc_st<NT>(st + (f0), cat4(crow2(g[0], lo2(x0), g[1], lo2(x1)), crow2(g[0], hi2(x0), g[1], hi2(x1))));
-        mc_st<NT>(st + (f0 | m0), cat4(crow2(g[2], lo2(x0), g[3], lo2(x1)), crow2(g[2], hi2(x0), g[3], hi2(x1))));
+        st4<NT>(st + (f0), cat4(crow2(g[0], lo2(x0), g[1], lo2(x1)), crow2(g[0], hi2(x0), g[1], hi2(x1))));
+        st4<NT>(st + (f0 | m0), cat4(crow2(g[2], lo2(x0), g[3], lo2(x1)), crow2(g[2], hi2(x0), g[3], hi2(x1))));
       }
     } else if (a.op == 3) {
       // SWAP moves only the (1, 0) and (0, 1) amplitudes
-      const float4 x1 = mc_ld<NT>(st + (f0 | m0)), x2 = mc_ld<NT>(st + (f0 | m1));
-      mc_st<NT>(st + (f0 | m0), x2); mc_st<NT>(st + (f0 | m1), x1);
+      const float4 x1 = ld4<NT>(st + (f0 | m0)), x2 = ld4<NT>(st + (f0 | m1));
+      st4<NT>(st + (f0 | m0), x2); st4<NT>(st + (f0 | m1), x1);
     } else {
-      float4 x0 = mc_ld<NT>(st + (f0)), x1 = mc_ld<NT>(st + (f0 | m0)), x2 = mc_ld<NT>(st + (f0 | m1)), x3 = mc_ld<NT>(st + (f0 | m0 | m1));
+      float4 x0 = ld4<NT>(st + (f0)), x1 = ld4<NT>(st + (f0 | m0)), x2 = ld4<NT>(st + (f0 | m1)), x3 = ld4<NT>(st + (f0 | m0 | m1));
       {
         float2 l0 = lo2(x0), l1 = lo2(x1), l2 = lo2(x2), l3 = lo2(x3);
         float2 h0 = hi2(x0), h1 = hi2(x1), h2 = hi2(x2), h3 = hi2(x3);
         mat4(g, l0, l1, l2, l3);
         mat4(g, h0, h1, h2, h3);
-        mc_st<NT>(st + (f0), cat4(l0, h0)); mc_st<NT>(st + (f0 | m0), cat4(l1, h1));
-        mc_st<NT>(st + (f0 | m1), cat4(l2, h2)); mc_st<NT>(st + (f0 | m0 | m1), cat4(l3, h3));
+        st4<NT>(st + (f0), cat4(l0, h0)); st4<NT>(st + (f0 | m0), cat4(l1, h1));
+        st4<NT>(st + (f0 | m1), cat4(l2, h2)); st4<NT>(st + (f0 | m0 | m1), cat4(l3, h3));
       }
     }
   }
@@ -655,7 +666,7 @@ __device__ __forceinline__ I mc1_window(I item, const Mc1Args& a) {
 __device__ __forceinline__ float4 pick4(bool c, float4 a, float4 b) { return c ? a : b; }
 
 // OP 0: dense 2x2 (compensated), OP 1: X.
-template <typename I, int MODE, int OP, int FULL>
+template <typename I, int MODE, int OP, int FULL, bool NT>
 __global__ __launch_bounds__(kThreads) void k_mc1(float4* __restrict__ st, I n_items, Mc1Args a,
                                                   const float2* __restrict__ gdev, M2 gv) {
   float2 g[4];
@@ -668,15 +679,15 @@ __global__ __launch_bounds__(kThreads) void k_mc1(float4* __restrict__ st, I n_i
   const bool act = ((uint32_t)lane & a.lmask) == a.lmask;
   if (!FULL && !act && MODE != 1) return;
   if (MODE == 0) {
-    const float4 v = ld_nt(st + s0);
+    const float4 v = ld4<NT>(st + s0);
     float4 r;
     if (OP == 1) r = cat4(hi2(v), lo2(v));
     else r = cat4(crow2(g[0], lo2(v), g[1], hi2(v)), crow2(g[2], lo2(v), g[3], hi2(v)));
-    st_nt(st + s0, FULL ? pick4(act, r, v) : r);
+    st4<NT>(st + s0, FULL ? pick4(act, r, v) : r);
   } else if (MODE == 1) {
     // the partner lane has the same control bits, so it is active with this one
     const bool ld = FULL || act;
-    const float4 v = ld ? ld_nt(st + s0) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 v = ld ? ld4<NT>(st + s0) : make_float4(0.f, 0.f, 0.f, 0.f);
     const int m = 1 << (a.t - 1);
     const float4 p = shfl_xor4(v, m);
     if (!ld) return;
@@ -689,10 +700,10 @@ __global__ __launch_bounds__(kThreads) void k_mc1(float4* __restrict__ st, I n_i
       r = cat4(crow2(ga, lo2(v), gb, lo2(p)), crow2(ga, hi2(v), gb, hi2(p)));
     }
     if (a.c0) r = cat4(lo2(v), hi2(r));
-    st_nt(st + s0, FULL ? pick4(act, r, v) : r);
+    st4<NT>(st + s0, FULL ? pick4(act, r, v) : r);
   } else {
     const I s1 = s0 + (I(1) << (a.t - 1));
-    const float4 v0 = ld_nt(st + s0), v1 = ld_nt(st + s1);
+    const float4 v0 = ld4<NT>(st + s0), v1 = ld4<NT>(st + s1);
     float4 r0, r1;
     if (OP == 1) {
       r0 = v1; r1 = v0;
@@ -701,27 +712,27 @@ __global__ __launch_bounds__(kThreads) void k_mc1(float4* __restrict__ st, I n_i
       r1 = cat4(crow2(g[2], lo2(v0), g[3], lo2(v1)), crow2(g[2], hi2(v0), g[3], hi2(v1)));
     }
     if (a.c0) { r0 = cat4(lo2(v0), hi2(r0)); r1 = cat4(lo2(v1), hi2(r1)); }
-    st_nt(st + s0, FULL ? pick4(act, r0, v0) : r0);
-    st_nt(st + s1, FULL ? pick4(act, r1, v1) : r1);
+    st4<NT>(st + s0, FULL ? pick4(act, r0, v0) : r0);
+    st4<NT>(st + s1, FULL ? pick4(act, r1, v1) : r1);
   }
 }
 
-template <typename I, int MODE, int OP>
+template <typename I, int MODE, int OP, bool NT>
 static void launch_mc1(hipStream_t s, void* state, uint64_t items, const Mc1Args& a, bool full,
                        const float2* gdev, const M2& gv) {
   const dim3 grid = grid1d(ceil_div(items, kThreads / 64)), block(kThreads);
   if (full)
-    hipLaunchKernelGGL((k_mc1<I, MODE, OP, 1>), grid, block, 0, s, (float4*)state, (I)items, a, gdev, gv);
+    hipLaunchKernelGGL((k_mc1<I, MODE, OP, 1, NT>), grid, block, 0, s, (float4*)state, (I)items, a, gdev, gv);
   else
-    hipLaunchKernelGGL((k_mc1<I, MODE, OP, 0>), grid, block, 0, s, (float4*)state, (I)items, a, gdev, gv);
+    hipLaunchKernelGGL((k_mc1<I, MODE, OP, 0, NT>), grid, block, 0, s, (float4*)state, (I)items, a, gdev, gv);
 }
 
-template <typename I, int OP>
+template <typename I, int OP, bool NT>
 static void launch_mc1_mode(hipStream_t s, void* state, uint64_t items, const Mc1Args& a, int mode,
                             bool full, const float2* gdev, const M2& gv) {
-  if (mode == 0) launch_mc1<I, 0, OP>(s, state, items, a, full, gdev, gv);
-  if (mode == 1) launch_mc1<I, 1, OP>(s, state, items, a, full, gdev, gv);
-  if (mode == 2) launch_mc1<I, 2, OP>(s, state, items, a, full, gdev, gv);
+  if (mode == 0) launch_mc1<I, 0, OP, NT>(s, state, items, a, full, gdev, gv);
+  if (mode == 1) launch_mc1<I, 1, OP, NT>(s, state, items, a, full, gdev, gv);
+  if (mode == 2) launch_mc1<I, 2, OP, NT>(s, state, items, a, full, gdev, gv);
 }
 
 // Returns false when the state is too small for 64-slot windows.
@@ -749,12 +760,15 @@ static bool mc1_launch(hipStream_t s, void* state, int n, const McSpec& spec, in
   const bool full = (a.lmask & 3u) != 0;
   const bool wide = (1ull << (n - 1)) > 0xffffffffull;
   M2 gv = {};
+  const bool nt = nt_for_qubits(n);
   if (op == 0) {
-    if (wide) launch_mc1_mode<uint64_t, 0>(s, state, items, a, mode, full, gdev, gv);
-    else launch_mc1_mode<uint32_t, 0>(s, state, items, a, mode, full, gdev, gv);
+    if (wide) launch_mc1_mode<uint64_t, 0, true>(s, state, items, a, mode, full, gdev, gv);
+    else if (nt) launch_mc1_mode<uint32_t, 0, true>(s, state, items, a, mode, full, gdev, gv);
+    else launch_mc1_mode<uint32_t, 0, false>(s, state, items, a, mode, full, gdev, gv);
   } else {
-    if (wide) launch_mc1_mode<uint64_t, 1>(s, state, items, a, mode, full, gdev, gv);
-    else launch_mc1_mode<uint32_t, 1>(s, state, items, a, mode, full, gdev, gv);
+    if (wide) launch_mc1_mode<uint64_t, 1, true>(s, state, items, a, mode, full, gdev, gv);
+    else if (nt) launch_mc1_mode<uint32_t, 1, true>(s, state, items, a, mode, full, gdev, gv);
+    else launch_mc1_mode<uint32_t, 1, false>(s, state, items, a, mode, full, gdev, gv);
   }
   return true;
 }
@@ -793,7 +807,7 @@ static int mc_impl(void* stream, void* state, int n, const float2* gdev, const i
     a.t0 -= 1;
     a.t1 = op >= 2 ? a.t1 - 1 : 0;
     const uint64_t blocks = ceil_div(items, (uint64_t)kThreads * kUm);
-    const bool nt = a.pos[0] >= 3;
+    const bool nt = a.pos[0] >= 3 && nt_for_qubits(n);
     if (!wide && nt)
       hipLaunchKernelGGL((k_mc4<uint32_t, kUm, true>), grid1d(blocks), dim3(kThreads), 0, s,
                          (float4*)state, (uint32_t)items, a, gdev);

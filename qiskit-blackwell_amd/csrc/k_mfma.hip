@@ -350,6 +350,7 @@ __device__ __forceinline__ void tile_rc(uint32_t j, const Block5Tile& a, uint32_
   }
 }
 
+template <bool NT>
 __global__ __launch_bounds__(256) void k_block5_lds(float2* __restrict__ st, Block5Tile a,
                                                     const float2* __restrict__ U) {
   __shared__ float2 X[32 * kRowStride];
@@ -361,7 +362,7 @@ __global__ __launch_bounds__(256) void k_block5_lds(float2* __restrict__ st, Blo
 #pragma unroll
   for (int k = 0; k < QSIM_MAX_QUBITS; ++k)
     if (k < a.nh) base |= ((b >> k) & 1ull) << a.hp[k];
-  float4* st4 = reinterpret_cast<float4*>(st);
+  float4* sp = reinterpret_cast<float4*>(st);
   // bit 0 of the tile index is qubit 0: its code tells where amplitude 2s+1 goes
   const uint32_t c0 = a.tcode[0];
   // all eight 16-byte loads of a thread are issued before the first LDS write
@@ -372,7 +373,7 @@ __global__ __launch_bounds__(256) void k_block5_lds(float2* __restrict__ st, Blo
     uint32_t r, cc;
     uint64_t off;
     tile_rc(2u * (threadIdx.x + 256 * k), a, r, cc, off);
-    v[k] = ld_nt(st4 + ((base | off) >> 1));
+    v[k] = ld4<NT>(sp + ((base | off) >> 1));
     const uint32_t r1 = c0 < 8 ? (r | (1u << c0)) : r;
     const uint32_t cc1 = c0 < 8 ? cc : (cc | (1u << (c0 - 8)));
     lds0[k] = r * kRowStride + cc;
@@ -420,7 +421,7 @@ __global__ __launch_bounds__(256) void k_block5_lds(float2* __restrict__ st, Blo
     tile_rc(2u * (threadIdx.x + 256 * k), a, r, cc, off);
     const uint32_t r1 = c0 < 8 ? (r | (1u << c0)) : r;
     const uint32_t cc1 = c0 < 8 ? cc : (cc | (1u << (c0 - 8)));
-    st_nt(st4 + ((base | off) >> 1), cat4(X[r * kRowStride + cc], X[r1 * kRowStride + cc1]));
+    st4<NT>(sp + ((base | off) >> 1), cat4(X[r * kRowStride + cc], X[r1 * kRowStride + cc1]));
   }
 }
 
@@ -456,7 +457,8 @@ int qsim_apply_block_mfma(void* stream, void* state, int n_qubits, const void* b
   int qmin = a.q[0];
   for (int j = 1; j < 5; ++j) qmin = std::min(qmin, a.q[j]);
   // Wide kernel (16-byte lanes); variants chosen from 30-qubit measurements:
-  //   no block qubit in 0..3: prefetch + nontemporal, two workgroups per CU;
+  //   no block qubit in 0..3: prefetch, two workgroups per CU, nontemporal
+  //   above the MALL size (cached below it, so the state stays resident);
   //   lowest block qubit in 1..3: prefetch, cached accesses, one per CU;
   //   qubit 0 in the block: no prefetch, cached accesses, two per CU.
   if (n_qubits >= 11 && n_qubits <= 32 && low <= 2 && (qmin != 0 || a.q[0] == 0)) {
@@ -469,8 +471,11 @@ int qsim_apply_block_mfma(void* stream, void* state, int n_qubits, const void* b
     else if (qmin < 4)
       hipLaunchKernelGGL((k_block5w<1, 0, 0>), grid(1), dim3(256), 0, s, (float2*)state, n_sets, a,
                          (const float2*)block_dev);
-    else
+    else if (nt_for_qubits(n_qubits))
       hipLaunchKernelGGL((k_block5w<1, 1, 0>), grid(2), dim3(256), 0, s, (float2*)state, n_sets, a,
+                         (const float2*)block_dev);
+    else
+      hipLaunchKernelGGL((k_block5w<1, 0, 0>), grid(2), dim3(256), 0, s, (float2*)state, n_sets, a,
                          (const float2*)block_dev);
     return launch_status("apply_block_mfma");
   }
@@ -494,8 +499,12 @@ int qsim_apply_block_mfma(void* stream, void* state, int n_qubits, const void* b
     }
     tl.nh = 0;
     for (int k = 7; k < a.nc; ++k) tl.hp[tl.nh++] = a.cp[k];
-    hipLaunchKernelGGL(k_block5_lds, dim3(1ull << tl.nh), dim3(256), 0, as_stream(stream),
-                       (float2*)state, tl, (const float2*)block_dev);
+    if (nt_for_qubits(n_qubits))
+      hipLaunchKernelGGL(k_block5_lds<true>, dim3(1ull << tl.nh), dim3(256), 0, as_stream(stream),
+                         (float2*)state, tl, (const float2*)block_dev);
+    else
+      hipLaunchKernelGGL(k_block5_lds<false>, dim3(1ull << tl.nh), dim3(256), 0, as_stream(stream),
+                         (float2*)state, tl, (const float2*)block_dev);
     return launch_status("apply_block_mfma");
   }
   const uint64_t n_sets = (1ull << (n_qubits - 5)) / 32;

@@ -53,7 +53,7 @@ __device__ __forceinline__ double p2(float2 a) {
 
 // Pass 1.  mode 0: sum |a|^2.  mode 1: also the part with bit t == 0
 // (out[2*b] = total, out[2*b+1] = part with bit t = 0).
-template <int MODE>
+template <int MODE, bool NT>
 __global__ __launch_bounds__(kThreads) void k_red1(const float4* __restrict__ st, uint64_t n4,
                                                    uint64_t chunk, int t, double* __restrict__ out) {
   __shared__ double sh[4];
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kThreads) void k_red1(const float4* __restrict__ st
   for (; f + (kRedUnroll - 1) * kThreads < end; f += kRedUnroll * kThreads) {
     float4 v[kRedUnroll];
 #pragma unroll
-    for (int u = 0; u < kRedUnroll; ++u) v[u] = ld_nt(st + f + u * kThreads);
+    for (int u = 0; u < kRedUnroll; ++u) v[u] = ld4<NT>(st + f + u * kThreads);
 #pragma unroll
     for (int u = 0; u < kRedUnroll; ++u) {
       const double a = p2(lo2(v[u])), b = p2(hi2(v[u]));
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(kThreads) void k_red1(const float4* __restrict__ st
     }
   }
   for (; f < end; f += kThreads) {
-    const float4 v = ld_nt(st + f);
+    const float4 v = ld4<NT>(st + f);
     const double a = p2(lo2(v)), b = p2(hi2(v));
     s += a + b;
     if (MODE == 1) {
@@ -129,11 +129,18 @@ static RedPlan red_plan(int n) {
 int reduce_impl(hipStream_t s, const void* state, int n, int mode, int t, double* ws) {
   const RedPlan p = red_plan(n);
   double* res = ws + 2 * kRedBlocks;
-  if (mode == 0)
-    hipLaunchKernelGGL(k_red1<0>, dim3(p.blocks), dim3(kThreads), 0, s, (const float4*)state,
+  const bool nt = nt_for_qubits(n);
+  if (mode == 0 && nt)
+    hipLaunchKernelGGL((k_red1<0, true>), dim3(p.blocks), dim3(kThreads), 0, s, (const float4*)state,
+                       p.n4, p.chunk, t, ws);
+  else if (mode == 0)
+    hipLaunchKernelGGL((k_red1<0, false>), dim3(p.blocks), dim3(kThreads), 0, s, (const float4*)state,
+                       p.n4, p.chunk, t, ws);
+  else if (nt)
+    hipLaunchKernelGGL((k_red1<1, true>), dim3(p.blocks), dim3(kThreads), 0, s, (const float4*)state,
                        p.n4, p.chunk, t, ws);
   else
-    hipLaunchKernelGGL(k_red1<1>, dim3(p.blocks), dim3(kThreads), 0, s, (const float4*)state,
+    hipLaunchKernelGGL((k_red1<1, false>), dim3(p.blocks), dim3(kThreads), 0, s, (const float4*)state,
                        p.n4, p.chunk, t, ws);
   if (int rc = launch_status("reduce pass 1")) return rc;
   hipLaunchKernelGGL(k_red2, dim3(1), dim3(kThreads), 0, s, ws, p.blocks, mode == 0 ? 1 : 2, res);
@@ -144,6 +151,7 @@ int reduce_impl(hipStream_t s, const void* state, int n, int mode, int t, double
 // res[0] = total, res[1] = p0 (bit t = 0).  The outcome is decided on the
 // device: 0 if random_val * total < p0, else 1; the kept half is scaled by
 // 1/sqrt(p_outcome) in FP64, the other half is zeroed.
+template <bool NT>
 __global__ __launch_bounds__(kThreads) void k_collapse(float4* __restrict__ st, uint64_t n4, int t,
                                                        double u, const double* __restrict__ res,
                                                        int* __restrict__ outcome) {
@@ -154,32 +162,34 @@ __global__ __launch_bounds__(kThreads) void k_collapse(float4* __restrict__ st, 
   const uint64_t f = gblock<uint64_t>() * kThreads + threadIdx.x;
   if (f == 0) *outcome = o;
   if (f >= n4) return;
-  float4 v = ld_nt(st + f);
+  float4 v = ld4<NT>(st + f);
   const uint64_t i0 = 2 * f;
   const int b0 = (int)((i0 >> t) & 1), b1 = (int)(((i0 + 1) >> t) & 1);
   const double k0 = b0 == o ? sc : 0.0, k1 = b1 == o ? sc : 0.0;
   v.x = (float)(v.x * k0); v.y = (float)(v.y * k0);
   v.z = (float)(v.z * k1); v.w = (float)(v.w * k1);
-  st_nt(st + f, v);
+  st4<NT>(st + f, v);
 }
 
+template <bool NT>
 __global__ __launch_bounds__(kThreads) void k_scale(float4* __restrict__ st, uint64_t n4,
                                                     const double* __restrict__ res) {
   const double tot = res[0];
   const double sc = tot > 0.0 ? 1.0 / sqrt(tot) : 0.0;
   const uint64_t f = gblock<uint64_t>() * kThreads + threadIdx.x;
   if (f >= n4) return;
-  float4 v = ld_nt(st + f);
+  float4 v = ld4<NT>(st + f);
   v.x = (float)(v.x * sc); v.y = (float)(v.y * sc);
   v.z = (float)(v.z * sc); v.w = (float)(v.w * sc);
-  st_nt(st + f, v);
+  st4<NT>(st + f, v);
 }
 
+template <bool NT>
 __global__ __launch_bounds__(kThreads) void k_probs(const float4* __restrict__ st, uint64_t n4,
                                                     float2* __restrict__ probs) {
   const uint64_t f = gblock<uint64_t>() * kThreads + threadIdx.x;
   if (f >= n4) return;
-  const float4 v = ld_nt(st + f);
+  const float4 v = ld4<NT>(st + f);
   probs[f] = make_float2((float)p2(lo2(v)), (float)p2(hi2(v)));
 }
 
@@ -237,6 +247,7 @@ __device__ __forceinline__ void leaf_scan(const float4* __restrict__ leaf, int l
 // butterfly of wave_sum.  The in-leaf search recomputes the cumulative sums
 // with leaf_scan; where rounding makes the two disagree, an x past the scan's
 // total takes the leaf's last nonzero amplitude.
+template <bool NT>
 __global__ __launch_bounds__(kThreads) void k_leaf_totals(const float4* __restrict__ st,
                                                           uint64_t n_amps, uint64_t n_leaves,
                                                           double* __restrict__ tot) {
@@ -247,11 +258,11 @@ __global__ __launch_bounds__(kThreads) void k_leaf_totals(const float4* __restri
   const float4* lf = st + leaf * (kLeaf / 2);
   double v = 0.0;
   if (2 * lane < nl) {
-    const float4 a = ld_nt(lf + lane);
+    const float4 a = ld4<NT>(lf + lane);
     v = p2(lo2(a)) + p2(hi2(a));
   }
   if (128 + 2 * lane < nl) {
-    const float4 b = ld_nt(lf + 64 + lane);
+    const float4 b = ld4<NT>(lf + 64 + lane);
     v += p2(lo2(b)) + p2(hi2(b));
   }
   v = wave_sum(v);
@@ -450,7 +461,7 @@ __global__ __launch_bounds__(kThreads) void k_leaf_totals_batch(const qsim_sampl
   if (leaf >= n_leaves) return;
   const qsim_sample_rec& r = recs[find_rec(recs, n_recs, leaf, true)];
   const float4* lf = reinterpret_cast<const float4*>(r.state) + (leaf - r.leaf_off) * (kLeaf / 2);
-  const float4 a = ld_nt(lf + lane), b = ld_nt(lf + 64 + lane);
+  const float4 a = lf[lane], b = lf[64 + lane];   // small states, cached (MALL-resident)
   double v = p2(lo2(a)) + p2(hi2(a));
   v += p2(lo2(b)) + p2(hi2(b));
   v = wave_sum(v);
@@ -529,8 +540,12 @@ int qsim_measure_probs(void* stream, const void* state, int n_qubits, void* prob
   if (int rc = check_n(n_qubits)) return rc;
   if (!state || !probs_dev) return fail(QSIM_EINVAL, "NULL pointer argument");
   const uint64_t n4 = (1ull << n_qubits) / 2;
-  hipLaunchKernelGGL(k_probs, grid1d(ceil_div(n4, kThreads)), dim3(kThreads), 0, as_stream(stream),
-                     (const float4*)state, n4, (float2*)probs_dev);
+  if (nt_for_qubits(n_qubits))
+    hipLaunchKernelGGL(k_probs<true>, grid1d(ceil_div(n4, kThreads)), dim3(kThreads), 0,
+                       as_stream(stream), (const float4*)state, n4, (float2*)probs_dev);
+  else
+    hipLaunchKernelGGL(k_probs<false>, grid1d(ceil_div(n4, kThreads)), dim3(kThreads), 0,
+                       as_stream(stream), (const float4*)state, n4, (float2*)probs_dev);
   return launch_status("measure_probs");
 }
 
@@ -544,8 +559,12 @@ int qsim_measure_qubit(void* stream, void* state, int n_qubits, int target, doub
   if (int rc = reduce_impl(s, state, n_qubits, 1, target, ws)) return rc;
   const uint64_t n4 = (1ull << n_qubits) / 2;
   int* dev_out = (int*)(ws + 2 * kRedBlocks + 4);
-  hipLaunchKernelGGL(k_collapse, grid1d(ceil_div(n4, kThreads)), dim3(kThreads), 0, s, (float4*)state,
-                     n4, target, random_val, ws + 2 * kRedBlocks, dev_out);
+  if (nt_for_qubits(n_qubits))
+    hipLaunchKernelGGL(k_collapse<true>, grid1d(ceil_div(n4, kThreads)), dim3(kThreads), 0, s,
+                       (float4*)state, n4, target, random_val, ws + 2 * kRedBlocks, dev_out);
+  else
+    hipLaunchKernelGGL(k_collapse<false>, grid1d(ceil_div(n4, kThreads)), dim3(kThreads), 0, s,
+                       (float4*)state, n4, target, random_val, ws + 2 * kRedBlocks, dev_out);
   if (int rc = launch_status("measure_qubit collapse")) return rc;
   int host = -1;
   if (hipMemcpyAsync(&host, dev_out, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess)
@@ -562,8 +581,12 @@ int qsim_renormalize(void* stream, void* state, int n_qubits, void* workspace) {
   double* ws = (double*)workspace;
   if (int rc = reduce_impl(s, state, n_qubits, 0, 0, ws)) return rc;
   const uint64_t n4 = (1ull << n_qubits) / 2;
-  hipLaunchKernelGGL(k_scale, grid1d(ceil_div(n4, kThreads)), dim3(kThreads), 0, s, (float4*)state,
-                     n4, ws + 2 * kRedBlocks);
+  if (nt_for_qubits(n_qubits))
+    hipLaunchKernelGGL(k_scale<true>, grid1d(ceil_div(n4, kThreads)), dim3(kThreads), 0, s,
+                       (float4*)state, n4, ws + 2 * kRedBlocks);
+  else
+    hipLaunchKernelGGL(k_scale<false>, grid1d(ceil_div(n4, kThreads)), dim3(kThreads), 0, s,
+                       (float4*)state, n4, ws + 2 * kRedBlocks);
   return launch_status("renormalize");
 }
 
@@ -587,8 +610,12 @@ int qsim_sample(void* stream, const void* state, int n_qubits, const void* unifo
   double* pre = tot + p.nl;
   double* seg = pre + p.nl;
   double* total = seg + p.ns;
-  hipLaunchKernelGGL(k_leaf_totals, grid1d(ceil_div(p.nl, 4)), dim3(kThreads), 0, s,
-                     (const float4*)state, p.n_amps, p.nl, tot);
+  if (nt_for_qubits(n_qubits))
+    hipLaunchKernelGGL(k_leaf_totals<true>, grid1d(ceil_div(p.nl, 4)), dim3(kThreads), 0, s,
+                       (const float4*)state, p.n_amps, p.nl, tot);
+  else
+    hipLaunchKernelGGL(k_leaf_totals<false>, grid1d(ceil_div(p.nl, 4)), dim3(kThreads), 0, s,
+                       (const float4*)state, p.n_amps, p.nl, tot);
   if (int rc = launch_status("sample: leaf totals")) return rc;
   if (p.ns == 1) {
     hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kThreads), 0, s, tot, p.nl, pre, total);
