@@ -48,6 +48,9 @@ from . import workloads as W
 LDS_MAX_QUBITS = 14
 # HBM-lane states up to this size are sampled together (qsim_sample_batch)
 SAMPLE_BATCH_MAX_QUBITS = 20
+# state bytes per HBM-lane batch: the 256 MB MALL (Infinity Cache) of MI355X,
+# which keeps a batch's states resident between its block steps
+HBM_BATCH_BYTES = 1 << 28
 GOLDEN = 0x9E3779B97F4A7C15
 
 
@@ -397,13 +400,18 @@ class Server:
     max_batch        most requests per LDS launch
     max_hbm_jobs     HBM-lane launch units (batches or single circuits) in flight
     max_hbm_batch    most HBM-lane circuits per batch
+    max_hbm_batch_bytes  most state bytes per HBM-lane batch (a larger
+                     circuit runs alone); the default is the 256 MB MALL of
+                     MI355X, so a batch stays cache-resident across its block
+                     steps
     planner_threads  host threads that build HBM-lane block plans ahead
     plan_ahead       most queued HBM-lane circuits planned ahead of admission
     """
 
     def __init__(self, sim=None, budget_bytes=None, max_batch=2048, max_hbm_jobs=4,
                  device="cuda:0", lds_factory=None, hbm_factory=None, planner_threads=8,
-                 plan_ahead=256, max_hbm_batch=64, hbm_batch_factory=None):
+                 plan_ahead=256, max_hbm_batch=64, hbm_batch_factory=None,
+                 max_hbm_batch_bytes=HBM_BATCH_BYTES):
         """lds_factory(reqs) / hbm_factory(req) build lane jobs with launch(),
         done() and finish(now); hbm_batch_factory(jobs) runs several HBM-lane
         jobs as one unit (None: one job per unit).  The defaults run on the
@@ -423,6 +431,7 @@ class Server:
         self.planner_threads = int(planner_threads)
         self.plan_ahead = int(plan_ahead)
         self.max_hbm_batch = int(max_hbm_batch)
+        self.max_hbm_batch_bytes = int(max_hbm_batch_bytes)
         self.hbm_batch_factory = hbm_batch_factory
         self.in_use = 0
         self.peak_in_use = 0
@@ -523,7 +532,7 @@ class Server:
             # HBM lane: ready jobs (planned, within budget) go out together;
             # a job without a gate payload runs alone
             while queue_hbm and len(hbm_inflight) < self.max_hbm_jobs:
-                group = []
+                group, gbytes = [], 0
                 limit = self.max_hbm_batch if self.hbm_batch_factory is not None else 1
                 while queue_hbm and len(group) < limit:
                     j = queue_hbm[0]
@@ -534,9 +543,12 @@ class Server:
                     if group and not batchable:
                         break
                     r = j.reqs[0]
+                    if group and gbytes + r.state_bytes() > self.max_hbm_batch_bytes:
+                        break
                     need = self.admit_bytes(r)
                     if self.in_use + need > self.budget:
                         break
+                    gbytes += r.state_bytes()
                     queue_hbm.popleft()
                     n_planned = max(0, n_planned - 1)
                     self.in_use += need

@@ -116,8 +116,9 @@ __global__ __launch_bounds__(64 * W) void k_block5(float2* __restrict__ st, uint
 // sets w*spw .. w*spw + spw - 1; each set's record is found by a binary
 // search over set_begin (the same for the whole wave), and its groups go
 // through exactly the MFMA sequence of k_block5, so an instance's result
-// does not depend on what else is in the batch.  Row offsets are 32-bit
-// (n <= 32, checked on the host).
+// does not depend on what else is in the batch.  The loads of a wave's next
+// set are issued before the current set's MFMA chains when both sets belong
+// to one record.  Row offsets are 32-bit (n <= 32, checked on the host).
 __global__ __launch_bounds__(256) void k_block5_batch(const qsim_block_rec* __restrict__ recs,
                                                       int n_recs, uint64_t total_sets, int spw) {
   const int lane = threadIdx.x & 63;
@@ -128,52 +129,69 @@ __global__ __launch_bounds__(256) void k_block5_batch(const qsim_block_rec* __re
   uint32_t rof[16];
   int qv[5], qs[5];
   float2* st = nullptr;
-  uint64_t sb = 0;
+  uint64_t sb = 0, se = 0;          // sets [sb, se) belong to record cur
+  float2 x[16], y[16];
+  uint64_t base = 0;
+  bool have = false;                // x already holds this set's amplitudes
   for (int k = 0; k < spw; ++k) {
     const uint64_t set = wave * spw + k;
     if (set >= total_sets) break;
-    int lo = 0, hi = n_recs - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (recs[mid].set_begin <= set) lo = mid; else hi = mid - 1;
-    }
-    if (lo != cur) {
-      cur = lo;
-      const qsim_block_rec& r = recs[lo];
-      st = reinterpret_cast<float2*>(r.state);
-      sb = r.set_begin;
-      const float2* U = reinterpret_cast<const float2*>(r.mat);
-#pragma unroll
-      for (int s = 0; s < 16; ++s) {
-        const float2 u = U[c * 32 + 2 * s + h];
-        ur[s] = u.x;
-        ui[s] = u.y;
+    if (!have) {
+      int lo = 0, hi = n_recs - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (recs[mid].set_begin <= set) lo = mid; else hi = mid - 1;
       }
+      if (lo != cur) {
+        cur = lo;
+        const qsim_block_rec& r = recs[lo];
+        st = reinterpret_cast<float2*>(r.state);
+        sb = r.set_begin;
+        se = lo + 1 < n_recs ? recs[lo + 1].set_begin : total_sets;
+        const float2* U = reinterpret_cast<const float2*>(r.mat);
 #pragma unroll
-      for (int j = 0; j < 5; ++j) qv[j] = r.q[j];
+        for (int s = 0; s < 16; ++s) {
+          const float2 u = U[c * 32 + 2 * s + h];
+          ur[s] = u.x;
+          ui[s] = u.y;
+        }
 #pragma unroll
-      for (int s = 0; s < 16; ++s) {
-        const int row = 2 * s + h;
-        uint32_t o = 0;
+        for (int j = 0; j < 5; ++j) qv[j] = r.q[j];
 #pragma unroll
-        for (int j = 0; j < 5; ++j) o |= (uint32_t)((row >> j) & 1) << qv[j];
-        rof[s] = o;
+        for (int s = 0; s < 16; ++s) {
+          const int row = 2 * s + h;
+          uint32_t o = 0;
+#pragma unroll
+          for (int j = 0; j < 5; ++j) o |= (uint32_t)((row >> j) & 1) << qv[j];
+          rof[s] = o;
+        }
+        // ascending block positions, for inserting zeros into the group index
+#pragma unroll
+        for (int j = 0; j < 5; ++j) qs[j] = qv[j];
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+#pragma unroll
+          for (int j = 0; j < 4 - i; ++j)
+            if (qs[j] > qs[j + 1]) { const int t = qs[j]; qs[j] = qs[j + 1]; qs[j + 1] = t; }
       }
-      // ascending block positions, for inserting zeros into the group index
+      base = (set - sb) * 32 + c;
 #pragma unroll
-      for (int j = 0; j < 5; ++j) qs[j] = qv[j];
+      for (int j = 0; j < 5; ++j) base = ins0(base, qs[j]);
 #pragma unroll
-      for (int i = 0; i < 5; ++i)
-#pragma unroll
-        for (int j = 0; j < 4 - i; ++j)
-          if (qs[j] > qs[j + 1]) { const int t = qs[j]; qs[j] = qs[j + 1]; qs[j + 1] = t; }
+      for (int s = 0; s < 16; ++s) x[s] = st[base | rof[s]];
     }
-    uint64_t base = (set - sb) * 32 + c;
+    // the wave's next set, when it belongs to the same record, is loaded
+    // before this set's MFMA chains
+    const uint64_t nset = set + 1;
+    const bool pre = k + 1 < spw && nset < se;
+    uint64_t nbase = 0;
+    if (pre) {
+      nbase = (nset - sb) * 32 + c;
 #pragma unroll
-    for (int j = 0; j < 5; ++j) base = ins0(base, qs[j]);
-    float2 x[16];
+      for (int j = 0; j < 5; ++j) nbase = ins0(nbase, qs[j]);
 #pragma unroll
-    for (int s = 0; s < 16; ++s) x[s] = st[base | rof[s]];
+      for (int s = 0; s < 16; ++s) y[s] = st[nbase | rof[s]];
+    }
     f32x16 accr = {0.f}, acci = {0.f};
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
@@ -190,6 +208,12 @@ __global__ __launch_bounds__(256) void k_block5_batch(const qsim_block_rec* __re
 #pragma unroll
       for (int j = 0; j < 5; ++j) o |= (uint32_t)((i >> j) & 1) << qv[j];
       st[base | o] = make_float2(accr[r], acci[r]);
+    }
+    have = pre;
+    if (pre) {
+      base = nbase;
+#pragma unroll
+      for (int s = 0; s < 16; ++s) x[s] = y[s];
     }
   }
 }
@@ -542,7 +566,7 @@ int qsim_apply_block_batch(void* stream, const qsim_block_rec* recs_host, const 
                   (long long)i, (unsigned long long)r.set_begin, (unsigned long long)total);
     total += 1ull << (r.n_qubits - 10);
   }
-  const int spw = total < 8192 ? 1 : (total < 65536 ? 2 : 4);
+  const int spw = total < 2048 ? 1 : (total < 16384 ? 2 : 4);
   const uint64_t waves = ceil_div(total, spw);
   if (ceil_div(waves, 4) > 0x7fffffffull) return fail(QSIM_ERANGE, "batch too large (%llu sets)",
                                                      (unsigned long long)total);

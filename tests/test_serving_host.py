@@ -127,3 +127,33 @@ def test_hbm_lane_batches_payload_requests_within_budget():
     assert sorted(sum(batches, [])) == list(range(10))
     assert s.peak_in_use <= budget
     assert rs[10].samples is not None and 10 not in sum(batches, [])
+
+
+def test_hbm_lane_batches_capped_by_state_bytes():
+    """A batch holds at most max_hbm_batch_bytes of states; a circuit larger
+    than the cap still runs, alone."""
+    from blackwell_backend.serving import Request, Server
+
+    class FakeHbm(FakeJob):
+        def __init__(self, req):
+            super().__init__(req, 0.002)
+            self.req, self.prepared = req, None
+
+    batches = []
+
+    def batch_factory(jobs):
+        batches.append([j.req.n_qubits for j in jobs])
+        return FakeJob([j.req for j in jobs], 0.002)
+
+    cap = 4 * (1 << 18) * 8                                  # four 18-qubit states
+    s = Server(budget_bytes=1 << 40, lds_factory=lambda rs: FakeJob(rs, 0.001), hbm_factory=FakeHbm,
+               hbm_batch_factory=batch_factory, max_hbm_batch=64, planner_threads=0,
+               max_hbm_batch_bytes=cap)
+    ns = [18] * 9 + [20] + [17] * 4
+    rs = [Request(i, 0.0, n, 16, 5 + i, gates=([(0, 1)], [None])) for i, n in enumerate(ns)]
+    st = s.serve(rs)
+    assert st["served"] == len(rs)
+    assert sorted(sum(batches, [])) == sorted(ns)
+    for b in batches:
+        assert len(b) == 1 or sum(8 << n for n in b) <= cap
+    assert [20] in batches                                   # larger than the cap: alone
