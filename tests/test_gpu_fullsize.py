@@ -7,7 +7,9 @@ that size:
     lane and float4 bits) followed by its reverse restores a random state
     bit for bit;
   * the FP64 leaf sampler at 32 qubits returns only indices whose amplitude
-    is nonzero and reproduces them for the same uniforms.
+    is nonzero and reproduces them for the same uniforms;
+  * at 26 qubits (just above the MALL size) a gate sequence through the
+    streaming forms of the kernels matches the FP64 oracle directly.
 The tests keep every state on the device (no 2^32-element host copies)."""
 import math
 
@@ -194,3 +196,66 @@ def test_33_qubit_state_64bit_paths():
     del st
     torch.cuda.empty_cache()
     assert p0 >= 1 - 1e-6 and abs(nrm - 1) <= 1e-6, (p0, nrm)
+
+
+def test_hbm_streaming_paths_26q_match_oracle():
+    """At 26 qubits (512 MiB) the state is larger than the 256 MB MALL, so the
+    kernels take their nontemporal forms, and 1Q stream targets other than
+    12 and 13 take the form with a second pair 32 KiB away.  A sequence
+    covering those paths is compared with the FP64 oracle on the same
+    complex64 input (compensated FP32 bar, relative to max |a|), and the
+    permutation gates in it are checked bit for bit on their own."""
+    import blackwell_kernels as bk
+    from oracle import statevector as SV
+    from helpers import rand_state, rand_unitary, close
+    n = 26
+    print(f"[fullsize] streaming paths at {n} qubits vs the FP64 oracle", flush=True)
+    dev = torch.device("cuda", 0)
+    psi = rand_state(n, 26)
+    st = torch.from_numpy(psi).to(dev)
+    ref = psi.astype(np.complex128)
+    u2 = [rand_unitary(2, 100 + k).astype(np.complex64) for k in range(5)]
+    u4 = [rand_unitary(4, 200 + k).astype(np.complex64) for k in range(3)]
+    d2 = np.array([np.exp(-0.4j), np.exp(0.7j)], dtype=np.complex64)
+    ops = [("1q", u2[0], 7), ("1q", u2[1], 12), ("1q", u2[2], 25), ("1q", u2[3], 3),
+           ("2q", u4[0], (7, 21)), ("2q", u4[1], (3, 20)), ("2q", u4[2], (0, 1)),
+           ("diag", d2, 5), ("phase", np.complex64(np.exp(0.3j)), 18),
+           ("mcx", None, ([17], 9)), ("mcx", None, ([2, 11, 25], 19)),
+           ("swap", None, (4, 23)), ("mcg", u2[4], ([20], 8))]
+    for kind, m, q in ops:
+        if kind == "1q":
+            bk.apply_gate(st, torch.from_numpy(m).to(dev), q)
+            ref = SV.apply_1q(ref, m, q)
+        elif kind == "2q":
+            bk.apply_gate_2q(st, torch.from_numpy(m).to(dev), *q)
+            ref = SV.apply_2q_kernel(ref, m, *q)
+        elif kind == "diag":
+            bk.apply_diagonal(st, torch.from_numpy(m).to(dev), q)
+            ref = SV.apply_diag(ref, m, q)
+        elif kind == "phase":
+            bk.apply_diagonal_phase(st, complex(m), q)
+            ref = SV.apply_diag_phase(ref, complex(m), q)
+        elif kind == "mcx":
+            bk.apply_mcx(st, *q)
+            ref = ref[SV.mcx_perm(n, *q)]
+        elif kind == "swap":
+            bk.apply_mcswap(st, [], *q)
+            ref = ref[SV.mcswap_perm(n, [], *q)]
+        else:
+            bk.apply_mcgate(st, torch.from_numpy(m).to(dev), q[0], q[1])
+            ref = SV.apply_mc(ref, m, q[0], [q[1]])
+    got = st.cpu().numpy()
+    ok, err = close(ref, got, rtol=2e-5, atol=0)
+    assert ok, err
+    assert abs(float(bk.norm2(st)) - float(np.vdot(ref, ref).real)) < 1e-5
+    # permutations alone are bit-exact on this path
+    perm_in = torch.from_numpy(psi).to(dev)
+    bk.apply_mcx(perm_in, [17], 9)
+    bk.apply_mcx(perm_in, [2, 11, 25], 19)
+    bk.apply_mcswap(perm_in, [], 4, 23)
+    want = psi[SV.mcx_perm(n, [17], 9)]
+    want = want[SV.mcx_perm(n, [2, 11, 25], 19)]
+    want = want[SV.mcswap_perm(n, [], 4, 23)]
+    assert np.array_equal(perm_in.cpu().numpy().view(np.uint32), want.view(np.uint32))
+    del st, perm_in
+    torch.cuda.empty_cache()
