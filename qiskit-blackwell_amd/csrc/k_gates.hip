@@ -299,10 +299,12 @@ __device__ __forceinline__ void perm_rows0(const float2 g[16], bool b1, float2 h
 template <typename I, int S, bool NT>
 __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I n_work, int lo,
                                                      int hi, const float2* __restrict__ gdev,
-                                                     M4 gv) {
+                                                     M4 gv, PassOrder po) {
   float2 g[16];
   load_m4(gdev, gv, g);
-  const I w0 = (S > 0 ? xcd_block<I>() : gblock<I>()) * kThreads + threadIdx.x;
+  bool tail = false;
+  const I blk = NT ? pass_block<I, (S > 0)>(po, tail) : (S > 0 ? xcd_block<I>() : gblock<I>());
+  const I w0 = blk * kThreads + threadIdx.x;
   const bool live = w0 < n_work;
   const I w = live ? w0 : n_work - 1;
   const int lane = threadIdx.x & 63;
@@ -315,8 +317,8 @@ __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I 
     mat4(g, a0, a1, a2, a3);
     mat4(g, b0, b1, b2, b3);
     if (live) {
-      st4<NT>(st + f0, cat4(a0, b0)); st4<NT>(st + f1, cat4(a1, b1));
-      st4<NT>(st + f2, cat4(a2, b2)); st4<NT>(st + f3, cat4(a3, b3));
+      st4p<NT>(st + f0, cat4(a0, b0), tail); st4p<NT>(st + f1, cat4(a1, b1), tail);
+      st4p<NT>(st + f2, cat4(a2, b2), tail); st4p<NT>(st + f3, cat4(a3, b3), tail);
     }
   } else if (S == 1) {
     // hi is a stream: slot f0 has bit(hi) = 0, slot f1 bit(hi) = 1
@@ -348,7 +350,7 @@ __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I 
       o0 = cat4(y[0][0], y[1][0]);
       o1 = cat4(y[0][1], y[1][1]);
     }
-    if (live) { st4<NT>(st + f0, o0); st4<NT>(st + f1, o1); }
+    if (live) { st4p<NT>(st + f0, o0, tail); st4p<NT>(st + f1, o1, tail); }
   } else {
     const float4 v = ld4<NT>(st + w);
     const int mh = 1 << (hi - 1);
@@ -372,7 +374,7 @@ __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I 
       o = cat4(crow4(h, lo2(v), lo2(pl), lo2(ph), lo2(pb)),
                crow4(h, hi2(v), hi2(pl), hi2(ph), hi2(pb)));
     }
-    if (live) st4<NT>(st + w, o);
+    if (live) st4p<NT>(st + w, o, tail);
   }
 }
 
@@ -388,20 +390,21 @@ int gate2q_impl(void* stream, void* state, int n, const float2* gdev, const M4& 
   const int S = (lo >= 7) + (hi >= 7);
   const uint64_t work = ((1ull << n) / 2) >> S;
   const dim3 grid = grid1d(ceil_div(work, kThreads)), block(kThreads);
+  const PassOrder po = nt_for_qubits(n) ? next_pass(8ull << n) : PassOrder{0, 0};
   if (n > 32) {
-    if (S == 0) hipLaunchKernelGGL((k_gate2q<uint64_t, 0, true>), grid, block, 0, s, (float4*)state, work, lo, hi, gdev, gv);
-    if (S == 1) hipLaunchKernelGGL((k_gate2q<uint64_t, 1, true>), grid, block, 0, s, (float4*)state, work, lo, hi, gdev, gv);
-    if (S == 2) hipLaunchKernelGGL((k_gate2q<uint64_t, 2, true>), grid, block, 0, s, (float4*)state, work, lo, hi, gdev, gv);
+    if (S == 0) hipLaunchKernelGGL((k_gate2q<uint64_t, 0, true>), grid, block, 0, s, (float4*)state, work, lo, hi, gdev, gv, po);
+    if (S == 1) hipLaunchKernelGGL((k_gate2q<uint64_t, 1, true>), grid, block, 0, s, (float4*)state, work, lo, hi, gdev, gv, po);
+    if (S == 2) hipLaunchKernelGGL((k_gate2q<uint64_t, 2, true>), grid, block, 0, s, (float4*)state, work, lo, hi, gdev, gv, po);
   } else if (nt_for_qubits(n)) {
     const uint32_t w = (uint32_t)work;
-    if (S == 0) hipLaunchKernelGGL((k_gate2q<uint32_t, 0, true>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
-    if (S == 1) hipLaunchKernelGGL((k_gate2q<uint32_t, 1, true>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
-    if (S == 2) hipLaunchKernelGGL((k_gate2q<uint32_t, 2, true>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
+    if (S == 0) hipLaunchKernelGGL((k_gate2q<uint32_t, 0, true>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv, po);
+    if (S == 1) hipLaunchKernelGGL((k_gate2q<uint32_t, 1, true>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv, po);
+    if (S == 2) hipLaunchKernelGGL((k_gate2q<uint32_t, 2, true>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv, po);
   } else {
     const uint32_t w = (uint32_t)work;
-    if (S == 0) hipLaunchKernelGGL((k_gate2q<uint32_t, 0, false>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
-    if (S == 1) hipLaunchKernelGGL((k_gate2q<uint32_t, 1, false>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
-    if (S == 2) hipLaunchKernelGGL((k_gate2q<uint32_t, 2, false>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv);
+    if (S == 0) hipLaunchKernelGGL((k_gate2q<uint32_t, 0, false>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv, po);
+    if (S == 1) hipLaunchKernelGGL((k_gate2q<uint32_t, 1, false>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv, po);
+    if (S == 2) hipLaunchKernelGGL((k_gate2q<uint32_t, 2, false>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv, po);
   }
   return launch_status("apply_gate_2q");
 }

@@ -13,6 +13,7 @@
 // full Dot2 would need at 8 TB/s).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <stdint.h>
 #include <stdarg.h>
 #include "qsim_gfx950.h"
@@ -174,6 +175,50 @@ template <bool NT>
 __device__ __forceinline__ void st4(float4* p, float4 v) {
   if constexpr (NT) st_nt(p, v);
   else *p = v;
+}
+
+// Pass order across consecutive launches of streaming kernels.  A pass over
+// a state larger than the MALL ends with its last-processed part written
+// through the cache (cached stores), and the next pass runs in the opposite
+// direction, so it starts on what is still in the MALL.  tail1024 is the
+// part of each pass (in 1/1024) stored with cached writes: the MALL size
+// over the state size.  The order changes nothing in the results; it is
+// flipped by a process-wide counter at every such launch.
+struct PassOrder {
+  int rev;
+  int tail1024;
+};
+inline PassOrder next_pass(uint64_t state_bytes) {
+  static std::atomic<unsigned> counter{0};
+  PassOrder p;
+  p.rev = (int)(counter.fetch_add(1, std::memory_order_relaxed) & 1u);
+  const uint64_t t = state_bytes ? (kMallBytes * 1024) / state_bytes : 1024;
+  p.tail1024 = (int)(t < 1024 ? t : 1024);
+  return p;
+}
+// Address-order block of this workgroup under the pass order, and whether it
+// lies in the pass's tail.  XCD: the XCD-major numbering of xcd_block (each
+// XCD walks its own contiguous eighth, which it then walks backwards in a
+// reversed pass).
+template <typename I, bool XCD>
+__device__ __forceinline__ I pass_block(const PassOrder& po, bool& tail) {
+  const I nb = (I)gridDim.x * (I)gridDim.y, b = (I)blockIdx.y * (I)gridDim.x + (I)blockIdx.x;
+  const bool x = XCD && (nb & 7) == 0;
+  const I len = x ? nb >> 3 : nb;
+  const I pos = x ? b >> 3 : b;
+  const I bx = x ? (b & 7) * len + pos : b;
+  tail = (uint64_t)pos * 1024 >= (uint64_t)len * (uint64_t)(1024 - po.tail1024);
+  return po.rev ? nb - 1 - bx : bx;
+}
+// Store of a streaming kernel: nontemporal, except in the pass's tail.
+template <bool NT>
+__device__ __forceinline__ void st4p(float4* p, float4 v, bool tail) {
+  if constexpr (NT) {
+    if (tail) *p = v;
+    else st_nt(p, v);
+  } else {
+    *p = v;
+  }
 }
 
 // Cross-lane exchange of a float4 with lane ^ m (m < 64).
