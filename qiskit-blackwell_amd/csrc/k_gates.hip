@@ -86,11 +86,12 @@ int build_mc(McSpec& s, int n, const int64_t* controls, int n_controls, const in
 
 // ============================================================ state init
 template <typename I, bool NT>
-__global__ __launch_bounds__(kThreads) void k_init(float4* __restrict__ st, I n4, I per_state4) {
-  const I f = gblock<I>() * kThreads + threadIdx.x;
+__global__ __launch_bounds__(kThreads) void k_init(float4* __restrict__ st, I n4, I per_state4, PassOrder po) {
+  bool tail = false;
+  const I f = pass_blk<I, NT, false>(po, tail) * kThreads + threadIdx.x;
   if (f >= n4) return;
   const bool first = (f & (per_state4 - 1)) == 0;
-  st4<NT>(st + f, make_float4(first ? 1.f : 0.f, 0.f, 0.f, 0.f));
+  st4p<NT>(st + f, make_float4(first ? 1.f : 0.f, 0.f, 0.f, 0.f), tail);
 }
 
 static int init_impl(void* stream, void* states, uint64_t n_states, int n) {
@@ -102,13 +103,13 @@ static int init_impl(void* stream, void* states, uint64_t n_states, int n) {
   const uint64_t blocks = ceil_div(n4, kThreads);
   if (n4 > 0xffffffffull)
     hipLaunchKernelGGL((k_init<uint64_t, true>), grid1d(blocks), dim3(kThreads), 0, as_stream(stream),
-                       (float4*)states, n4, per4);
+                       (float4*)states, n4, per4, next_pass(n4 * 16));
   else if (nt_for_bytes(n4 * 16))
     hipLaunchKernelGGL((k_init<uint32_t, true>), grid1d(blocks), dim3(kThreads), 0, as_stream(stream),
-                       (float4*)states, (uint32_t)n4, (uint32_t)per4);
+                       (float4*)states, (uint32_t)n4, (uint32_t)per4, next_pass(n4 * 16));
   else
     hipLaunchKernelGGL((k_init<uint32_t, false>), grid1d(blocks), dim3(kThreads), 0, as_stream(stream),
-                       (float4*)states, (uint32_t)n4, (uint32_t)per4);
+                       (float4*)states, (uint32_t)n4, (uint32_t)per4, PassOrder{0, 0});
   return launch_status("state_init");
 }
 
@@ -137,16 +138,17 @@ __device__ __forceinline__ void load_m2(const float2* gdev, const M2& gv, float2
 // bit n-2 and t < n keeps every pair inside one state.
 template <typename I, int MODE, bool NT>
 __global__ __launch_bounds__(kThreads) void k_gate1q(float4* __restrict__ st, I n_work, int t,
-                                                     const float2* __restrict__ gdev, M2 gv) {
+                                                     const float2* __restrict__ gdev, M2 gv, PassOrder po) {
+  bool tail = false;
   float2 g[4];
   load_m2(gdev, gv, g);
-  const I w0 = (MODE >= 2 ? xcd_block<I>() : gblock<I>()) * kThreads + threadIdx.x;
+  const I w0 = pass_blk<I, NT, (MODE >= 2)>(po, tail) * kThreads + threadIdx.x;
   const bool live = w0 < n_work;
   const I w = live ? w0 : n_work - 1;
   if (MODE == 0) {
     const float4 v = ld4<NT>(st + w);
     const float4 r = cat4(crow2(g[0], lo2(v), g[1], hi2(v)), crow2(g[2], lo2(v), g[3], hi2(v)));
-    if (live) st4<NT>(st + w, r);
+    if (live) st4p<NT>(st + w, r, tail);
   } else if (MODE == 1) {
     // lane l holds slot w with w & 63 == l; bit (t-1) of the slot index is
     // bit t of the amplitude index.  Partner lanes of live lanes are live
@@ -157,14 +159,14 @@ __global__ __launch_bounds__(kThreads) void k_gate1q(float4* __restrict__ st, I 
     const bool low = (threadIdx.x & m) == 0;
     const float2 ga = low ? g[0] : g[3], gb = low ? g[1] : g[2];
     const float4 r = cat4(crow2(ga, lo2(v), gb, lo2(p)), crow2(ga, hi2(v), gb, hi2(p)));
-    if (live) st4<NT>(st + w, r);
+    if (live) st4p<NT>(st + w, r, tail);
   } else if (MODE == 2) {
     const int s = t - 1;
     const I f0 = ins0(w, s), f1 = f0 + (I(1) << s);
     const float4 v0 = ld4<NT>(st + f0), v1 = ld4<NT>(st + f1);
     const float4 r0 = cat4(crow2(g[0], lo2(v0), g[1], lo2(v1)), crow2(g[0], hi2(v0), g[1], hi2(v1)));
     const float4 r1 = cat4(crow2(g[2], lo2(v0), g[3], lo2(v1)), crow2(g[2], hi2(v0), g[3], hi2(v1)));
-    if (live) { st4<NT>(st + f0, r0); st4<NT>(st + f1, r1); }
+    if (live) { st4p<NT>(st + f0, r0, tail); st4p<NT>(st + f1, r1, tail); }
   } else {
     const int s = t - 1, ps = 11;
     const I f0 = ins0(ins0(w, s < ps ? s : ps), s < ps ? ps : s);
@@ -176,8 +178,8 @@ __global__ __launch_bounds__(kThreads) void k_gate1q(float4* __restrict__ st, I 
     const float4 r2 = cat4(crow2(g[0], lo2(v2), g[1], lo2(v3)), crow2(g[0], hi2(v2), g[1], hi2(v3)));
     const float4 r3 = cat4(crow2(g[2], lo2(v2), g[3], lo2(v3)), crow2(g[2], hi2(v2), g[3], hi2(v3)));
     if (live) {
-      st4<NT>(st + f0, r0); st4<NT>(st + f0 + ms, r1);
-      st4<NT>(st + f0 + mp, r2); st4<NT>(st + f0 + mp + ms, r3);
+      st4p<NT>(st + f0, r0, tail); st4p<NT>(st + f0 + ms, r1, tail);
+      st4p<NT>(st + f0 + mp, r2, tail); st4p<NT>(st + f0 + mp + ms, r3, tail);
     }
   }
 }
@@ -218,24 +220,25 @@ int gate1q_impl(void* stream, void* states, uint64_t n_states, int n, const floa
   int mode = t == 0 ? 0 : (t < 7 ? 1 : 2);
   if (mode == 2 && t != 12 && t != 13 && nt_for_bytes(slots * 16)) mode = 3;
   const uint64_t work = mode == 3 ? slots / 4 : (mode == 2 ? slots / 2 : slots);
+  const PassOrder po = nt_for_bytes(slots * 16) ? next_pass(slots * 16) : PassOrder{0, 0};
   const dim3 grid = grid1d(ceil_div(work, kThreads)), block(kThreads);
   if (slots > 0xffffffffull) {
-    if (mode == 0) hipLaunchKernelGGL((k_gate1q<uint64_t, 0, true>), grid, block, 0, s, (float4*)states, work, t, gdev, gv);
-    if (mode == 1) hipLaunchKernelGGL((k_gate1q<uint64_t, 1, true>), grid, block, 0, s, (float4*)states, work, t, gdev, gv);
-    if (mode == 2) hipLaunchKernelGGL((k_gate1q<uint64_t, 2, true>), grid, block, 0, s, (float4*)states, work, t, gdev, gv);
-    if (mode == 3) hipLaunchKernelGGL((k_gate1q<uint64_t, 3, true>), grid, block, 0, s, (float4*)states, work, t, gdev, gv);
+    if (mode == 0) hipLaunchKernelGGL((k_gate1q<uint64_t, 0, true>), grid, block, 0, s, (float4*)states, work, t, gdev, gv, po);
+    if (mode == 1) hipLaunchKernelGGL((k_gate1q<uint64_t, 1, true>), grid, block, 0, s, (float4*)states, work, t, gdev, gv, po);
+    if (mode == 2) hipLaunchKernelGGL((k_gate1q<uint64_t, 2, true>), grid, block, 0, s, (float4*)states, work, t, gdev, gv, po);
+    if (mode == 3) hipLaunchKernelGGL((k_gate1q<uint64_t, 3, true>), grid, block, 0, s, (float4*)states, work, t, gdev, gv, po);
   } else if (nt_for_bytes(slots * 16)) {
     const uint32_t w = (uint32_t)work;
-    if (mode == 0) hipLaunchKernelGGL((k_gate1q<uint32_t, 0, true>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
-    if (mode == 1) hipLaunchKernelGGL((k_gate1q<uint32_t, 1, true>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
-    if (mode == 2) hipLaunchKernelGGL((k_gate1q<uint32_t, 2, true>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
-    if (mode == 3) hipLaunchKernelGGL((k_gate1q<uint32_t, 3, true>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
+    if (mode == 0) hipLaunchKernelGGL((k_gate1q<uint32_t, 0, true>), grid, block, 0, s, (float4*)states, w, t, gdev, gv, po);
+    if (mode == 1) hipLaunchKernelGGL((k_gate1q<uint32_t, 1, true>), grid, block, 0, s, (float4*)states, w, t, gdev, gv, po);
+    if (mode == 2) hipLaunchKernelGGL((k_gate1q<uint32_t, 2, true>), grid, block, 0, s, (float4*)states, w, t, gdev, gv, po);
+    if (mode == 3) hipLaunchKernelGGL((k_gate1q<uint32_t, 3, true>), grid, block, 0, s, (float4*)states, w, t, gdev, gv, po);
   } else {
     const uint32_t w = (uint32_t)work;
-    if (mode == 0) hipLaunchKernelGGL((k_gate1q<uint32_t, 0, false>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
-    if (mode == 1) hipLaunchKernelGGL((k_gate1q<uint32_t, 1, false>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
-    if (mode == 2) hipLaunchKernelGGL((k_gate1q<uint32_t, 2, false>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
-    if (mode == 3) hipLaunchKernelGGL((k_gate1q<uint32_t, 3, false>), grid, block, 0, s, (float4*)states, w, t, gdev, gv);
+    if (mode == 0) hipLaunchKernelGGL((k_gate1q<uint32_t, 0, false>), grid, block, 0, s, (float4*)states, w, t, gdev, gv, po);
+    if (mode == 1) hipLaunchKernelGGL((k_gate1q<uint32_t, 1, false>), grid, block, 0, s, (float4*)states, w, t, gdev, gv, po);
+    if (mode == 2) hipLaunchKernelGGL((k_gate1q<uint32_t, 2, false>), grid, block, 0, s, (float4*)states, w, t, gdev, gv, po);
+    if (mode == 3) hipLaunchKernelGGL((k_gate1q<uint32_t, 3, false>), grid, block, 0, s, (float4*)states, w, t, gdev, gv, po);
   }
   return launch_status("apply_gate");
 }
@@ -413,7 +416,8 @@ int gate2q_impl(void* stream, void* state, int n, const float2* gdev, const M4& 
 // kind 0: diag(d0, d1) on bit t;  kind 1: diag(d00, d01, d10, d11) on (lo, hi).
 template <typename I, int U, int KIND, bool NT>
 __global__ __launch_bounds__(kThreads) void k_diag(float4* __restrict__ st, I n4, int t, int hi,
-                                                   const float2* __restrict__ ddev, M2 dv) {
+                                                   const float2* __restrict__ ddev, M2 dv, PassOrder po) {
+  bool tail = false;
   constexpr int kind = KIND;
   float2 d0, d1, d2, d3;
   if (ddev) {
@@ -422,7 +426,7 @@ __global__ __launch_bounds__(kThreads) void k_diag(float4* __restrict__ st, I n4
   } else {
     d0 = dv.g[0]; d1 = dv.g[1]; d2 = dv.g[2]; d3 = dv.g[3];
   }
-  const I base = gblock<I>() * (kThreads * U) + threadIdx.x;
+  const I base = pass_blk<I, NT, false>(po, tail) * (kThreads * U) + threadIdx.x;
   float4 v[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) {
@@ -439,7 +443,7 @@ __global__ __launch_bounds__(kThreads) void k_diag(float4* __restrict__ st, I n4
     const float2 e0 = h0 ? (b0 ? d3 : d2) : (b0 ? d1 : d0);
     const float2 e1 = h1 ? (b1 ? d3 : d2) : (b1 ? d1 : d0);
     const float4 r = cat4(cmulc(lo2(v[u]), e0), cmulc(hi2(v[u]), e1));
-    if (f < n4) st4<NT>(st + f, r);
+    if (f < n4) st4p<NT>(st + f, r, tail);
   }
 }
 
@@ -453,8 +457,9 @@ __global__ __launch_bounds__(kThreads) void k_diag(float4* __restrict__ st, I n4
 // accesses are nontemporal.
 template <typename I, int U, int MODE, bool NT>
 __global__ __launch_bounds__(kThreads) void k_phase(float4* __restrict__ st, I n_items, int t,
-                                                    float2 ph) {
-  const I base = gblock<I>() * (kThreads * U) + threadIdx.x;
+                                                    float2 ph, PassOrder po) {
+  bool tail = false;
+  const I base = pass_blk<I, NT, false>(po, tail) * (kThreads * U) + threadIdx.x;
   const int s = t - 1;
   I f[U];
   float4 v[U];
@@ -472,7 +477,7 @@ __global__ __launch_bounds__(kThreads) void k_phase(float4* __restrict__ st, I n
     if (MODE == 0) r = cat4(lo2(v[u]), cmulc(hi2(v[u]), ph));
     else if (MODE == 1 && !((f[u] >> s) & 1)) r = v[u];
     else r = cat4(cmulc(lo2(v[u]), ph), cmulc(hi2(v[u]), ph));
-    if (w < n_items) st4<NT>(st + f[u], r);
+    if (w < n_items) st4p<NT>(st + f[u], r, tail);
   }
 }
 
@@ -482,7 +487,8 @@ template <typename I, int KIND, bool NT>
 static void launch_diag(hipStream_t s, uint64_t blocks, void* state, uint64_t n4, int t, int hi,
                         const float2* ddev, const M2& dv) {
   hipLaunchKernelGGL((k_diag<I, kUd, KIND, NT>), grid1d(blocks), dim3(kThreads), 0, s,
-                     (float4*)state, (I)n4, t, hi, ddev, dv);
+                     (float4*)state, (I)n4, t, hi, ddev, dv,
+                     NT ? next_pass(16 * n4) : PassOrder{0, 0});
 }
 
 static int diag_impl(void* stream, void* state, int n, int kind, int q0, int q1,
@@ -520,10 +526,10 @@ static void launch_phase(hipStream_t s, void* state, uint64_t items, int t, floa
   const dim3 grid = grid1d(ceil_div(items, (uint64_t)kThreads * kUd));
   if (nt)
     hipLaunchKernelGGL((k_phase<I, kUd, MODE, true>), grid, dim3(kThreads), 0, s, (float4*)state,
-                       (I)items, t, ph);
+                       (I)items, t, ph, next_pass(16 * items));
   else
     hipLaunchKernelGGL((k_phase<I, kUd, MODE, false>), grid, dim3(kThreads), 0, s, (float4*)state,
-                       (I)items, t, ph);
+                       (I)items, t, ph, PassOrder{0, 0});
 }
 
 static int phase_impl(void* stream, void* state, int n, float2 ph, int t) {
@@ -618,7 +624,8 @@ __global__ __launch_bounds__(kThreads) void k_mc(float2* __restrict__ st, I n_it
 // (nontemporal 32 B runs measured 30-40 % slower).
 template <typename I, int U, bool NT>
 __global__ __launch_bounds__(kThreads) void k_mc4(float4* __restrict__ st, I n_items, McArgs a,
-                                                  const float2* __restrict__ gdev) {
+                                                  const float2* __restrict__ gdev, PassOrder po) {
+  bool tail = false;
   float2 g[16];
   if (a.op == 0) {
 #pragma unroll
@@ -627,7 +634,7 @@ __global__ __launch_bounds__(kThreads) void k_mc4(float4* __restrict__ st, I n_i
 #pragma unroll
     for (int k = 0; k < 16; ++k) g[k] = gdev[k];
   }
-  const I base = gblock<I>() * (kThreads * U) + threadIdx.x;
+  const I base = pass_blk<I, NT, false>(po, tail) * (kThreads * U) + threadIdx.x;
   const I m0 = I(1) << a.t0;
   const I m1 = (a.op >= 2) ? (I(1) << a.t1) : I(0);
 #pragma unroll
@@ -638,15 +645,15 @@ __global__ __launch_bounds__(kThreads) void k_mc4(float4* __restrict__ st, I n_i
     if (a.op < 2) {
       const float4 x0 = ld4<NT>(st + (f0)), x1 = ld4<NT>(st + (f0 | m0));
       if (a.op == 1) {
-        st4<NT>(st + (f0), x1); st4<NT>(st + (f0 | m0), x0);
+        st4p<NT>(st + (f0), x1, tail); st4p<NT>(st + (f0 | m0), x0, tail);
       } else {
-        st4<NT>(st + (f0), cat4(crow2(g[0], lo2(x0), g[1], lo2(x1)), crow2(g[0], hi2(x0), g[1], hi2(x1))));
-        st4<NT>(st + (f0 | m0), cat4(crow2(g[2], lo2(x0), g[3], lo2(x1)), crow2(g[2], hi2(x0), g[3], hi2(x1))));
+        st4p<NT>(st + (f0), cat4(crow2(g[0], lo2(x0), g[1], lo2(x1)), crow2(g[0], hi2(x0), g[1], hi2(x1))), tail);
+        st4p<NT>(st + (f0 | m0), cat4(crow2(g[2], lo2(x0), g[3], lo2(x1)), crow2(g[2], hi2(x0), g[3], hi2(x1))), tail);
       }
     } else if (a.op == 3) {
       // SWAP moves only the (1, 0) and (0, 1) amplitudes
       const float4 x1 = ld4<NT>(st + (f0 | m0)), x2 = ld4<NT>(st + (f0 | m1));
-      st4<NT>(st + (f0 | m0), x2); st4<NT>(st + (f0 | m1), x1);
+      st4p<NT>(st + (f0 | m0), x2, tail); st4p<NT>(st + (f0 | m1), x1, tail);
     } else {
       float4 x0 = ld4<NT>(st + (f0)), x1 = ld4<NT>(st + (f0 | m0)), x2 = ld4<NT>(st + (f0 | m1)), x3 = ld4<NT>(st + (f0 | m0 | m1));
       {
@@ -654,8 +661,8 @@ __global__ __launch_bounds__(kThreads) void k_mc4(float4* __restrict__ st, I n_i
         float2 h0 = hi2(x0), h1 = hi2(x1), h2 = hi2(x2), h3 = hi2(x3);
         mat4(g, l0, l1, l2, l3);
         mat4(g, h0, h1, h2, h3);
-        st4<NT>(st + (f0), cat4(l0, h0)); st4<NT>(st + (f0 | m0), cat4(l1, h1));
-        st4<NT>(st + (f0 | m1), cat4(l2, h2)); st4<NT>(st + (f0 | m0 | m1), cat4(l3, h3));
+        st4p<NT>(st + (f0), cat4(l0, h0), tail); st4p<NT>(st + (f0 | m0), cat4(l1, h1), tail);
+        st4p<NT>(st + (f0 | m1), cat4(l2, h2), tail); st4p<NT>(st + (f0 | m0 | m1), cat4(l3, h3), tail);
       }
     }
   }
@@ -698,10 +705,11 @@ __device__ __forceinline__ float4 pick4(bool c, float4 a, float4 b) { return c ?
 // OP 0: dense 2x2 (compensated), OP 1: X.
 template <typename I, int MODE, int OP, int FULL, bool NT>
 __global__ __launch_bounds__(kThreads) void k_mc1(float4* __restrict__ st, I n_items, Mc1Args a,
-                                                  const float2* __restrict__ gdev, M2 gv) {
+                                                  const float2* __restrict__ gdev, M2 gv, PassOrder po) {
+  bool tail = false;
   float2 g[4];
   if (OP == 0) load_m2(gdev, gv, g);
-  const I item = xcd_block<I>() * (kThreads / 64) + (threadIdx.x >> 6);
+  const I item = pass_blk<I, NT, true>(po, tail) * (kThreads / 64) + (threadIdx.x >> 6);
   if (item >= n_items) return;      // uniform per wave
   const int lane = threadIdx.x & 63;
   const I w = mc1_window(item, a);
@@ -713,7 +721,7 @@ __global__ __launch_bounds__(kThreads) void k_mc1(float4* __restrict__ st, I n_i
     float4 r;
     if (OP == 1) r = cat4(hi2(v), lo2(v));
     else r = cat4(crow2(g[0], lo2(v), g[1], hi2(v)), crow2(g[2], lo2(v), g[3], hi2(v)));
-    st4<NT>(st + s0, FULL ? pick4(act, r, v) : r);
+    st4p<NT>(st + s0, FULL ? pick4(act, r, v) : r, tail);
   } else if (MODE == 1) {
     // the partner lane has the same control bits, so it is active with this one
     const bool ld = FULL || act;
@@ -730,7 +738,7 @@ __global__ __launch_bounds__(kThreads) void k_mc1(float4* __restrict__ st, I n_i
       r = cat4(crow2(ga, lo2(v), gb, lo2(p)), crow2(ga, hi2(v), gb, hi2(p)));
     }
     if (a.c0) r = cat4(lo2(v), hi2(r));
-    st4<NT>(st + s0, FULL ? pick4(act, r, v) : r);
+    st4p<NT>(st + s0, FULL ? pick4(act, r, v) : r, tail);
   } else {
     const I s1 = s0 + (I(1) << (a.t - 1));
     const float4 v0 = ld4<NT>(st + s0), v1 = ld4<NT>(st + s1);
@@ -742,8 +750,8 @@ __global__ __launch_bounds__(kThreads) void k_mc1(float4* __restrict__ st, I n_i
       r1 = cat4(crow2(g[2], lo2(v0), g[3], lo2(v1)), crow2(g[2], hi2(v0), g[3], hi2(v1)));
     }
     if (a.c0) { r0 = cat4(lo2(v0), hi2(r0)); r1 = cat4(lo2(v1), hi2(r1)); }
-    st4<NT>(st + s0, FULL ? pick4(act, r0, v0) : r0);
-    st4<NT>(st + s1, FULL ? pick4(act, r1, v1) : r1);
+    st4p<NT>(st + s0, FULL ? pick4(act, r0, v0) : r0, tail);
+    st4p<NT>(st + s1, FULL ? pick4(act, r1, v1) : r1, tail);
   }
 }
 
@@ -751,10 +759,12 @@ template <typename I, int MODE, int OP, bool NT>
 static void launch_mc1(hipStream_t s, void* state, uint64_t items, const Mc1Args& a, bool full,
                        const float2* gdev, const M2& gv) {
   const dim3 grid = grid1d(ceil_div(items, kThreads / 64)), block(kThreads);
+  // bytes a pass touches: a 1 KiB window per item, two in mode 2
+  const PassOrder po = NT ? next_pass(items * (MODE == 2 ? 2048 : 1024)) : PassOrder{0, 0};
   if (full)
-    hipLaunchKernelGGL((k_mc1<I, MODE, OP, 1, NT>), grid, block, 0, s, (float4*)state, (I)items, a, gdev, gv);
+    hipLaunchKernelGGL((k_mc1<I, MODE, OP, 1, NT>), grid, block, 0, s, (float4*)state, (I)items, a, gdev, gv, po);
   else
-    hipLaunchKernelGGL((k_mc1<I, MODE, OP, 0, NT>), grid, block, 0, s, (float4*)state, (I)items, a, gdev, gv);
+    hipLaunchKernelGGL((k_mc1<I, MODE, OP, 0, NT>), grid, block, 0, s, (float4*)state, (I)items, a, gdev, gv, po);
 }
 
 template <typename I, int OP, bool NT>
@@ -838,18 +848,19 @@ static int mc_impl(void* stream, void* state, int n, const float2* gdev, const i
     a.t1 = op >= 2 ? a.t1 - 1 : 0;
     const uint64_t blocks = ceil_div(items, (uint64_t)kThreads * kUm);
     const bool nt = a.pos[0] >= 3 && nt_for_qubits(n);
+    const PassOrder po = nt ? next_pass(items * (op >= 2 ? 64 : 32)) : PassOrder{0, 0};
     if (!wide && nt)
       hipLaunchKernelGGL((k_mc4<uint32_t, kUm, true>), grid1d(blocks), dim3(kThreads), 0, s,
-                         (float4*)state, (uint32_t)items, a, gdev);
+                         (float4*)state, (uint32_t)items, a, gdev, po);
     else if (!wide)
       hipLaunchKernelGGL((k_mc4<uint32_t, kUm, false>), grid1d(blocks), dim3(kThreads), 0, s,
-                         (float4*)state, (uint32_t)items, a, gdev);
+                         (float4*)state, (uint32_t)items, a, gdev, po);
     else if (nt)
       hipLaunchKernelGGL((k_mc4<uint64_t, kUm, true>), grid1d(blocks), dim3(kThreads), 0, s,
-                         (float4*)state, items, a, gdev);
+                         (float4*)state, items, a, gdev, po);
     else
       hipLaunchKernelGGL((k_mc4<uint64_t, kUm, false>), grid1d(blocks), dim3(kThreads), 0, s,
-                         (float4*)state, items, a, gdev);
+                         (float4*)state, items, a, gdev, po);
   } else {
     const uint64_t items = 1ull << (n - spec.n_pos);
     const uint64_t blocks = ceil_div(items, (uint64_t)kThreads * kUm);

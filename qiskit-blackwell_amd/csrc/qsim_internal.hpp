@@ -177,6 +177,35 @@ __device__ __forceinline__ void st4(float4* p, float4 v) {
   else *p = v;
 }
 
+// Cross-lane exchange of a float4 with lane ^ m (m < 64).
+__device__ __forceinline__ float4 shfl_xor4(float4 v, int m) {
+  float4 r;
+  r.x = __shfl_xor(v.x, m, 64);
+  r.y = __shfl_xor(v.y, m, 64);
+  r.z = __shfl_xor(v.z, m, 64);
+  r.w = __shfl_xor(v.w, m, 64);
+  return r;
+}
+
+// Flat block index of a 1D grid laid out as grid1d() does (gridDim.y rows of
+// gridDim.x blocks): an AMD dispatch holds at most 2^32 - 1 work-items per
+// dimension, so launches over states of more than 2^31 slots use rows.
+template <typename I>
+__device__ __forceinline__ I gblock() {
+  return (I)blockIdx.y * (I)gridDim.x + (I)blockIdx.x;
+}
+
+// XCD-aware block order.  Workgroups are dispatched round-robin over the 8
+// XCDs (block b -> XCD b % 8); renumbering so that XCD x walks the x-th
+// eighth of the grid keeps each XCD's L2 and DRAM pages on one contiguous
+// region instead of 4 KiB stripes shared by all eight.
+template <typename I>
+__device__ __forceinline__ I xcd_block() {
+  const I nb = (I)gridDim.x * (I)gridDim.y, b = gblock<I>();
+  if (nb & 7) return b;
+  return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
 // Pass order across consecutive launches of streaming kernels.  A pass over
 // a state larger than the MALL ends with its last-processed part written
 // through the cache (cached stores), and the next pass runs in the opposite
@@ -210,6 +239,13 @@ __device__ __forceinline__ I pass_block(const PassOrder& po, bool& tail) {
   tail = (uint64_t)pos * 1024 >= (uint64_t)len * (uint64_t)(1024 - po.tail1024);
   return po.rev ? nb - 1 - bx : bx;
 }
+// Block index of a streaming kernel: under the pass order when NT (states
+// larger than the MALL), else the plain numbering (XCD-major when XCD).
+template <typename I, bool NT, bool XCD>
+__device__ __forceinline__ I pass_blk(const PassOrder& po, bool& tail) {
+  if constexpr (NT) return pass_block<I, XCD>(po, tail);
+  else return XCD ? xcd_block<I>() : gblock<I>();
+}
 // Store of a streaming kernel: nontemporal, except in the pass's tail.
 template <bool NT>
 __device__ __forceinline__ void st4p(float4* p, float4 v, bool tail) {
@@ -221,34 +257,6 @@ __device__ __forceinline__ void st4p(float4* p, float4 v, bool tail) {
   }
 }
 
-// Cross-lane exchange of a float4 with lane ^ m (m < 64).
-__device__ __forceinline__ float4 shfl_xor4(float4 v, int m) {
-  float4 r;
-  r.x = __shfl_xor(v.x, m, 64);
-  r.y = __shfl_xor(v.y, m, 64);
-  r.z = __shfl_xor(v.z, m, 64);
-  r.w = __shfl_xor(v.w, m, 64);
-  return r;
-}
-
-// Flat block index of a 1D grid laid out as grid1d() does (gridDim.y rows of
-// gridDim.x blocks): an AMD dispatch holds at most 2^32 - 1 work-items per
-// dimension, so launches over states of more than 2^31 slots use rows.
-template <typename I>
-__device__ __forceinline__ I gblock() {
-  return (I)blockIdx.y * (I)gridDim.x + (I)blockIdx.x;
-}
-
-// XCD-aware block order.  Workgroups are dispatched round-robin over the 8
-// XCDs (block b -> XCD b % 8); renumbering so that XCD x walks the x-th
-// eighth of the grid keeps each XCD's L2 and DRAM pages on one contiguous
-// region instead of 4 KiB stripes shared by all eight.
-template <typename I>
-__device__ __forceinline__ I xcd_block() {
-  const I nb = (I)gridDim.x * (I)gridDim.y, b = gblock<I>();
-  if (nb & 7) return b;
-  return (b & 7) * (nb >> 3) + (b >> 3);
-}
 
 // Uniform k of a seeded SplitMix64 stream (Steele, Lea and Flood, "Fast
 // splittable pseudorandom number generators", OOPSLA 2014): the 64-bit state
