@@ -618,7 +618,11 @@ def main():
                      "traffic": GATE2Q_TRAFFIC_B_PER_AMP * (1 << n),
                      "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per k_gate2q dispatch, "
                                        "scaled by a 2 GiB device copy (--pmc-calibration)",
-                     "kernel": "k_gate2q (dense 4x4, 16 B/amp-update)"},
+                     "kernel": "k_gate2q (dense 4x4, 16 B/amp-update)",
+                     "note": "achieved counts algorithmic bytes; consecutive launches alternate direction and "
+                             "leave their last MALL-sized part (256 MiB) in the Infinity Cache, which serves "
+                             "the start of the next pass; the counters above still see one read and one "
+                             "write of the state per launch"},
         "fused": {"value": fused_value, "unit": "gate-equivalent amp-updates/s",
                   "ms_per_step": ftotal / args.steps * 1e3, "blocks": flaunch,
                   "speedup_vs_plain": fused_value / value,
