@@ -259,8 +259,12 @@ def test_reductions_and_measure(bk, dev):
         pr = bk.measure_probs(st).cpu().numpy()
         assert np.allclose(pr, np.abs(psi.astype(np.complex128)) ** 2, rtol=1e-6, atol=0)
         bk.renormalize(st)
-        got = st.cpu().numpy()
-        assert abs(np.sum(np.abs(got.astype(np.complex128)) ** 2) - 1) < 1e-6
+        got = st.cpu().numpy().astype(np.complex128)
+        assert abs(np.sum(np.abs(got) ** 2) - 1) < 1e-6
+        # the output is psi / ||psi|| itself, not just some unit vector:
+        # element-wise against the FP64 scaling, relative to the largest amplitude
+        want = psi.astype(np.complex128) / np.sqrt(ref)
+        assert np.max(np.abs(got - want)) <= 1e-6 * np.max(np.abs(want)), n
         for t in range(n):
             for u in (0.1, 0.77):
                 s2 = to_dev(psi, dev)

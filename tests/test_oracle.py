@@ -159,6 +159,73 @@ def test_diagonals_bruteforce():
     assert np.allclose(O.apply_diag_phase(psi, d[1], 2), ref)
 
 
+def test_apply_diag_bruteforce_every_target():
+    # reference rule (bwk/csrc/apply_diagonal_sm120.cu:43-59): each amplitude
+    # is multiplied by diag[bit(idx, target)]; distinct d0, d1 catch a swap
+    d = np.array([np.exp(0.4j) * 0.9, np.exp(-2.1j) * 1.1])
+    for n in range(1, 6):
+        psi = rand_psi(n, 40 + n)
+        for t in range(n):
+            ref = np.empty_like(psi)
+            for i in range(psi.size):
+                ref[i] = psi[i] * (d[1] if (i >> t) & 1 else d[0])
+            assert np.allclose(O.apply_diag(psi, d, t), ref, rtol=1e-15, atol=0), (n, t)
+            # a diagonal 2x2 through the generic matrix routine gives the same
+            assert np.allclose(O.apply_diag(psi, d, t), O.apply_1q(psi, np.diag(d), t), atol=1e-15)
+
+
+def _brute_mc2q(psi, g, controls, t0, t1):
+    """The reference's controlled 4x4 (bwk/csrc/apply_mc2qgate_sm120.cu:88-123):
+    groups enumerated with every affected bit zero, controls then set;
+    a00, a01 (bit lo), a10 (bit hi), a11, row r = 2*bit(hi) + bit(lo)."""
+    lo, hi = min(t0, t1), max(t0, t1)
+    cm = sum(1 << c for c in controls)
+    aff = cm | (1 << lo) | (1 << hi)
+    out = psi.copy()
+    for base in range(psi.size):
+        if base & aff:
+            continue
+        i00 = base | cm
+        idx = [i00, i00 | 1 << lo, i00 | 1 << hi, i00 | 1 << lo | 1 << hi]
+        a = [psi[k] for k in idx]
+        for r in range(4):
+            out[idx[r]] = g[r][0] * a[0] + g[r][1] * a[1] + g[r][2] * a[2] + g[r][3] * a[3]
+    return out
+
+
+def test_apply_mc2q_kernel_bruteforce():
+    rng = np.random.default_rng(12)
+    for n in (3, 4, 5):
+        for t0, t1 in itertools.permutations(range(n), 2):
+            rest = [q for q in range(n) if q not in (t0, t1)]
+            for k in range(0, min(3, len(rest)) + 1):
+                for ctl in itertools.combinations(rest, k):
+                    psi = rand_psi(n, 7 * t0 + t1 + 100 * k)
+                    g = rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4))
+                    got = O.apply_mc2q_kernel(psi, g, list(ctl), t0, t1)
+                    assert np.allclose(got, _brute_mc2q(psi, g, ctl, t0, t1), atol=1e-13), \
+                        (n, t0, t1, ctl)
+    # the argument order of the targets does not matter (row bit 0 = lower target)
+    psi = rand_psi(5, 3)
+    g = rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4))
+    assert np.array_equal(O.apply_mc2q_kernel(psi, g, [4], 1, 3), O.apply_mc2q_kernel(psi, g, [4], 3, 1))
+    # a controlled SWAP matrix reproduces the Fredkin permutation
+    sw = np.eye(4)[[0, 2, 1, 3]]
+    assert np.allclose(O.apply_mc2q_kernel(psi, sw, [0, 4], 1, 3), psi[O.mcswap_perm(5, [0, 4], 1, 3)])
+
+
+def test_probs_bruteforce():
+    # reference rule (bwk/csrc/measure_sm120.cu:115-130): p_i = re^2 + im^2
+    for n in (1, 4, 7):
+        psi = rand_psi(n, 60 + n) * 1.3
+        p = O.probs(psi)
+        ref = np.array([(psi[i] * np.conj(psi[i])).real for i in range(psi.size)])
+        assert np.allclose(p, ref, rtol=1e-15, atol=0)
+        assert abs(p.sum() - 1.69) < 1e-12          # Parseval: the norm squared
+    assert np.allclose(O.probs(np.array([0.6, 0.8j, -0.6 + 0.8j, 0])), [0.36, 0.64, 1.0, 0.0],
+                       rtol=1e-15, atol=0)
+
+
 # ------------------------------------------------------------------ closed forms
 def test_ghz_uniform_bell():
     for n in (2, 5, 9):
