@@ -18,6 +18,11 @@ TEST INFRASTRUCTURE ONLY (tests/, smoke() and bench.py's cpu_baseline leg).
 * chebyshev_evolve_fp64: the three-term recurrence of chebyshev.py:111-156
   in complex128.
 * expm_reference: scipy.sparse.linalg.expm_multiply, the independent check.
+* pauli_xz_groups / matvec_c / chebyshev_evolve_c: the same recurrence in C
+  with OpenMP (oracle/cheb_fp64.c), with the matrix elements computed from
+  the Pauli strings on the fly; fast enough for the 22-qubit chain with
+  several right-hand sides.  Pinned against pauli_dense (matvec) and
+  expm_multiply (evolution) in tests/test_oracle.py.
 """
 import numpy as np
 import scipy.sparse as sp
@@ -136,3 +141,67 @@ def chebyshev_evolve_fp64(h, psi, t, spectral_bound, precision=1e-10):
 def expm_reference(h, psi, t):
     from scipy.sparse.linalg import expm_multiply
     return expm_multiply(-1j * t * sp.csr_matrix(h), np.asarray(psi, dtype=np.complex128))
+
+
+def pauli_xz_groups(labels, coeffs):
+    """Terms as (x mask, z mask, coefficient) grouped by x mask, in Qiskit
+    label order (rightmost character = qubit 0; Y sets both masks).
+    Returns (xm uint64 [ng], toff int32 [ng+1], zm uint64 [nt],
+    coef float64 [nt, 2])."""
+    groups = {}
+    for lab, c in zip(labels, coeffs):
+        x = z = 0
+        for pos, ch in enumerate(reversed(lab)):
+            if ch in "XY":
+                x |= 1 << pos
+            if ch in "ZY":
+                z |= 1 << pos
+        groups.setdefault(x, []).append((z, complex(c)))
+    xs = sorted(groups)
+    toff = [0]
+    zm, cf = [], []
+    for x in xs:
+        for z, c in groups[x]:
+            zm.append(z)
+            cf.append((c.real, c.imag))
+        toff.append(len(zm))
+    return (np.array(xs, dtype=np.uint64), np.array(toff, dtype=np.int32),
+            np.array(zm, dtype=np.uint64), np.array(cf, dtype=np.float64).reshape(-1, 2))
+
+
+def _rhs_block(psi_rows):
+    """(rows of vectors [R, N]) -> RHS-contiguous complex128 [N, R]."""
+    v = np.atleast_2d(np.asarray(psi_rows, dtype=np.complex128))
+    return np.ascontiguousarray(v.T), v.shape[0]
+
+
+def matvec_c(labels, coeffs, n, psi_rows, scale=1.0, threads=0):
+    """scale * H v for each row v of psi_rows, by oracle/cheb_fp64.c."""
+    from oracle import build_ref
+    lib = build_ref.load()
+    lib.ph_set_threads(int(threads))
+    xm, toff, zm, cf = pauli_xz_groups(labels, coeffs)
+    v, r = _rhs_block(psi_rows)
+    y = np.empty_like(v)
+    lib.ph_matvec(n, len(xm), xm.ctypes.data, toff.ctypes.data, zm.ctypes.data, cf.ctypes.data, r,
+                  v.ctypes.data, y.ctypes.data, float(scale))
+    return np.ascontiguousarray(y.T)
+
+
+def chebyshev_evolve_c(labels, coeffs, n, psi_rows, t, spectral_bound, precision=1e-13, threads=0):
+    """exp(-iHt) v for each row v of psi_rows: the Chebyshev recurrence in
+    complex128 on host cores (oracle/cheb_fp64.c).  Returns (rows, K)."""
+    from oracle import build_ref
+    lib = build_ref.load()
+    lib.ph_set_threads(int(threads))
+    c, K, _ = chebyshev_coefficients(t, spectral_bound, precision)
+    cc = np.ascontiguousarray(np.stack([c.real, c.imag], axis=1))
+    xm, toff, zm, cf = pauli_xz_groups(labels, coeffs)
+    v, r = _rhs_block(psi_rows)
+    out = np.empty_like(v)
+    rc = lib.ph_cheb_evolve(n, len(xm), xm.ctypes.data, toff.ctypes.data, zm.ctypes.data, cf.ctypes.data,
+                            r, 1.0 / float(spectral_bound), int(K), cc.ctypes.data, v.ctypes.data,
+                            out.ctypes.data)
+    if rc != 0:
+        raise MemoryError("chebyshev_evolve_c: work vectors could not be allocated")
+    return np.ascontiguousarray(out.T), K

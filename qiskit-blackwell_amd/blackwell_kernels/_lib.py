@@ -175,8 +175,14 @@ _ws = _Workspace()
 
 
 def workspace(kind, nbytes, device):
-    """Per-thread, per-device scratch tensor of at least nbytes (grown on demand)."""
-    key = (kind, str(device))
+    """Scratch tensor of at least nbytes for launches on torch's current
+    stream of `device`, one per (thread, kind, device, stream), grown on
+    demand.  A buffer is only ever used on the stream it was allocated on:
+    torch's caching allocator hands a freed block out again in that stream's
+    order, so replacing a buffer that queued kernels still read is safe, and
+    two streams never share scratch."""
+    stream = torch.cuda.current_stream(device)
+    key = (kind, str(device), stream.cuda_stream)
     buf = _ws.bufs.get(key)
     if buf is None or buf.numel() < nbytes:
         buf = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)

@@ -259,3 +259,99 @@ def test_hbm_streaming_paths_26q_match_oracle():
     assert np.array_equal(perm_in.cpu().numpy().view(np.uint32), want.view(np.uint32))
     del st, perm_in
     torch.cuda.empty_cache()
+
+
+def test_chebyshev_22q_32rhs_matches_fp64():
+    """The Chebyshev workload at its own size: the 22-qubit open chain
+    (ZZ + XX + YY + X, seeded couplings), 32 normalised complex-Gaussian
+    right-hand sides evolved together, Bessel tail < 1e-7.
+      * t = 1: the first 4 against the FP64 Chebyshev oracle on host cores
+        (oracle/cheb_fp64.c, matrix elements from the Pauli strings, tail
+        < 1e-13; pinned against expm_multiply in tests/test_oracle.py):
+        fidelity >= 1 - 1e-7 and max |d amp| <= 1e-5 max |a| per vector.
+      * t = 10 (K ~ 490, the longest recurrence): exp(-iH 5) applied twice
+        equals exp(-iH 10) (fidelity >= 1 - 1e-7), and norm and energy
+        <H> (FP64 host products on the GPU result) are conserved."""
+    from blackwell_backend import workloads as W
+    from blackwell_backend.chebyshev import chebyshev_evolve_batch
+    from blackwell_backend.hamiltonian import HamiltonianCSR
+    from oracle import hamiltonian as OH
+    from oracle import statevector as SV
+    n, R, seed = 22, 32, 5
+    dev = torch.device("cuda", 0)
+    op = W.heisenberg_chain(n, seed)
+    h = HamiltonianCSR.from_sparse_pauli_op(op, device=dev)
+    N = 1 << n
+    rng = np.random.default_rng(seed + 1)
+    X = (rng.standard_normal((R, N)) + 1j * rng.standard_normal((R, N))).astype(np.complex64)
+    X /= np.linalg.norm(X, axis=1, keepdims=True).astype(np.float32)
+    Xd = torch.from_numpy(X).to(dev)
+    print(f"[fullsize] chebyshev {n}q, {R} RHS, t = 1 on the GPU", flush=True)
+    Y1, info = chebyshev_evolve_batch(Xd, h, 1.0, 1e-7)
+    got = Y1[:4].cpu().numpy().astype(np.complex128)
+    print(f"[fullsize] K = {info['chebyshev_degree']}; FP64 oracle on 4 RHS", flush=True)
+    ref, K64 = OH.chebyshev_evolve_c(op.labels, op.coeffs, n, X[:4].astype(np.complex128), 1.0,
+                                     h.spectral_bound, precision=1e-13)
+    for k in range(4):
+        f = SV.fidelity(ref[k], got[k])
+        d = float(np.max(np.abs(ref[k] - got[k])))
+        print(f"[fullsize]   rhs {k}: fidelity 1 - {1 - f:.3e}, max|d amp| {d:.3e} "
+              f"(max|a| {np.max(np.abs(ref[k])):.3e})", flush=True)
+        assert f >= 1 - 1e-7, (k, f)
+        assert d <= 1e-5 * np.max(np.abs(ref[k])), (k, d)
+    del Y1
+    print("[fullsize] t = 5, 5 + 5 and 10 on the GPU", flush=True)
+    Y10, info10 = chebyshev_evolve_batch(Xd, h, 10.0, 1e-7)
+    X4 = Xd[:4].contiguous()
+    Y5, _ = chebyshev_evolve_batch(X4, h, 5.0, 1e-7)
+    Y55, _ = chebyshev_evolve_batch(Y5, h, 5.0, 1e-7)
+    a10 = Y10[:4].cpu().numpy().astype(np.complex128)
+    a55 = Y55.cpu().numpy().astype(np.complex128)
+    nrm = np.linalg.norm(Y10.cpu().numpy().astype(np.complex128), axis=1)
+    e0 = np.real(np.sum(np.conj(X[:4].astype(np.complex128)) *
+                        OH.matvec_c(op.labels, op.coeffs, n, X[:4].astype(np.complex128)), axis=1))
+    e10 = np.real(np.sum(np.conj(a10) * OH.matvec_c(op.labels, op.coeffs, n, a10), axis=1))
+    print(f"[fullsize] K(10) = {info10['chebyshev_degree']}; max norm dev {np.max(np.abs(nrm - 1)):.3e}; "
+          f"max energy drift {np.max(np.abs(e10 - e0)):.3e}", flush=True)
+    for k in range(4):
+        f = SV.fidelity(a55[k], a10[k])
+        print(f"[fullsize]   rhs {k}: exp(-iH5)^2 vs exp(-iH10) fidelity 1 - {1 - f:.3e}, "
+              f"max|d| {np.max(np.abs(a55[k] - a10[k])):.3e}", flush=True)
+        assert f >= 1 - 1e-7, (k, f)
+    assert np.max(np.abs(nrm - 1)) <= 1e-5
+    assert np.max(np.abs(e10 - e0)) <= 1e-5 * np.max(np.abs(e0) + 1)
+    del Y10, Y5, Y55, Xd
+    torch.cuda.empty_cache()
+
+
+def test_mfma_block_variants_26q_match_oracle():
+    """Above the MALL size (26 qubits, 512 MiB) each 5-qubit block kernel
+    takes its streaming form.  One block per dispatch case, applied in
+    sequence and compared with the FP64 oracle (row bit j <-> qubits[j])
+    after every block, so a transposed, conjugated or mis-ordered block is
+    seen directly rather than through a round trip:
+      (5, 9, 14, 20, 25) and the same set unsorted: 16-byte-lane kernel,
+          nontemporal, no block qubit in 0..3;
+      (2, 7, 11, 18, 24): 16-byte-lane kernel, lowest block qubit in 1..3;
+      (0, 6, 13, 19, 23): 16-byte-lane kernel with qubit 0 as row bit 0;
+      (1, 3, 4, 17, 22) and (22, 4, 1, 17, 3): LDS-staged kernel
+          (three block qubits among 0..4), nontemporal."""
+    import blackwell_kernels as bk
+    from oracle import statevector as SV
+    from helpers import rand_state, rand_unitary, close
+    n = 26
+    dev = torch.device("cuda", 0)
+    psi = rand_state(n, 2626)
+    st = torch.from_numpy(psi).to(dev)
+    ref = psi.astype(np.complex128)
+    blocks = [(5, 9, 14, 20, 25), (25, 5, 20, 9, 14), (2, 7, 11, 18, 24), (0, 6, 13, 19, 23),
+              (1, 3, 4, 17, 22), (22, 4, 1, 17, 3)]
+    for k, qs in enumerate(blocks):
+        print(f"[fullsize] MFMA block {qs} at {n} qubits vs the FP64 oracle", flush=True)
+        u = rand_unitary(32, 500 + k).astype(np.complex64)
+        bk.apply_block(st, torch.from_numpy(u).to(dev), qs)
+        ref = SV.apply_matrix(ref, u.astype(np.complex128), list(qs))
+        ok, err = close(ref, st.cpu().numpy(), rtol=8e-6 * (k + 1), atol=0)
+        assert ok, (qs, err)
+    del st
+    torch.cuda.empty_cache()

@@ -395,3 +395,38 @@ def test_pauli_csr_grouped_matches_kronecker():
     # exact zeros removed: XX+YY cancel on rows where the two bits agree
     assert len(ci) < 64 * (1 + 5 + 6)
     assert np.all(v != 0)
+
+
+def test_c_pauli_matvec_matches_kronecker():
+    # (P v)[i] from the strings (oracle/cheb_fp64.c) against explicit np.kron
+    # products of the 2x2 Paulis, with complex coefficients and every Pauli
+    rng = np.random.default_rng(21)
+    for n in (1, 3, 6):
+        labels = ["".join(rng.choice(list("IXYZ"), n)) for _ in range(14)]
+        coeffs = rng.uniform(-1, 1, 14) + 1j * rng.uniform(-1, 1, 14)
+        v = np.stack([rand_psi(n, 70 + k) for k in range(3)])
+        got = OH.matvec_c(labels, coeffs, n, v, scale=0.5)
+        want = 0.5 * (OH.pauli_dense(labels, coeffs) @ v.T).T
+        assert np.allclose(got, want, atol=1e-13), n
+    # the single-qubit Paulis in closed form
+    e0, e1 = np.array([1, 0], complex), np.array([0, 1], complex)
+    assert np.allclose(OH.matvec_c(["Y"], [1.0], 1, e0), [[0, 1j]])
+    assert np.allclose(OH.matvec_c(["Y"], [1.0], 1, e1), [[-1j, 0]])
+    assert np.allclose(OH.matvec_c(["Z"], [1.0], 1, e1), [[0, -1]])
+
+
+def test_c_chebyshev_matches_expm_multiply():
+    from blackwell_backend import workloads as W
+    for n, t in ((8, 1.0), (10, 5.0)):
+        op = W.heisenberg_chain(n, seed=3)
+        bound = float(np.sum(np.abs(op.coeffs)))
+        v = np.stack([rand_psi(n, 80 + k) for k in range(3)])
+        got, K = OH.chebyshev_evolve_c(op.labels, op.coeffs, n, v, t, bound)
+        h = OH.pauli_sparse(op.labels, op.coeffs)
+        for k in range(3):
+            want = OH.expm_reference(h, v[k], t)
+            assert np.max(np.abs(got[k] - want)) < 1e-11, (n, t, k)
+        assert K > bound * t
+    # H = Z on one qubit: exact phases
+    out, _ = OH.chebyshev_evolve_c(["Z"], [1.0], 1, np.array([0.6, 0.8j]), 1.3, 1.0)
+    assert np.allclose(out[0], [0.6 * cmath.exp(-1.3j), 0.8j * cmath.exp(1.3j)], atol=1e-12)
