@@ -1,19 +1,24 @@
 """Benchmark: amp-updates/s and %HBM peak of the statevector gate kernels on
 one MI355X.
 
-Workload (one step): the 28-qubit random-SU(4) volume circuit, depth 28
-(392 Haar 4x4 gates on seeded random pairs, complex64 state of 2 GiB), run
-from |0...0> through the plain per-gate kernels.  value = 392 * 2^28 /
-seconds per step.  The same circuit is also run as fused 5-qubit blocks on
-FP32 MFMA ("fused"), and both results are checked against an FP64 CPU
-statevector of the same circuit family at a size the CPU finishes quickly.
+Workload (one step): the 30-qubit random-SU(4) volume circuit, depth 30
+(450 Haar 4x4 gates on seeded random pairs, complex64 state of 8 GiB), run
+from |0...0> through the plain per-gate kernels.  value = 450 * 2^30 /
+seconds per step.  The same circuit also runs as fused 5-qubit blocks on
+FP32 MFMA ("fused"); both forms are checked against an FP64 CPU statevector
+of the same circuit family at 22 qubits ("parity").
 
 Extra figures in the same JSON line:
-  roofline      dominant kernel (dense 2Q gate), 16 B per amplitude update,
-                average launch duration from HIP events on the launch stream
-  per_gate_30q  each gate class on a 30-qubit state (8 GiB)
-  cpu_baseline  FP64 C/OpenMP statevector (oracle/sv_fp64.c) on host cores,
-                bounded sample of the same circuit
+  roofline        dominant kernel (dense 2Q gate), 16 B per amplitude update,
+                  average launch duration from HIP events on the launch stream
+  per_gate_30q    each gate class on a 30-qubit state (8 GiB)
+  c2_28q          the 28-qubit SU(4) circuit (depth 28), plain and fused
+  c3/c5/serving   the 32-qubit ansatz, the 22-qubit Chebyshev evolution and
+                  the mixed 10..24-qubit request stream
+  cpu_baseline    FP64 C/OpenMP statevector (oracle/sv_fp64.c) on the host
+                  cores, bounded sample of the same 30-qubit circuit
+  cpu_legs        the CPU path on the 12-qubit ladder circuit (whole circuit)
+                  and on the Chebyshev workload (scipy expm_multiply, FP64)
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
@@ -37,6 +42,9 @@ HBM_PEAK_GBS = 8000.0
 # by the counts of a 2 GiB device-to-device copy run first with
 # --pmc-calibration): one read and one write of the state, nothing more.
 GATE2Q_TRAFFIC_B_PER_AMP = 16.0
+GATE2Q_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, per k_gate2q dispatch of this "
+                         "script at 30 qubits, each scaled by the same counter on a 8 GiB device copy "
+                         "(--pmc-calibration)")
 MFMA_FP32_PEAK_TFLOPS = 157.3
 
 
@@ -446,10 +454,24 @@ def ab_suite_bench(shots=8192, warmup=3, repeats=5):
             "min_speedup": float(min(ratios)), "circuits": rows}
 
 
+def host_cores():
+    """(threads to use, description): the CPU share this process may use (its
+    affinity mask, capped by OMP_NUM_THREADS when set), with the CPU model and
+    the ISA level of the oracle build that runs on it."""
+    from oracle import build_ref
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(avail, omp) if omp > 0 else avail
+    return threads, {"cpu_model": build_ref.cpu_model(), "isa": build_ref.host_isa(),
+                     "affinity_cpus": avail, "os_cpu_count": os.cpu_count(),
+                     "omp_num_threads_env": omp or None}
+
+
 def cpu_leg(sim, n, depth, seed, time_it=True):
     """The CPU leg: the FP64 C/OpenMP statevector (oracle/sv_fp64.c) checks
     the GPU result of the same circuit family at 22 qubits ("parity") and,
-    with time_it, is timed on the host cores as the CPU baseline."""
+    with time_it, is timed on the host cores as the CPU baseline, beside the
+    CPU path on the 12-qubit ladder and the Chebyshev workload."""
     out = {}
     try:
         out["parity"] = fidelity_check(sim, 22, 22, seed, ("plain", "blocks"))
@@ -460,16 +482,25 @@ def cpu_leg(sim, n, depth, seed, time_it=True):
             out["cpu_baseline"] = cpu_baseline(n, depth, seed)
         except Exception as e:
             out["cpu_baseline"] = {"error": repr(e)}
+        out["cpu_legs"] = {}
+        for key, fn in (("c1_ladder_12q", lambda: cpu_c1(sim)), ("c5_chebyshev_22q", cpu_c5)):
+            try:
+                log(f"[bench] CPU leg {key}")
+                out["cpu_legs"][key] = fn()
+            except Exception as e:
+                out["cpu_legs"][key] = {"error": repr(e)}
     return out
 
 
 def cpu_baseline(n, depth, seed, budget_s=15.0):
-    """FP64 C/OpenMP statevector on host cores: amp-updates/s on a bounded
-    prefix of the same circuit (gates applied until budget_s elapses)."""
+    """FP64 C/OpenMP statevector on the host cores: amp-updates/s on a
+    bounded prefix of the same circuit (gates applied until budget_s
+    elapses).  Aer is not installed, so the FP64 C oracle stands in for the
+    reference's Aer CPU path."""
     from blackwell_backend import workloads as W
     from oracle import build_ref
     lib = build_ref.load()
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    threads, info = host_cores()
     threads = lib.sv_set_threads(threads)
     psi = np.empty(1 << n, dtype=np.complex128)
     lib.sv_init(psi.ctypes.data, n)
@@ -490,9 +521,141 @@ def cpu_baseline(n, depth, seed, budget_s=15.0):
     el = time.perf_counter() - t0
     del psi
     return {"value": done * (1 << n) / el, "unit": "amp-updates/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", **info,
             "sample": f"first {done} of {n * depth // 2} gates of the {n}-qubit depth-{depth} "
-                      f"SU(4) circuit, FP64 C/OpenMP statevector (oracle/sv_fp64.c), {el:.1f} s"}
+                      f"SU(4) circuit (FP64 state {(16 << n) >> 30} GiB), FP64 C/OpenMP statevector "
+                      f"(oracle/sv_fp64.c, built -march={lib.isa}), {el:.1f} s; Aer CPU is not installed"}
+
+
+def cpu_c1(sim, warmup=3, repeats=5, seed=1234):
+    """The 12-qubit rotation ladder (564 gates) on the CPU path, whole
+    circuit, median of 5 after 3 warmups (tests/ab_test.py:154-165), beside
+    the GPU's run_statevector of the same circuit."""
+    from blackwell_backend import workloads as W
+    from blackwell_backend.circuit import instructions
+    from oracle import build_ref
+    from oracle import statevector as O
+    lib = build_ref.load()
+    threads, info = host_cores()
+    n = 12
+    qc = W.c1_rotation_ladder(n, 12, seed)
+    prog = []
+    for name, qs, _, params in instructions(qc):
+        if name == "cx":
+            prog.append(("cx", (ctypes_int_array([qs[0]]), qs[1])))
+        else:
+            u = O.gate_1q(name, params)
+            v = np.empty(8)
+            v[0::2] = u.reshape(-1).real
+            v[1::2] = u.reshape(-1).imag
+            prog.append(("1q", (v, qs[0])))
+    psi = np.empty(1 << n, dtype=np.complex128)
+
+    def run_cpu():
+        lib.sv_init(psi.ctypes.data, n)
+        for kind, a in prog:
+            if kind == "cx":
+                lib.sv_apply_mcx(psi.ctypes.data, n, a[0], 1, a[1])
+            else:
+                lib.sv_apply_1q(psi.ctypes.data, n, a[0].ctypes.data_as(_pd()), a[1])
+
+    # a 12-qubit state is small for a thread team: time one thread and all
+    # of them, and keep the faster as the baseline
+    per_threads = {}
+    for nt in sorted({1, threads}):
+        lib.sv_set_threads(nt)
+        for _ in range(warmup):
+            run_cpu()
+        ts = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            run_cpu()
+            ts.append(time.perf_counter() - t0)
+        per_threads[nt] = float(np.median(ts))
+    lib.sv_set_threads(threads)
+    best = min(per_threads, key=per_threads.get)
+    ref = psi.copy()
+    for _ in range(warmup):
+        sim.run_statevector(qc)
+    gts = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        sv, _ = sim.run_statevector(qc)
+        gts.append(time.perf_counter() - t0)
+    return {"workload": f"{n}-qubit rotation ladder, 12 layers, {len(prog)} gates, from |0..0>",
+            "cpu_ms": round(per_threads[best] * 1e3, 3), "cpu_threads": best,
+            "cpu_ms_by_threads": {str(k): round(v * 1e3, 3) for k, v in per_threads.items()},
+            "cpu_kind": "FP64 C statevector (oracle/sv_fp64.c); Aer CPU is not installed",
+            "gpu_run_statevector_ms": round(float(np.median(gts)) * 1e3, 3),
+            "gpu_fidelity": O.fidelity(ref, sv), "gpu_max_abs_damp": O.max_abs_diff_phase_aligned(ref, sv),
+            **info}
+
+
+def ctypes_int_array(vals):
+    import ctypes
+    return (ctypes.c_int * len(vals))(*vals)
+
+
+def cpu_c5(n=22, seed=5, t=1.0, n_rhs=1):
+    """The Chebyshev workload's CPU path: scipy expm_multiply (FP64) of
+    exp(-iHt) on the first checked right-hand side, on the CSR the GPU
+    uses (copied to the host and widened to complex128), beside the GPU's
+    time per right-hand side at 32 RHS."""
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import expm_multiply
+    from blackwell_backend import workloads as W
+    from blackwell_backend.hamiltonian import HamiltonianCSR
+    from blackwell_backend.chebyshev import chebyshev_evolve_batch
+    from oracle import statevector as O
+    dev = torch.device("cuda", 0)
+    h = HamiltonianCSR.from_sparse_pauli_op(W.heisenberg_chain(n, seed), device=dev)
+    N = 1 << n
+    rng = np.random.default_rng(seed + 1)
+    X = (rng.standard_normal((32, N)) + 1j * rng.standard_normal((32, N))).astype(np.complex64)
+    X /= np.linalg.norm(X, axis=1, keepdims=True).astype(np.float32)
+    Xd = torch.from_numpy(X).to(dev)
+    chebyshev_evolve_batch(Xd[:2], h, 0.1, 1e-7)
+    Y, info = chebyshev_evolve_batch(Xd, h, t, 1e-7)
+    gpu = Y[:n_rhs].cpu().numpy()
+    A = sp.csr_matrix((h.values.cpu().numpy().astype(np.complex128), h.col_idx.cpu().numpy(),
+                       h.row_ptr.cpu().numpy()), shape=(N, N))
+    del Y, Xd
+    torch.cuda.empty_cache()
+    t0 = time.perf_counter()
+    ref = expm_multiply(-1j * t * A, X[:n_rhs].astype(np.complex128).T).T
+    cpu_s = time.perf_counter() - t0
+    fid = [O.fidelity(ref[k], gpu[k]) for k in range(n_rhs)]
+    return {"workload": f"exp(-iHt), {n}-qubit chain, t = {t:g}, nnz {h.nnz}",
+            "cpu_kind": "scipy.sparse.linalg.expm_multiply, FP64, one thread (scipy's sparse products)",
+            "cpu_s_per_rhs": round(cpu_s / n_rhs, 3), "cpu_rhs": n_rhs,
+            "gpu_s_per_rhs_at_32": round(info["elapsed_s"] / 32, 5),
+            "gpu_K": info["chebyshev_degree"],
+            "gpu_fidelity_vs_expm": fid,
+            "gpu_max_abs_damp_vs_expm": [float(np.max(np.abs(ref[k] - gpu[k]))) for k in range(n_rhs)]}
+
+
+def c2_bench(n=28, depth=28, seed=2024, steps=5, warmup=2):
+    """The 28-qubit random-SU(4) circuit (depth 28, 392 gates, 2 GiB):
+    plain per-gate kernels and fused 5-qubit MFMA blocks."""
+    from blackwell_backend import BlackwellSimulator, workloads as W
+    sim = BlackwellSimulator("cuda:0")
+    stream = torch.cuda.current_stream(sim.device)
+    qc = W.su4_volume(n, depth, seed)
+    gates = (n // 2) * depth
+    out = {"workload": f"random-SU(4) volume circuit, {n} qubits, depth {depth}, {gates} gates"}
+    for mode in ("plain", "blocks"):
+        st, total, wall, kern, launches = run_steps(sim, qc, mode, steps, warmup, stream)
+        del st
+        per = float(np.median(kern)) / launches
+        row = {"ms_per_step": round(total / steps * 1e3, 3),
+               "amp_updates_per_s": gates * float(1 << n) / (total / steps), "launches": launches,
+               "ms_per_launch": round(per * 1e3, 4),
+               "GB_per_s_per_launch": round(16.0 * (1 << n) / per / 1e9, 1)}
+        row["frac_hbm_peak_per_launch"] = round(row["GB_per_s_per_launch"] / HBM_PEAK_GBS, 4)
+        out[mode] = row
+    out["fused_speedup"] = out["plain"]["ms_per_step"] / out["blocks"]["ms_per_step"]
+    torch.cuda.empty_cache()
+    return out
 
 
 def serving_bench(n_requests=4096, seed=99, shots=1024):
@@ -518,6 +681,7 @@ def serving_bench(n_requests=4096, seed=99, shots=1024):
     rate = 1.2 * plain_cap["circuits_per_s"]
     plain_rt = S.serve_plain(reqs(rate), sim, realtime=True)
     served_rt = S.Server(sim).serve(reqs(rate), realtime=True)
+    fid = served_fidelity(sim, ns, seed, shots)
     return {
         "workload": f"{n_requests} requests, n = 10 + k, P(k) ~ 0.25*0.75^k (k <= 14), random-SU(4) "
                     f"depth n, {shots} shots each",
@@ -528,12 +692,50 @@ def serving_bench(n_requests=4096, seed=99, shots=1024):
         "poisson_rate_per_s": rate,
         "poisson_served": {k: served_rt[k] for k in ("circuits_per_s", "p50_latency_s", "p99_latency_s")},
         "poisson_plain": {k: plain_rt[k] for k in ("circuits_per_s", "p50_latency_s", "p99_latency_s")},
+        "fidelity": fid,
     }
+
+
+def served_fidelity(sim, ns, seed, shots, per_lane=16):
+    """Re-serve a seeded sample of the stream's requests, 16 from each lane,
+    as one mixed run with their final states copied back, and compare every
+    state with the FP64 C statevector of the same gates (outside the timed
+    runs)."""
+    from blackwell_backend import serving as S, workloads as W
+    from oracle import build_ref
+    from oracle import statevector as O
+    rng = np.random.default_rng(seed + 7)
+    lds = [i for i in range(len(ns)) if ns[i] <= S.LDS_MAX_QUBITS]
+    hbm = [i for i in range(len(ns)) if ns[i] > S.LDS_MAX_QUBITS]
+    pick = sorted(list(rng.choice(lds, min(per_lane, len(lds)), replace=False)) +
+                  list(rng.choice(hbm, min(per_lane, len(hbm)), replace=False)))
+    reqs = [S.su4_request(int(i), 0.0, int(ns[i]), shots, seed * 100003 + int(i), keep_state=True)
+            for i in pick]
+    S.Server(sim).serve(reqs, realtime=False)
+    lib = build_ref.load()
+    lib.sv_set_threads(host_cores()[0])
+    rows = []
+    for r in reqs:
+        n = r.n_qubits
+        pairs, us = W.su4_arrays(n, n, r.seed)
+        psi = np.empty(1 << n, dtype=np.complex128)
+        lib.sv_init(psi.ctypes.data, n)
+        for (a, b), u in zip(pairs, us):
+            v = np.empty(32)
+            v[0::2] = np.asarray(u).reshape(-1).real
+            v[1::2] = np.asarray(u).reshape(-1).imag
+            lib.sv_apply_2q(psi.ctypes.data, n, v.ctypes.data_as(_pd()), int(a), int(b))
+        rows.append((n, r.lane, O.fidelity(psi, r.state), float(np.max(np.abs(psi - r.state)))))
+    return {"requests": len(rows), "qubits": sorted({x[0] for x in rows}),
+            "lanes": {ln: sum(1 for x in rows if x[1] == ln) for ln in ("lds", "hbm")},
+            "min_fidelity": min(x[2] for x in rows), "max_abs_damp": max(x[3] for x in rows),
+            "reference": "FP64 C statevector (oracle/sv_fp64.c) of the same gates"}
 
 
 # extra sections: name -> (JSON key, function)
 SECTIONS = {
     "per_gate": ("per_gate_30q", lambda: per_gate_sweep(30)),
+    "c2": ("c2_28q", lambda: c2_bench()),
     "ab": ("ab_suite", lambda: ab_suite_bench()),
     "c3": ("c3_rotation_mcx_32q", lambda: c3_bench()),
     "c5": ("c5_chebyshev_22q", lambda: c5_bench()),
@@ -541,6 +743,7 @@ SECTIONS = {
     "observables": ("observables_30q", lambda: observables_sweep(30)),
     "stride": ("stride_sweep_30q", lambda: stride_sweep(30)),
 }
+DEFAULT_SECTIONS = ("per_gate", "observables", "c2", "ab", "c3", "c5", "serving")
 
 
 def main():
@@ -548,8 +751,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--qubits", type=int, default=28)
-    ap.add_argument("--no-extras", action="store_true", help="skip the 30q sweep and CPU baseline")
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the extra sections and the timed CPU legs")
+    ap.add_argument("--no-fused", action="store_true", help="skip the fused-block run of the headline circuit")
     ap.add_argument("--only", choices=sorted(SECTIONS),
                     help="run one extra section alone and print its JSON (for profiling it)")
     ap.add_argument("--pmc-calibration", action="store_true",
@@ -581,7 +786,8 @@ def main():
         del src, dst
         log(f"[bench] calibration copy of {(8 << n) >> 20} MiB done")
     log(f"[bench] compile + warmup plain ({n} qubits, {n_gates} gates)")
-    _, total, wall, kern, launches = run_steps(sim, qc, "plain", args.steps, args.warmup, stream)
+    st, total, wall, kern, launches = run_steps(sim, qc, "plain", args.steps, args.warmup, stream)
+    del st
     ms_step = total / args.steps * 1e3
     updates = n_gates * float(1 << n)
     value = updates / (total / args.steps)
@@ -589,16 +795,8 @@ def main():
     achieved = 16.0 * (1 << n) / per_launch_s / 1e9
     log(f"[bench] plain: {ms_step:.1f} ms/step, {value:.3e} amp-updates/s, "
         f"{achieved:.0f} GB/s per 2Q launch")
-    log("[bench] fused 5-qubit MFMA blocks")
-    _, ftotal, fwall, fkern, flaunch = run_steps(sim, qc, "blocks", args.steps, args.warmup, stream)
-    fused_value = updates / (ftotal / args.steps)
-    blk_s = float(np.median(fkern)) / flaunch
-    blk_bytes = 16.0 * (1 << n)
-    blk_flops = 256.0 * (1 << n)
-    log(f"[bench] fused: {ftotal / args.steps * 1e3:.1f} ms/step, {fused_value:.3e} gate-amp-updates/s,"
-        f" {flaunch} blocks")
     line = {
-        "metric": "amp-updates/s (plain per-gate kernels, 28q random-SU(4) depth 28)",
+        "metric": f"amp-updates/s at {n} qubits (plain per-gate kernels, random-SU(4) depth {depth})",
         "value": value,
         "unit": "amp-updates/s",
         "n_gpus": 1,
@@ -609,37 +807,48 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64 (fp32 compensated)",
-        "data": "synthetic (seeded Haar SU(4) gates, seed 2024)",
+        "data": f"synthetic (seeded Haar SU(4) gates, seed {seed})",
         "config": {"workload": f"random-SU(4) volume circuit, {n} qubits, depth {depth}, "
                                f"{n_gates} gates, complex64 state {(8 << n) >> 30} GiB, plain per-gate",
                    "qubits": n, "depth": depth, "gates": n_gates, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": GATE2Q_TRAFFIC_B_PER_AMP * (1 << n),
-                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per k_gate2q dispatch, "
-                                       "scaled by a 2 GiB device copy (--pmc-calibration)",
+                     "traffic_source": GATE2Q_TRAFFIC_SOURCE,
                      "kernel": "k_gate2q (dense 4x4, 16 B/amp-update)",
-                     "note": "achieved counts algorithmic bytes; consecutive launches alternate direction and "
-                             "leave their last MALL-sized part (256 MiB) in the Infinity Cache, which serves "
-                             "the start of the next pass; the counters above still see one read and one "
-                             "write of the state per launch"},
-        "fused": {"value": fused_value, "unit": "gate-equivalent amp-updates/s",
-                  "ms_per_step": ftotal / args.steps * 1e3, "blocks": flaunch,
-                  "speedup_vs_plain": fused_value / value,
-                  "block_GB_per_s": blk_bytes / blk_s / 1e9,
-                  "block_mfma_TFLOP_per_s": blk_flops / blk_s / 1e12,
-                  "frac_hbm": blk_bytes / blk_s / 1e9 / HBM_PEAK_GBS,
-                  "frac_mfma_fp32": blk_flops / blk_s / 1e12 / MFMA_FP32_PEAK_TFLOPS},
+                     "note": "achieved = 16 B x 2^n algorithmic bytes / median kernel-event time per "
+                             "launch over the timed steps; the FETCH_SIZE / WRITE_SIZE counters see one "
+                             "read and one write of the state per launch"},
     }
+    if not args.no_fused:
+        log("[bench] fused 5-qubit MFMA blocks")
+        st, ftotal, fwall, fkern, flaunch = run_steps(sim, qc, "blocks", args.steps, args.warmup, stream)
+        del st
+        fused_value = updates / (ftotal / args.steps)
+        blk_s = float(np.median(fkern)) / flaunch
+        blk_bytes = 16.0 * (1 << n)
+        blk_flops = 256.0 * (1 << n)
+        log(f"[bench] fused: {ftotal / args.steps * 1e3:.1f} ms/step, {fused_value:.3e} gate-amp-updates/s,"
+            f" {flaunch} blocks")
+        line["fused"] = {"value": fused_value, "unit": "gate-equivalent amp-updates/s",
+                         "ms_per_step": ftotal / args.steps * 1e3, "blocks": flaunch,
+                         "block_qubits": 5,
+                         "speedup_vs_plain": fused_value / value,
+                         "block_GB_per_s": blk_bytes / blk_s / 1e9,
+                         "block_mfma_TFLOP_per_s": blk_flops / blk_s / 1e12,
+                         "frac_hbm": blk_bytes / blk_s / 1e9 / HBM_PEAK_GBS,
+                         "frac_mfma_fp32": blk_flops / blk_s / 1e12 / MFMA_FP32_PEAK_TFLOPS}
+    torch.cuda.empty_cache()
     if not args.no_extras:
-        for name in ("per_gate", "observables", "ab", "c3", "c5", "serving"):
+        for name in DEFAULT_SECTIONS:
             key, fn = SECTIONS[name]
             try:
                 log(f"[bench] {key}")
                 line[key] = fn()
             except Exception as e:  # reported, not hidden
                 line[key] = {"error": repr(e)}
-    log("[bench] CPU leg: FP64 C/OpenMP statevector (parity at 22 qubits, then the timed baseline)")
+            torch.cuda.empty_cache()
+    log("[bench] CPU leg: FP64 C/OpenMP statevector (parity at 22 qubits, then the timed baselines)")
     line.update(cpu_leg(sim, n, depth, seed, time_it=not args.no_extras))
     print(json.dumps(line), flush=True)
 

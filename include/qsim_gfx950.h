@@ -315,12 +315,18 @@ typedef struct qsim_instance {
  * one workgroup per instance, state resident in LDS for the whole program.
  * Sample k of an instance uses u_k = SplitMix64(seed + (k+1)*0x9E3779B97F4A7C15)
  * >> 11 scaled by 2^-53, and returns the smallest index i with
- * cdf(i) > u_k * ||psi||^2 (FP64).  Generalises qv8_simulate
- * (bwk/csrc/qv8_sim_sm120.cu:121-151) and qv4_simulate
- * (bwk/csrc/qv4_sim_sm120.cu:139-169) to per-instance n and programs.    */
-int qsim_run_instances(void* stream, void* states_base, const void* inst_dev, int n_inst,
-                       const void* ops_dev, const void* coefs_dev, void* shots_dev,
-                       int max_qubits);
+ * cdf(i) > u_k * ||psi||^2 (FP64).  inst_host / ops_host must hold the bytes
+ * of the device copies; they are validated (n_qubits <= max_qubits, op,
+ * state, shot and coefficient ranges against n_ops_total, n_state_slots
+ * complex64 slots, n_shot_slots int64 slots and n_coefs) before anything
+ * is launched.  Generalises qv8_simulate (bwk/csrc/qv8_sim_sm120.cu:121-151)
+ * and qv4_simulate (bwk/csrc/qv4_sim_sm120.cu:139-169) to per-instance n
+ * and programs.                                                          */
+int qsim_run_instances(void* stream, void* states_base, uint64_t n_state_slots,
+                       const qsim_instance* inst_host, const void* inst_dev, int n_inst,
+                       const qsim_op* ops_host, const void* ops_dev, uint64_t n_ops_total,
+                       const void* coefs_dev, uint64_t n_coefs, void* shots_dev,
+                       uint64_t n_shot_slots, int max_qubits);
 
 /* One (instance, block) of a batched block step (new): block_rec.state is the
  * device address of a complex64 state of n_qubits (10..32), .mat the device
@@ -388,15 +394,22 @@ int qsim_chebyshev_step_accum(void* stream, const void* s, const void* t_prev, v
                               void* accum, int64_t n, double c_re, double c_im);
 /* Batched recurrence over n_rhs right-hand sides stored RHS-contiguous
  * (element (row, j) at row*n_rhs + j).  One launch does
- *   w   = scale * A * t_cur                     (SpMM, compensated)
+ *   w   = scale * A * t_cur                     (SpMM, per-row sums below)
  *   t_nx = first ? w : 2*w - t_prev              (written over t_prev)
  *   acc += c * t_nx                              (Kahan: acc + comp)
- * first != 0 computes T1 = scale*A*T0 (t_prev ignored).  (new; the
- * reference runs one vector: blackwell_backend/chebyshev.py:141-156)     */
+ * first != 0 computes T1 = scale*A*T0 (t_prev ignored).  Row sums:
+ *   QSIM_CHEB_ACC_FP64  every FP32 x FP32 product (exact in FP64) summed
+ *                       with FP64 FMAs, rounded once to complex64;
+ *   QSIM_CHEB_ACC_FP32C TwoProduct + Neumaier compensated FP32 (the
+ *                       reference's TwoProduct/Kahan scheme,
+ *                       bwk/csrc/spmv_sm120.cu:26-54).
+ * (new; the reference runs one vector: blackwell_backend/chebyshev.py:141-156) */
+#define QSIM_CHEB_ACC_FP64  0
+#define QSIM_CHEB_ACC_FP32C 1
 int qsim_cheb_spmm_step(void* stream, const void* row_ptr, const void* col_idx, const void* values,
                         int64_t n_rows, int n_rhs, double scale,
                         const void* t_cur, void* t_prev_next, void* acc, void* comp,
-                        double c_re, double c_im, int first);
+                        double c_re, double c_im, int first, int accumulate);
 /* acc = c * t, comp = 0 (first Chebyshev term), n elements. (new)        */
 int qsim_cheb_init_acc(void* stream, void* acc, void* comp, const void* t, int64_t n,
                        double c_re, double c_im);

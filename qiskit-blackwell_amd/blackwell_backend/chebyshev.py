@@ -9,11 +9,18 @@ requested precision (the reference trims from a fixed estimate
 K_est = alpha + 2 ln(1/eps) + 20, which can sit only a few terms above the
 need at large alpha; here K_est only seeds the search).
 
-Device work per term is ONE launch (qsim_cheb_spmm_step): compensated
-SpMM of the scaled Hamiltonian against the current term, the recurrence
+Device work per term is ONE launch (qsim_cheb_spmm_step): SpMM of the
+scaled Hamiltonian against the current term, the recurrence
 T_{k+1} = 2 H~ T_k - T_{k-1} written over T_{k-1}, and the Kahan/Neumaier
 accumulation acc += c_k T_{k+1} with its compensation vector.  Any number
 of right-hand sides runs in the same launch, stored RHS-contiguous.
+
+Row sums of the SpMM (accumulate=):
+  "fp64"   every FP32 x FP32 product is exact in FP64 and is summed with
+           FP64 FMAs (default: the more accurate form, and the cheaper one on
+           CDNA4, where FP64 FMA issues at the FP32 rate);
+  "fp32c"  TwoProduct + Neumaier compensated FP32, the reference's
+           TwoProduct/Kahan row sum (bwk/csrc/spmv_sm120.cu:26-54).
 """
 import math
 import time
@@ -53,9 +60,15 @@ def chebyshev_coefficients(t, spectral_bound, precision=1e-10):
     return coeffs, K, tail
 
 
-def chebyshev_evolve_batch(states, hamiltonian_csr, t, precision=1e-10):
+ACCUMULATE = {"fp64": 0, "fp32c": 1}
+
+
+def chebyshev_evolve_batch(states, hamiltonian_csr, t, precision=1e-10, accumulate="fp64"):
     """Evolve R states at once.  states: complex64 [R, 2^n] (or [2^n]) on the
     device.  Returns (evolved [R, 2^n] complex64, info)."""
+    if accumulate not in ACCUMULATE:
+        raise ValueError(f"accumulate must be one of {sorted(ACCUMULATE)}, got {accumulate!r}")
+    mode = ACCUMULATE[accumulate]
     single = states.dim() == 1
     X = states.reshape(1, -1) if single else states
     R, N = X.shape
@@ -78,12 +91,13 @@ def chebyshev_evolve_batch(states, hamiltonian_csr, t, precision=1e-10):
     call("qsim_cheb_init_acc", s, ptr(acc), ptr(comp), ptr(A), n_el, c0.real, c0.imag)
     args = (ptr(h.row_ptr), ptr(h.col_idx), ptr(h.values), N, R, scale)
     c1 = complex(coeffs[1])
-    call("qsim_cheb_spmm_step", s, *args, ptr(A), ptr(B), ptr(acc), ptr(comp), c1.real, c1.imag, 1)
+    call("qsim_cheb_spmm_step", s, *args, ptr(A), ptr(B), ptr(acc), ptr(comp), c1.real, c1.imag, 1,
+         mode)
     cur, prv = B, A
     for k in range(2, K + 1):
         ck = complex(coeffs[k])
         call("qsim_cheb_spmm_step", s, *args, ptr(cur), ptr(prv), ptr(acc), ptr(comp),
-             ck.real, ck.imag, 0)
+             ck.real, ck.imag, 0, mode)
         cur, prv = prv, cur
     call("qsim_cheb_finish", s, ptr(prv), ptr(acc), ptr(comp), n_el)
     out = prv.transpose(0, 1).contiguous()
@@ -100,11 +114,12 @@ def chebyshev_evolve_batch(states, hamiltonian_csr, t, precision=1e-10):
         "workspace_bytes": 4 * n_el * 8,
         "elapsed_s": elapsed,
         "n_rhs": R,
+        "accumulate": accumulate,
     }
     return (out[0] if single else out), info
 
 
-def chebyshev_evolve(state, hamiltonian_csr, t, precision=1e-10, bk=None):
+def chebyshev_evolve(state, hamiltonian_csr, t, precision=1e-10, bk=None, accumulate="fp64"):
     """exp(-iHt)|state> for one state [2^n]; returns (result, info)
     (chebyshev.py:81-172).  `bk` is accepted for signature parity."""
-    return chebyshev_evolve_batch(state, hamiltonian_csr, t, precision)
+    return chebyshev_evolve_batch(state, hamiltonian_csr, t, precision, accumulate)

@@ -10,8 +10,12 @@ device tensors per op: blackwell_backend/simulator.py:348-404).
 Modes:
   plain  - one kernel per op (per-gate baseline);
   blocks - 5-qubit dense blocks on FP32 MFMA (n >= 10), other entries plain;
+  blocks/<k> - the same with k-qubit blocks, k = 4, 5, 6 (k = 4 needs
+           n >= 11 and k = 6 n >= 13 for their MFMA kernels);
   tiles  - LDS tile programs, many ops per HBM pass;
-  auto   - tiles with the whole state in one tile when n <= 13, else blocks.
+  auto   - tiles with the whole state in one tile when n <= 13, else blocks;
+  tune   - every applicable mode timed once on the device, fastest kept
+           (the fusion width is one of the choices).
 """
 import ctypes
 import importlib
@@ -32,7 +36,9 @@ WHOLE_OP_US_12Q = 2.0
 BLOCK_US = 5.0
 # plan steps a HIP graph can hold: library launches on the current stream with
 # device-resident arguments only (no host uploads, no host-side decisions)
-GRAPH_SAFE_STEPS = ("blocks", "tile", "mcgate")
+GRAPH_SAFE_STEPS = ("blocks", "mfma", "tile", "mcgate")
+# fusion widths of the MFMA block kernels and the smallest state each runs on
+BLOCK_WIDTHS = {4: 11, 5: BLOCK_MIN_QUBITS, 6: 13}
 TUNE_REPS = 3
 # Tile programs above 13 qubits: a tile of m qubits is one workgroup, so m
 # trades ops fused per HBM pass against workgroups per pass (2^(n-m)).
@@ -48,6 +54,11 @@ TILE_PASS_US = 5.0
 TILE_OP_US_16Q = 0.85
 TILE_OP_WEIGHT = {"m2": 1.4, "mk": 2.0}
 BLOCK_US_16Q = 8.2
+
+
+def block_modes(n):
+    """The MFMA block modes that can run on n qubits ("blocks" is width 5)."""
+    return [f"blocks/{k}" if k != 5 else "blocks" for k, nmin in sorted(BLOCK_WIDTHS.items()) if n >= nmin]
 
 
 def default_tile_shape(n):
@@ -193,14 +204,32 @@ class CompiledCircuit:
             cands.append("whole")
         else:
             cands += [f"tiles/{m}/{low}" for m, low in tune_tile_shapes(n)]
-        if n >= BLOCK_MIN_QUBITS:
-            cands.append("blocks")
+        cands += block_modes(n)
+        best = self._time_modes(cands, device)
+        self._tuned[key] = best
+        return best
+
+    def tuned_block_width(self, device):
+        """(mode, {width: ms}): the fusion width of the MFMA block plan
+        measured fastest for this circuit on `device` (same timing rule as
+        tuned_mode, candidates blocks/4, blocks/5, blocks/6)."""
+        key = ("width", str(device))
+        if key not in self._tuned:
+            times = {}
+            best = self._time_modes(block_modes(self.n_qubits), device, times)
+            self._tuned[key] = (best, {int(m.split("/")[1]): t for m, t in times.items()})
+        return self._tuned[key]
+
+    def _time_modes(self, cands, device, times=None):
+        n = self.n_qubits
+        if not cands:
+            return "plain"
         state = torch.empty(1 << n, dtype=torch.complex64, device=device)
         stream = torch.cuda.current_stream(device)
         best, best_ms = None, None
         for mode in cands:
             plan = self.plan(mode, device)
-            times = []
+            reps = []
             for r in range(TUNE_REPS + 1):
                 self.init_state(state)
                 a = torch.cuda.Event(enable_timing=True)
@@ -210,18 +239,22 @@ class CompiledCircuit:
                 b.record(stream)
                 b.synchronize()
                 if r:
-                    times.append(a.elapsed_time(b))
-            ms = float(np.median(times))
+                    reps.append(a.elapsed_time(b))
+            ms = float(np.median(reps))
+            if times is not None:
+                times[mode] = ms
             if best_ms is None or ms < best_ms:
                 best, best_ms = mode, ms
-        self._tuned[key] = best
         return best
 
-    def block_entries(self):
-        """plan_blocks of the ops (computed once per compiled circuit)."""
+    def block_entries(self, k=5):
+        """plan_blocks of the ops with k-qubit blocks (computed once per
+        compiled circuit and width)."""
         if self._blocks is None:
-            self._blocks = C.plan_blocks(self.ops, self.n_qubits, k=5)
-        return self._blocks
+            self._blocks = {}
+        if k not in self._blocks:
+            self._blocks[k] = C.plan_blocks(self.ops, self.n_qubits, k=k)
+        return self._blocks[k]
 
     # ---------------------------------------------------------------- plans
     def plan(self, mode, device):
@@ -297,6 +330,15 @@ class CompiledCircuit:
             if n < 10:
                 return self._build_tiles(C.plan_tiles(self.ops, n, m=n), device)
             return self._build_blocks(self.block_entries(), device)
+        if mode.startswith("blocks/"):
+            try:
+                k = int(mode.split("/")[1])
+            except ValueError:
+                raise ValueError(f"block mode must be 'blocks/<k>' (got {mode!r})") from None
+            if k not in BLOCK_WIDTHS or n < BLOCK_WIDTHS[k]:
+                raise ValueError(f"{mode!r} needs k in {sorted(BLOCK_WIDTHS)} and n >= "
+                                 f"{BLOCK_WIDTHS.get(k, '?')} (n = {n})")
+            return self._build_blocks(self.block_entries(k), device)
         raise ValueError(f"unknown execution mode {mode!r}")
 
     def _build_plain(self, ops, device):
@@ -357,7 +399,12 @@ class CompiledCircuit:
                 run.append(e)
                 continue
             flush()
-            if isinstance(e, C.Block):
+            k = len(e.qubits) if isinstance(e, C.Block) else 0
+            if isinstance(e, C.Block) and k in BLOCK_WIDTHS and n >= BLOCK_WIDTHS[k]:
+                # one MFMA launch with the block on the device (graph-safe)
+                mats.append(e.mat.astype(np.complex64).reshape(-1))
+                steps.append(("mfma", (len(mats) - 1, _lib.int_array(e.qubits), k)))
+            elif isinstance(e, C.Block):
                 mats.append(e.mat.astype(np.complex64).reshape(-1))
                 steps.append(("block", (len(mats) - 1, list(e.qubits))))
             else:
@@ -457,6 +504,9 @@ class _Plan:
         elif kind == "mcgate":
             idx, ctl, nc, t = a
             call("qsim_apply_mcgate", s, sp, n, self._mat_ptr(idx), ctl, nc, t)
+        elif kind == "mfma":
+            idx, qs, k = a
+            call("qsim_apply_block_mfma", s, sp, n, self._mat_ptr(idx), qs, k)
         elif kind == "block":
             idx, qs = a
             k = len(qs)

@@ -68,11 +68,14 @@ def splitmix_uniforms(seed, count):
 
 class Request:
     """One circuit to serve.  Either `circuit` (any circuit with Qiskit's
-    data layout) or a (`program`, `n_qubits`) pair from a program builder."""
+    data layout) or a (`program`, `n_qubits`) pair from a program builder.
+    With keep_state the final statevector (complex64) is copied back into
+    .state beside the samples."""
     __slots__ = ("rid", "arrival", "n_qubits", "shots", "seed", "circuit", "program", "gates",
-                 "t_admit", "t_done", "samples", "lane")
+                 "t_admit", "t_done", "samples", "lane", "keep_state", "state")
 
-    def __init__(self, rid, arrival, n_qubits, shots, seed, circuit=None, program=None, gates=None):
+    def __init__(self, rid, arrival, n_qubits, shots, seed, circuit=None, program=None, gates=None,
+                 keep_state=False):
         self.rid = rid
         self.arrival = float(arrival)
         self.n_qubits = int(n_qubits)
@@ -87,6 +90,8 @@ class Request:
         self.t_done = None
         self.samples = None
         self.lane = None
+        self.keep_state = bool(keep_state)
+        self.state = None
 
     def state_bytes(self):
         return 8 << self.n_qubits
@@ -124,11 +129,11 @@ def circuit_program(circuit):
     return tile_ops(ops, (1 << n) - 1)
 
 
-def su4_request(rid, arrival, n, shots, seed):
+def su4_request(rid, arrival, n, shots, seed, keep_state=False):
     """Request for the random-SU(4) volume circuit of depth n, carrying its
     gates (built here, on the client side, before it is submitted)."""
     return Request(rid, arrival, n, shots, seed, program=("su4", n, n, seed),
-                   gates=W.su4_arrays(n, n, seed))
+                   gates=W.su4_arrays(n, n, seed), keep_state=keep_state)
 
 
 def _materialise(req):
@@ -163,8 +168,9 @@ class _LdsBatch:
     def launch(self):
         inst = np.zeros(len(self.reqs), dtype=INSTANCE_DTYPE)
         all_ops, all_cf = [], []
-        op_off = cf_off = shot_off = 0
+        op_off = cf_off = shot_off = st_off = 0
         max_n = 1
+        self.state_off = []
         for i, r in enumerate(self.reqs):
             ops, cf = _materialise(r)
             ops = ops.copy()
@@ -176,7 +182,11 @@ class _LdsBatch:
             inst[i]["shots"] = r.shots
             inst[i]["n_qubits"] = r.n_qubits
             inst[i]["init"] = 1
-            inst[i]["want"] = 0
+            inst[i]["want"] = 1 if r.keep_state else 0
+            inst[i]["state_off"] = st_off
+            self.state_off.append(st_off)
+            if r.keep_state:
+                st_off += 1 << r.n_qubits
             all_ops.append(ops)
             all_cf.append(np.asarray(cf, dtype=np.complex64))
             op_off += len(ops)
@@ -190,14 +200,23 @@ class _LdsBatch:
             self.d_ops = torch.from_numpy(blob_ops.copy()).to(self.device)
             self.d_cf = torch.from_numpy(blob_cf).to(self.device)
             self.d_out = torch.empty(max(shot_off, 1), dtype=torch.int64, device=self.device)
-            dummy = self.d_out  # no HBM states in this lane (want = 0)
+            # final amplitudes only for keep_state requests (want = 1)
+            self.d_states = (torch.empty(st_off, dtype=torch.complex64, device=self.device)
+                             if st_off else self.d_out)
+            self.h_inst, self.h_ops = inst, blob_ops       # validated host copies of the records
             call("qsim_run_instances", ctypes.c_void_p(self.stream.cuda_stream),
-                 ctypes.c_void_p(dummy.data_ptr()), ctypes.c_void_p(self.d_inst.data_ptr()),
-                 len(self.reqs), ctypes.c_void_p(self.d_ops.data_ptr()),
-                 ctypes.c_void_p(self.d_cf.data_ptr()), ctypes.c_void_p(self.d_out.data_ptr()),
-                 max_n)
+                 ctypes.c_void_p(self.d_states.data_ptr()), st_off,
+                 ctypes.c_void_p(inst.ctypes.data), ctypes.c_void_p(self.d_inst.data_ptr()),
+                 len(self.reqs), ctypes.c_void_p(blob_ops.ctypes.data),
+                 ctypes.c_void_p(self.d_ops.data_ptr()), op_off,
+                 ctypes.c_void_p(self.d_cf.data_ptr()), int(blob_cf.size),
+                 ctypes.c_void_p(self.d_out.data_ptr()), shot_off, max_n)
             self.h_out = torch.empty(self.d_out.numel(), dtype=torch.int64, pin_memory=True)
             self.h_out.copy_(self.d_out, non_blocking=True)
+            self.h_states = None
+            if st_off:
+                self.h_states = torch.empty(st_off, dtype=torch.complex64, pin_memory=True)
+                self.h_states.copy_(self.d_states, non_blocking=True)
             self.event = torch.cuda.Event()
             self.event.record(self.stream)
 
@@ -208,10 +227,13 @@ class _LdsBatch:
         self.event.synchronize()
         out = self.h_out.numpy()
         off = 0
-        for r in self.reqs:
+        for r, so in zip(self.reqs, self.state_off):
             r.samples = out[off:off + r.shots].copy()
+            if r.keep_state:
+                r.state = self.h_states.numpy()[so:so + (1 << r.n_qubits)].copy()
             r.t_done = now
             off += r.shots
+        self.d_states = self.h_states = None
 
 
 # ---------------------------------------------------------------- HBM lane
@@ -289,6 +311,10 @@ class _HbmJob:
             self.d_out = sample_with_uniforms(self.state, u)
             self.h_out = torch.empty(r.shots, dtype=torch.int64, pin_memory=True)
             self.h_out.copy_(self.d_out, non_blocking=True)
+            self.h_state = None
+            if r.keep_state:
+                self.h_state = torch.empty(1 << r.n_qubits, dtype=torch.complex64, pin_memory=True)
+                self.h_state.copy_(self.state, non_blocking=True)
             self.event = torch.cuda.Event()
             self.event.record(self.stream)
 
@@ -298,6 +324,9 @@ class _HbmJob:
     def finish(self, now):
         self.event.synchronize()
         self.req.samples = self.h_out.numpy().copy()
+        if self.h_state is not None:
+            self.req.state = self.h_state.numpy().copy()
+            self.h_state = None
         self.req.t_done = now
         self.state = self.d_mats = None
         self.pool.release(self.staging)
@@ -374,6 +403,12 @@ class _HbmBatch:
                 self.d_out[int(shot_off[i]):int(shot_off[i]) + r.shots].copy_(d)
             self.h_out = torch.empty(self.d_out.numel(), dtype=torch.int64, pin_memory=True)
             self.h_out.copy_(self.d_out, non_blocking=True)
+            self.h_states = {}
+            for i, r in enumerate(self.reqs):
+                if r.keep_state:
+                    h = torch.empty(1 << r.n_qubits, dtype=torch.complex64, pin_memory=True)
+                    h.copy_(self.slab[int(soff[i]):int(soff[i]) + (1 << r.n_qubits)], non_blocking=True)
+                    self.h_states[i] = h
             self.shot_off = shot_off
             self.event = torch.cuda.Event()
             self.event.record(self.stream)
@@ -384,9 +419,12 @@ class _HbmBatch:
     def finish(self, now):
         self.event.synchronize()
         out = self.h_out.numpy()
-        for r, so in zip(self.reqs, self.shot_off[:-1]):
+        for i, (r, so) in enumerate(zip(self.reqs, self.shot_off[:-1])):
             r.samples = out[int(so):int(so) + r.shots].copy()
+            if i in self.h_states:
+                r.state = self.h_states[i].numpy().copy()
             r.t_done = now
+        self.h_states = {}
         self.slab = self.d_buf = self.d_out = None
         self.pool.release(self.staging)
         self.staging = None
@@ -639,6 +677,8 @@ def serve_plain(requests, sim, realtime=True):
             cc.plan("plain", dev).run(st)
         u = uniforms_splitmix(r.seed, r.shots, dev)
         r.samples = sample_with_uniforms(st, u).cpu().numpy()
+        if r.keep_state:
+            r.state = st.cpu().numpy()
         r.t_done = time.perf_counter() - t0
         r.lane = "plain"
     return summarize(reqs, time.perf_counter() - t0)

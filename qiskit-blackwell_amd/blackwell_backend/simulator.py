@@ -11,7 +11,9 @@ Differences by design:
     into plans of prepared launches (blackwell_backend.executor);
   * the default execution mode is "auto": the whole circuit in one LDS tile
     program when the state fits (n <= 13), 5-qubit FP32-MFMA blocks above;
-    mode="plain" runs one kernel per gate;
+    mode="plain" runs one kernel per gate, mode="blocks/<k>" fuses into
+    k-qubit MFMA blocks (k = 4, 5, 6) and mode="tune" times the modes and
+    fusion widths once per circuit and keeps the fastest;
   * sampling uses FP64 uniforms and an FP64 inverse CDF; counts are tallied
     with numpy instead of a Python loop over shots;
   * a circuit's global_phase multiplies the statevector (the reference
@@ -103,8 +105,8 @@ class BlackwellSimulator:
     GRAPH_MAX_QUBITS = 20
 
     def __init__(self, device="cuda", kernel_path=None, mode="auto", use_graphs=True):
-        if mode not in self.MODES and not str(mode).startswith("tiles/"):
-            raise ValueError(f"mode must be one of {self.MODES} or 'tiles/<m>/<low>'")
+        if mode not in self.MODES and not str(mode).startswith(("tiles/", "blocks/")):
+            raise ValueError(f"mode must be one of {self.MODES}, 'tiles/<m>/<low>' or 'blocks/<k>'")
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())

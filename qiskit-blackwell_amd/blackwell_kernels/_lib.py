@@ -73,7 +73,7 @@ PROTOTYPES = {
     "qsim_apply_dephasing": (_i, [_vp, _vp, _i, _i, _d, _d, _vp, _pint]),
     "qsim_apply_gate_batched": (_i, [_vp, _vp, _i64, _i, _vp, _i]),
     "qsim_apply_gates_batched_fused": (_i, [_vp, _vp, _i64, _i, _vp, _vp, _i]),
-    "qsim_run_instances": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _i]),
+    "qsim_run_instances": (_i, [_vp, _vp, _u64, _vp, _vp, _i, _vp, _vp, _u64, _vp, _u64, _vp, _u64, _i]),
     "qsim_qv8_simulate": (_i, [_vp, _vp, _vp, _i64, _i, _vp]),
     "qsim_qv4_simulate": (_i, [_vp, _vp, _vp, _i64, _i, _vp]),
     "qsim_pauli_csr_workspace_bytes": (_i64, [_i]),
@@ -83,7 +83,7 @@ PROTOTYPES = {
     "qsim_chebyshev_step": (_i, [_vp, _vp, _vp, _vp, _i64]),
     "qsim_chebyshev_accum": (_i, [_vp, _vp, _vp, _i64, _d, _d]),
     "qsim_chebyshev_step_accum": (_i, [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d]),
-    "qsim_cheb_spmm_step": (_i, [_vp, _vp, _vp, _vp, _i64, _i, _d, _vp, _vp, _vp, _vp, _d, _d, _i]),
+    "qsim_cheb_spmm_step": (_i, [_vp, _vp, _vp, _vp, _i64, _i, _d, _vp, _vp, _vp, _vp, _d, _d, _i, _i]),
     "qsim_cheb_init_acc": (_i, [_vp, _vp, _vp, _vp, _i64, _d, _d]),
     "qsim_cheb_finish": (_i, [_vp, _vp, _vp, _vp, _i64]),
 }

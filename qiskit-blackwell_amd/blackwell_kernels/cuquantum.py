@@ -189,19 +189,20 @@ def apply_mcswap(state, controls, qubit0, qubit1):
 def apply_block(state, block, qubits):
     """Dense 2^k x 2^k block on `qubits` (row bit j <-> qubits[j]).
 
-    k = 5 with n >= 10 runs on FP32 MFMA (qsim_apply_block_mfma); other k go
-    through a one-op LDS tile program.
+    k = 5 with n >= 10, k = 4 with n >= 11 and k = 6 with n >= 13 run on FP32
+    MFMA (qsim_apply_block_mfma); other blocks of up to 5 qubits go through a
+    one-op LDS tile program.
     """
     n = n_qubits_of(state)
     qs = [int(q) for q in qubits]
     k = len(qs)
     b = _dev_matrix(block, (1 << k, 1 << k), "block", state.device)
-    if k == 5 and n >= 10:
+    if (k == 5 and n >= 10) or (k == 4 and n >= 11) or (k == 6 and n >= 13):
         call("qsim_apply_block_mfma", stream_handle(state.device), ptr(state), n, ptr(b),
              _lib.int_array(qs), k)
         return state
     if k > 5:
-        raise RuntimeError("blocks of more than 5 qubits are not supported")
+        raise RuntimeError("blocks of 6 qubits need n >= 13; wider blocks are not supported")
     mask = 0
     for q in qs:
         mask |= 1 << q

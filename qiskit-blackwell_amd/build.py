@@ -3,8 +3,10 @@ include/qsim_gfx950.h.
 
 Every .hip file under csrc/ is compiled for gfx950 only, with FP contraction
 off (the compensated arithmetic relies on explicit FMAs) and without packed
-FP32 instructions, then linked into blackwell_kernels/libqsim_gfx950.so next
-to the ctypes loader.  Run as a script or call build().
+FP32 instructions; every .cpp file (host-only code, no HIP headers) is
+compiled by the host C++ compiler.  All are linked into
+blackwell_kernels/libqsim_gfx950.so next to the ctypes loader.  Run as a
+script or call build().
 """
 import concurrent.futures
 import os
@@ -36,8 +38,17 @@ def _hipcc():
     return exe
 
 
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-I", INCLUDE, "-I", CSRC]
+
+
 def _compile(src):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    if src.endswith(".cpp"):
+        cxx = shutil.which("g++") or "g++"
+        r = subprocess.run([cxx, *HOST_FLAGS, "-c", src, "-o", obj], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed on {src}:\n{r.stderr}")
+        return obj
     cmd = [_hipcc(), *FLAGS, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     # hipcc's host pass warns that it does not know the packed-fp32 feature
@@ -50,7 +61,7 @@ def _compile(src):
 def build(verbose=False):
     """Compile every csrc/*.hip for gfx950 and link the shared library."""
     os.makedirs(BUILD, exist_ok=True)
-    srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
     workers = min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
     with concurrent.futures.ThreadPoolExecutor(workers) as ex:
         objs = list(ex.map(_compile, srcs))

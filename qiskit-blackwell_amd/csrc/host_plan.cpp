@@ -1,7 +1,6 @@
-// Host-side runtime of libqsim_gfx950.so: the dense-block planner and the
-// block-plan launcher.  Pure host code (no kernels): the planner runs on any
-// machine, the launcher issues one MFMA block launch per entry without a
-// round trip through the caller's language per block.
+// Host-side runtime of libqsim_gfx950.so: the dense-block planner.  Pure
+// C++ (no HIP headers, no kernels), compiled by the host compiler, so it
+// also builds under the host sanitizers (tests/test_planner_sanitizers.py).
 //
 // Planner.  Ops arrive in circuit order, each with up to 8 qubits and a
 // dense FP64 matrix on them (row bit j <-> qubit j of the op).  A block is
@@ -16,7 +15,9 @@
 // applied to the rows of the running 2^k x 2^k matrix.  Blocks are padded with
 // the lowest unused qubits to exactly min(k, n) qubits.  An op wider than k
 // becomes a passthrough entry for the caller to run on its own kernel.
-#include "qsim_internal.hpp"
+#include "qsim_gfx950.h"
+#include "host_error.hpp"
+#include <stdint.h>
 #include <algorithm>
 #include <string.h>
 #include <vector>
@@ -194,20 +195,6 @@ int qsim_plan_blocks(int n_qubits, int k, int n_ops, const int32_t* op_nq, const
     out_ref[n_entries++] = n_blocks++;
   }
   *n_out = n_entries;
-  return QSIM_OK;
-}
-
-int qsim_apply_blocks(void* stream, void* state, int n_qubits, const void* blocks_dev,
-                      const int32_t* block_qubits, int n_blocks) {
-  if (n_blocks < 0) return fail(QSIM_EINVAL, "n_blocks < 0");
-  if (n_blocks > 0 && (!blocks_dev || !block_qubits)) return fail(QSIM_EINVAL, "NULL pointer argument");
-  const float2* b = (const float2*)blocks_dev;
-  for (int i = 0; i < n_blocks; ++i) {
-    int qs[5];
-    for (int j = 0; j < 5; ++j) qs[j] = block_qubits[i * 8 + j];
-    if (int rc = qsim_apply_block_mfma(stream, state, n_qubits, b + (size_t)i * 1024, qs, 5))
-      return rc;
-  }
   return QSIM_OK;
 }
 

@@ -106,46 +106,204 @@ __global__ __launch_bounds__(kThreads) void k_step_accum(const float2* __restric
   acc[i] = make_float2(z.x + v.x, z.y + v.y);
 }
 
-// Fused batched Chebyshev term.  Thread = (row, j), j fastest.
+// ---------------------------------------------------------------- 32 RHS
+// The batched term at 32 right-hand sides, the width of one RHS-contiguous
+// row (32 x 8 B = 256 B = one half-wave of float2).  One wave owns two
+// consecutive CSR rows: lanes 0..31 row r, lanes 32..63 row r + 1, lane & 31
+// the RHS.  Per nonzero a half-wave needs the column index and the value
+// (the same for its 32 lanes) and one 256-byte gathered row of T_k.
+//  * Row data: the wave's two rows are consecutive in the CSR arrays, so it
+//    stages them into its own LDS slice with coalesced full-wave loads
+//    (nontemporal: the CSR is read once per term) and reads them back with
+//    half-wave broadcasts.  Loading them per lane from global memory instead
+//    returns 12 B per lane per nonzero through the vector L1, three times the
+//    bytes of the gather itself.
+//  * Row sum, ACC = kAccF64: the product of two FP32 numbers is exact in FP64
+//    (24 + 24 bits < 53), so every product is converted and accumulated with
+//    FP64 FMAs.  CDNA4 issues FP64 FMA at the FP32 rate, so this costs 8 VALU
+//    operations per complex multiply-add against ~44 for the TwoProduct +
+//    Neumaier double-float form (ACC = kAccF32c), which caps a 22-qubit,
+//    32-RHS term at ~5 ms on VALU issue alone; the FP64 sum is also the more
+//    accurate of the two (53-bit accumulator, exact terms).
+//  * Epilogue (both forms): T_{k+1} = 2 scale (A T_k) - T_{k-1}, rounded to
+//    complex64 once, and acc += c_k T_{k+1} as a Neumaier pair (acc, comp) in
+//    FP32, as in the generic kernel.  T_{k-1}, acc and comp are streamed
+//    (nontemporal when the 32-RHS block exceeds the MALL), so the cache keeps
+//    the gathered rows of T_k.
+//  * Block order: order_shift s > 0 hands each XCD contiguous chunks of 2^s
+//    rows, chunk x of every group of 8 chunks going to XCD x, so that a
+//    gather through a low column bit stays in the XCD's own L2 while all 8
+//    XCDs still walk one front through the MALL.  s = 0 is the dispatch
+//    order (blocks round-robin over the XCDs).
+constexpr int kAccF64 = 0;
+constexpr int kAccF32c = 1;
+constexpr int kStageNnz = 128;      // nonzeros of a wave's two rows staged per chunk
+constexpr int kWaves = kThreads / 64;
+#ifndef QSIM_SPMM_ORDER_SHIFT
+#define QSIM_SPMM_ORDER_SHIFT 9
+#endif
+
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+template <bool NT>
+__device__ __forceinline__ float2 ld2(const float2* p) {
+  if constexpr (NT) {
+    const f32x2_t v = __builtin_nontemporal_load(reinterpret_cast<const f32x2_t*>(p));
+    return make_float2(v.x, v.y);
+  } else {
+    return *p;
+  }
+}
+template <bool NT>
+__device__ __forceinline__ void st2(float2* p, float2 v) {
+  if constexpr (NT) {
+    const f32x2_t w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<f32x2_t*>(p));
+  } else {
+    *p = v;
+  }
+}
+
+// Wave-level ordering of LDS writes and reads by other lanes of the same wave.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Logical block of this workgroup under the chunked XCD order (a bijection
+// on [0, gridDim.x) when gridDim.x is a multiple of 8 * chunk blocks; the
+// host passes s = 0 otherwise).
+__device__ __forceinline__ int64_t spmm_block(int s) {
+  const int64_t b = blockIdx.x;
+  if (s <= 0) return b;
+  const int g = s > 3 ? s - 3 : 0;            // log2(blocks per chunk): 8 rows per block
+  const int64_t xcd = b & 7, i = b >> 3;
+  const int64_t grp = i >> g, w = i & ((1ll << g) - 1);
+  return ((grp * 8 + xcd) << g) + w;
+}
+
+template <int ACC>
+struct RowSum;
+template <>
+struct RowSum<kAccF64> {
+  double r = 0.0, i = 0.0;
+  __device__ __forceinline__ void add(float2 v, float2 x) {
+    const double vr = v.x, vi = v.y, xr = x.x, xi = x.y;
+    r = __builtin_fma(vr, xr, r);
+    r = __builtin_fma(-vi, xi, r);
+    i = __builtin_fma(vr, xi, i);
+    i = __builtin_fma(vi, xr, i);
+  }
+  // scale * sum, rounded once to FP32 after the recurrence update
+  __device__ __forceinline__ float2 next(double scale, float2 tp, bool first) const {
+    const double wr = r * scale, wi = i * scale;
+    if (first) return make_float2((float)wr, (float)wi);
+    return make_float2((float)(2.0 * wr - (double)tp.x), (float)(2.0 * wi - (double)tp.y));
+  }
+};
+template <>
+struct RowSum<kAccF32c> {
+  float sr = 0.f, cr = 0.f, si = 0.f, ci = 0.f;
+  __device__ __forceinline__ void add(float2 v, float2 x) { cmac_acc(v, x, sr, cr, si, ci); }
+  __device__ __forceinline__ float2 next(double scale_d, float2 tp, bool first) const {
+    const float scale = (float)scale_d;
+    float p, q, wr, wi;
+    two_prod(sr, scale, p, q); wr = p + (q + cr * scale);
+    two_prod(si, scale, p, q); wi = p + (q + ci * scale);
+    if (first) return make_float2(wr, wi);
+    return make_float2(2.f * wr - tp.x, 2.f * wi - tp.y);
+  }
+};
+
+template <int ACC, bool NT>
+__global__ __launch_bounds__(kThreads) void k_cheb_spmm32(
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, const float2* __restrict__ val,
+    int64_t n_rows, double scale, const float2* __restrict__ tc, float2* __restrict__ tpn,
+    float2* __restrict__ acc, float2* __restrict__ comp, float2 c, int first, int order_shift) {
+  __shared__ int32_t s_ci[kWaves][kStageNnz];
+  __shared__ float2 s_v[kWaves][kStageNnz];
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63, h = lane >> 5, j = lane & 31;
+  const int64_t row0 = (spmm_block(order_shift) * kWaves + wave) * 2;
+  if (row0 >= n_rows) return;
+  const bool has1 = row0 + 1 < n_rows;
+  const int32_t b0 = rp[row0], b1 = rp[row0 + 1], b2 = has1 ? rp[row0 + 2] : b1;
+  const int32_t my_b = h ? b1 : b0, my_e = h ? b2 : b1;
+  const char* tcb = reinterpret_cast<const char*>(tc);
+  const uint32_t lane_off = (uint32_t)j * 8u;
+  RowSum<ACC> sum;
+  for (int32_t c0 = b0; c0 < b2; c0 += kStageNnz) {
+    const int32_t cend = c0 + kStageNnz < b2 ? c0 + kStageNnz : b2;
+    for (int32_t k = c0 + lane; k < cend; k += 64) {
+      s_ci[wave][k - c0] = __builtin_nontemporal_load(ci + k);
+      s_v[wave][k - c0] = ld2<true>(val + k);
+    }
+    wave_sync();
+    const int32_t lo = my_b > c0 ? my_b : c0, hi = my_e < cend ? my_e : cend;
+    int32_t k = lo;
+    for (; k + kGather <= hi; k += kGather) {
+      float2 v[kGather], x[kGather];
+#pragma unroll
+      for (int u = 0; u < kGather; ++u) {
+        const uint32_t col = (uint32_t)s_ci[wave][k + u - c0];
+        v[u] = s_v[wave][k + u - c0];
+        x[u] = *reinterpret_cast<const float2*>(tcb + (col * 256u + lane_off));
+      }
+#pragma unroll
+      for (int u = 0; u < kGather; ++u) sum.add(v[u], x[u]);
+    }
+    for (; k < hi; ++k) {
+      const uint32_t col = (uint32_t)s_ci[wave][k - c0];
+      sum.add(s_v[wave][k - c0], *reinterpret_cast<const float2*>(tcb + (col * 256u + lane_off)));
+    }
+    wave_sync();
+  }
+  if (h && !has1) return;
+  const int64_t e_idx = (row0 + h) * 32 + j;
+  const float2 tp = first ? make_float2(0.f, 0.f) : ld2<NT>(tpn + e_idx);
+  const float2 tn = sum.next(scale, tp, first != 0);
+  st2<NT>(tpn + e_idx, tn);
+  const float2 v = cmulc(c, tn);
+  float2 a = ld2<NT>(acc + e_idx), k2 = ld2<NT>(comp + e_idx);
+  neu_add(a.x, k2.x, v.x);
+  neu_add(a.y, k2.y, v.y);
+  st2<NT>(acc + e_idx, a);
+  st2<NT>(comp + e_idx, k2);
+}
+
+// Fused batched Chebyshev term for any number of right-hand sides (and for
+// 32-RHS blocks beyond 4 GiB).  Thread = (row, j), j fastest; the row data
+// come per lane from global memory.
+template <int ACC>
 __global__ __launch_bounds__(kThreads) void k_cheb_spmm(const int32_t* __restrict__ rp,
                                                         const int32_t* __restrict__ ci,
                                                         const float2* __restrict__ val,
-                                                        int64_t n_rows, int n_rhs, float scale,
+                                                        int64_t n_rows, int n_rhs, double scale,
                                                         const float2* __restrict__ tc,
                                                         float2* __restrict__ tpn,
                                                         float2* __restrict__ acc,
                                                         float2* __restrict__ comp, float2 c,
                                                         int first) {
-  const int64_t e_idx = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  const int64_t e_idx = gblock<int64_t>() * kThreads + threadIdx.x;
   const int64_t row = e_idx / n_rhs;
   const int j = (int)(e_idx - row * n_rhs);
   if (row >= n_rows) return;
-  float sr = 0.f, cr = 0.f, si = 0.f, cc = 0.f;
+  RowSum<ACC> sum;
   const int32_t b = rp[row], e = rp[row + 1];
   int32_t k = b;
   for (; k + kGather <= e; k += kGather) {
-    int32_t c[kGather];
+    int32_t cc[kGather];
     float2 v[kGather], xs[kGather];
 #pragma unroll
-    for (int u = 0; u < kGather; ++u) { c[u] = ci[k + u]; v[u] = val[k + u]; }
+    for (int u = 0; u < kGather; ++u) { cc[u] = ci[k + u]; v[u] = val[k + u]; }
 #pragma unroll
-    for (int u = 0; u < kGather; ++u) xs[u] = tc[(int64_t)c[u] * n_rhs + j];
+    for (int u = 0; u < kGather; ++u) xs[u] = tc[(int64_t)cc[u] * n_rhs + j];
 #pragma unroll
-    for (int u = 0; u < kGather; ++u) cmac_acc(v[u], xs[u], sr, cr, si, cc);
+    for (int u = 0; u < kGather; ++u) sum.add(v[u], xs[u]);
   }
-  for (; k < e; ++k) cmac_acc(val[k], tc[(int64_t)ci[k] * n_rhs + j], sr, cr, si, cc);
-  // w = scale * (A t_cur): the exact product errors of the scaling are kept
-  float p, q;
-  float wr, wi;
-  two_prod(sr, scale, p, q); wr = p + (q + cr * scale);
-  two_prod(si, scale, p, q); wi = p + (q + cc * scale);
-  float2 tn;
-  if (first) {
-    tn = make_float2(wr, wi);
-  } else {
-    const float2 tp = tpn[e_idx];
-    tn = make_float2(2.f * wr - tp.x, 2.f * wi - tp.y);
-  }
+  for (; k < e; ++k) sum.add(val[k], tc[(int64_t)ci[k] * n_rhs + j]);
+  const float2 tp = first ? make_float2(0.f, 0.f) : tpn[e_idx];
+  const float2 tn = sum.next(scale, tp, first != 0);
   tpn[e_idx] = tn;
   const float2 v = cmulc(c, tn);
   float2 a = acc[e_idx], k2 = comp[e_idx];
@@ -228,17 +386,37 @@ int qsim_chebyshev_step_accum(void* stream, const void* s, const void* t_prev, v
 int qsim_cheb_spmm_step(void* stream, const void* row_ptr, const void* col_idx, const void* values,
                         int64_t n_rows, int n_rhs, double scale, const void* t_cur,
                         void* t_prev_next, void* acc, void* comp, double c_re, double c_im,
-                        int first) {
+                        int first, int accumulate) {
   if (n_rows < 0 || n_rhs < 1) return fail(QSIM_EINVAL, "bad sizes (n_rows=%lld, n_rhs=%d)",
                                            (long long)n_rows, n_rhs);
+  if (accumulate != QSIM_CHEB_ACC_FP64 && accumulate != QSIM_CHEB_ACC_FP32C)
+    return fail(QSIM_EINVAL, "unknown accumulate mode %d", accumulate);
   if (n_rows == 0) return QSIM_OK;
   if (!row_ptr || !col_idx || !values || !t_cur || !t_prev_next || !acc || !comp)
     return fail(QSIM_EINVAL, "NULL pointer argument");
+  hipStream_t s = as_stream(stream);
+  const float2 cf = f2(c_re, c_im);
+  const bool f64 = accumulate == QSIM_CHEB_ACC_FP64;
+  // the 32-RHS kernel addresses T_k with 32-bit byte offsets (<= 2^24 rows)
+  if (n_rhs == 32 && n_rows <= (int64_t(1) << 24)) {
+    const int64_t blocks = (int64_t)ceil_div((uint64_t)n_rows, 2 * kWaves);
+    const int chunk_blocks = QSIM_SPMM_ORDER_SHIFT > 3 ? 1 << (QSIM_SPMM_ORDER_SHIFT - 3) : 1;
+    const int order = blocks % (8 * (int64_t)chunk_blocks) == 0 ? QSIM_SPMM_ORDER_SHIFT : 0;
+    const bool nt = nt_for_bytes((uint64_t)n_rows * 256u);
+    auto* kern = f64 ? (nt ? k_cheb_spmm32<kAccF64, true> : k_cheb_spmm32<kAccF64, false>)
+                     : (nt ? k_cheb_spmm32<kAccF32c, true> : k_cheb_spmm32<kAccF32c, false>);
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kThreads), 0, s, (const int32_t*)row_ptr,
+                       (const int32_t*)col_idx, (const float2*)values, n_rows, scale,
+                       (const float2*)t_cur, (float2*)t_prev_next, (float2*)acc, (float2*)comp, cf,
+                       first, order);
+    return launch_status("cheb_spmm_step");
+  }
   const int64_t total = n_rows * (int64_t)n_rhs;
-  hipLaunchKernelGGL(k_cheb_spmm, dim3(ceil_div(total, kThreads)), dim3(kThreads), 0,
-                     as_stream(stream), (const int32_t*)row_ptr, (const int32_t*)col_idx,
-                     (const float2*)values, n_rows, n_rhs, (float)scale, (const float2*)t_cur,
-                     (float2*)t_prev_next, (float2*)acc, (float2*)comp, f2(c_re, c_im), first);
+  auto* kern = f64 ? k_cheb_spmm<kAccF64> : k_cheb_spmm<kAccF32c>;
+  hipLaunchKernelGGL(kern, grid1d(ceil_div(total, kThreads)), dim3(kThreads), 0, s,
+                     (const int32_t*)row_ptr, (const int32_t*)col_idx, (const float2*)values,
+                     n_rows, n_rhs, scale, (const float2*)t_cur, (float2*)t_prev_next, (float2*)acc,
+                     (float2*)comp, cf, first);
   return launch_status("cheb_spmm_step");
 }
 

@@ -890,7 +890,7 @@ using namespace qsim;
 extern "C" {
 
 const char* qsim_version(void) { return "qsim-gfx950 0.1.0"; }
-int qsim_abi_version(void) { return 1; }
+int qsim_abi_version(void) { return 2; }
 const char* qsim_last_error(void) { return g_err; }
 
 int qsim_device_count(int* count) {

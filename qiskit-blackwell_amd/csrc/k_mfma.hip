@@ -449,6 +449,199 @@ __global__ __launch_bounds__(256) void k_block5_lds(float2* __restrict__ st, Blo
   }
 }
 
+// ------------------------------------------------------------ k = 4 and k = 6
+// LDS-staged blocks of K = 4 or 6 qubits, the k_block5_lds scheme with the
+// tile widened or narrowed: a workgroup of 4 waves owns 128 groups, the tile
+// is the 2^(K+7) amplitudes spanned by the K block qubits and the 7 lowest
+// non-block qubits (so it always holds qubits 0..6 and moves as runs of at
+// least 128 contiguous amplitudes), transposed on the way in to
+// X[row][group] in LDS (row stride padded by one element).  Wave w takes
+// groups 32w .. 32w + 31.
+//  * K = 6 (64 x 64 complex): v_mfma_f32_32x32x2_f32 over two 32-row output
+//    tiles and 32 K slices; the A fragments (U[32 t + c][2 s + h], 128 floats
+//    per lane) stay in VGPRs.  At 512 flop per amplitude (AI = 32 flop/B,
+//    above the 19.7 ridge) it is bound by the MFMA rate.
+//  * K = 4 (16 x 16 complex): v_mfma_f32_16x16x4_f32, whose operands are
+//    A[i][k] from lane i % 16 + 16 k (k = 0..3), B[k][j] from lane
+//    j % 16 + 16 k, and D[4 (l / 16) + r][l % 16] in register r of lane l;
+//    two 16-group column tiles per wave, 4 K slices.  AI = 8 flop/B: bound
+//    by HBM.
+template <int K>
+struct BlockKTile {
+  int n;
+  uint8_t tq[K + 7];     // tile qubits, ascending (tq[0] = 0)
+  uint8_t tcode[K + 7];  // tile bit k: < 8 -> row bit tcode[k]; >= 8 -> group bit tcode[k] - 8
+  int nh;                // non-block qubits above the tile
+  uint8_t hp[QSIM_MAX_QUBITS];
+};
+
+template <int K>
+__device__ __forceinline__ void tilek_rc(uint32_t j, const BlockKTile<K>& a, uint32_t& r, uint32_t& c,
+                                         uint64_t& off) {
+  r = 0; c = 0; off = 0;
+#pragma unroll
+  for (int k = 0; k < K + 7; ++k) {
+    const uint32_t bit = (j >> k) & 1u;
+    off |= (uint64_t)bit << a.tq[k];
+    const uint32_t code = a.tcode[k];
+    if (code < 8) r |= bit << code;
+    else c |= bit << (code - 8);
+  }
+}
+
+typedef float f32x4m __attribute__((ext_vector_type(4)));
+
+template <int K, bool NT>
+__global__ __launch_bounds__(256) void k_blockk_lds(float2* __restrict__ st, BlockKTile<K> a,
+                                                    const float2* __restrict__ U) {
+  constexpr int R = 1 << K;                     // rows
+  constexpr int SLOTS = (R * kTileGroups) / 2 / 256;   // float4 slots per thread
+  __shared__ float2 X[R * kRowStride];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint64_t base = 0;
+  const uint64_t b = blockIdx.x;
+#pragma unroll
+  for (int k = 0; k < QSIM_MAX_QUBITS; ++k)
+    if (k < a.nh) base |= ((b >> k) & 1ull) << a.hp[k];
+  float4* sp = reinterpret_cast<float4*>(st);
+  const uint32_t c0 = a.tcode[0];
+  {
+    float4 v[SLOTS];
+    uint32_t l0[SLOTS], l1[SLOTS];
+#pragma unroll
+    for (int k = 0; k < SLOTS; ++k) {
+      uint32_t r, cc;
+      uint64_t off;
+      tilek_rc<K>(2u * (threadIdx.x + 256 * k), a, r, cc, off);
+      v[k] = ld4<NT>(sp + ((base | off) >> 1));
+      const uint32_t r1 = c0 < 8 ? (r | (1u << c0)) : r;
+      const uint32_t cc1 = c0 < 8 ? cc : (cc | (1u << (c0 - 8)));
+      l0[k] = r * kRowStride + cc;
+      l1[k] = r1 * kRowStride + cc1;
+    }
+#pragma unroll
+    for (int k = 0; k < SLOTS; ++k) {
+      X[l0[k]] = lo2(v[k]);
+      X[l1[k]] = hi2(v[k]);
+    }
+  }
+  if constexpr (K == 6) {
+    const int h = lane >> 5, c = lane & 31;
+    float ur[2][32], ui[2][32];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int s = 0; s < 32; ++s) {
+        const float2 u = U[(32 * t + c) * 64 + 2 * s + h];
+        ur[t][s] = u.x;
+        ui[t][s] = u.y;
+      }
+    __syncthreads();
+    const int col = wv * 32 + c;
+    f32x16 ar0 = {0.f}, ai0 = {0.f}, ar1 = {0.f}, ai1 = {0.f};
+#pragma unroll
+    for (int s = 0; s < 32; ++s) {
+      const float2 x = X[(2 * s + h) * kRowStride + col];
+      ar0 = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[0][s], x.x, ar0, 0, 0, 0);
+      ar0 = __builtin_amdgcn_mfma_f32_32x32x2f32(-ui[0][s], x.y, ar0, 0, 0, 0);
+      ai0 = __builtin_amdgcn_mfma_f32_32x32x2f32(ui[0][s], x.x, ai0, 0, 0, 0);
+      ai0 = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[0][s], x.y, ai0, 0, 0, 0);
+      ar1 = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[1][s], x.x, ar1, 0, 0, 0);
+      ar1 = __builtin_amdgcn_mfma_f32_32x32x2f32(-ui[1][s], x.y, ar1, 0, 0, 0);
+      ai1 = __builtin_amdgcn_mfma_f32_32x32x2f32(ui[1][s], x.x, ai1, 0, 0, 0);
+      ai1 = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[1][s], x.y, ai1, 0, 0, 0);
+    }
+    // a wave reads and then overwrites only its own 32 columns, and LDS
+    // operations of one wave complete in order, so no barrier is needed here
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int i = 8 * (r >> 2) + 4 * h + (r & 3);
+      X[i * kRowStride + col] = make_float2(ar0[r], ai0[r]);
+      X[(32 + i) * kRowStride + col] = make_float2(ar1[r], ai1[r]);
+    }
+  } else {
+    static_assert(K == 4, "k_blockk_lds handles K = 4 and K = 6");
+    const int kq = lane >> 4, m = lane & 15;     // K index within a slice, row / column
+    float ur[4], ui[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const float2 u = U[m * 16 + 4 * s + kq];
+      ur[s] = u.x;
+      ui[s] = u.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+      const int col = wv * 32 + 16 * ct + m;
+      f32x4m accr = {0.f, 0.f, 0.f, 0.f}, acci = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const float2 x = X[(4 * s + kq) * kRowStride + col];
+        accr = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[s], x.x, accr, 0, 0, 0);
+        accr = __builtin_amdgcn_mfma_f32_16x16x4f32(-ui[s], x.y, accr, 0, 0, 0);
+        acci = __builtin_amdgcn_mfma_f32_16x16x4f32(ui[s], x.x, acci, 0, 0, 0);
+        acci = __builtin_amdgcn_mfma_f32_16x16x4f32(ur[s], x.y, acci, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) X[(4 * kq + r) * kRowStride + col] = make_float2(accr[r], acci[r]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < SLOTS; ++k) {
+    uint32_t r, cc;
+    uint64_t off;
+    tilek_rc<K>(2u * (threadIdx.x + 256 * k), a, r, cc, off);
+    const uint32_t r1 = c0 < 8 ? (r | (1u << c0)) : r;
+    const uint32_t cc1 = c0 < 8 ? cc : (cc | (1u << (c0 - 8)));
+    st4<NT>(sp + ((base | off) >> 1), cat4(X[r * kRowStride + cc], X[r1 * kRowStride + cc1]));
+  }
+}
+
+// Tile of the K block qubits qs[0..K) and the 7 lowest other qubits.
+template <int K>
+static BlockKTile<K> make_tilek(int n, const int* qs) {
+  BlockKTile<K> tl;
+  memset(&tl, 0, sizeof tl);
+  tl.n = n;
+  uint64_t bm = 0;
+  for (int j = 0; j < K; ++j) bm |= 1ull << qs[j];
+  int low[7], nl = 0;
+  for (int q = 0; q < n && nl < 7; ++q)
+    if (!(bm >> q & 1)) low[nl++] = q;
+  uint64_t tmask = bm;
+  for (int k = 0; k < 7; ++k) tmask |= 1ull << low[k];
+  int nt = 0;
+  for (int q = 0; q < n; ++q) {
+    if (!(tmask >> q & 1)) continue;
+    tl.tq[nt] = (uint8_t)q;
+    int code = -1;
+    for (int j = 0; j < K; ++j)
+      if (qs[j] == q) code = j;
+    if (code < 0)
+      for (int k = 0; k < 7; ++k)
+        if (low[k] == q) code = 8 + k;
+    tl.tcode[nt++] = (uint8_t)code;
+  }
+  tl.nh = 0;
+  for (int q = 0; q < n; ++q)
+    if (!(tmask >> q & 1)) tl.hp[tl.nh++] = (uint8_t)q;
+  return tl;
+}
+
+template <int K>
+static int launch_blockk(hipStream_t s, void* state, int n, const void* block_dev, const int* qs) {
+  const BlockKTile<K> tl = make_tilek<K>(n, qs);
+  const dim3 grid((unsigned)(1ull << tl.nh));
+  if (nt_for_qubits(n))
+    hipLaunchKernelGGL((k_blockk_lds<K, true>), grid, dim3(256), 0, s, (float2*)state, tl,
+                       (const float2*)block_dev);
+  else
+    hipLaunchKernelGGL((k_blockk_lds<K, false>), grid, dim3(256), 0, s, (float2*)state, tl,
+                       (const float2*)block_dev);
+  return launch_status("apply_block_mfma");
+}
+
 }  // namespace qsim
 
 using namespace qsim;
@@ -459,8 +652,20 @@ int qsim_apply_block_mfma(void* stream, void* state, int n_qubits, const void* b
                           const int* qubits, int k) {
   if (int rc = check_n(n_qubits)) return rc;
   if (!state || !block_dev || !qubits) return fail(QSIM_EINVAL, "NULL pointer argument");
-  if (k != 5) return fail(QSIM_ERANGE, "MFMA blocks support k = 5 (got %d)", k);
+  if (k < 4 || k > 6) return fail(QSIM_ERANGE, "MFMA blocks support k = 4, 5, 6 (got %d)", k);
+  if (n_qubits < k + 7 && k != 5)
+    return fail(QSIM_ERANGE, "MFMA blocks of %d qubits need n_qubits >= %d (got %d)", k, k + 7, n_qubits);
   if (n_qubits < 10) return fail(QSIM_ERANGE, "MFMA blocks need n_qubits >= 10 (got %d)", n_qubits);
+  if (k != 5) {
+    uint64_t seen = 0;
+    for (int j = 0; j < k; ++j) {
+      if (int rc = check_q(qubits[j], n_qubits, "block qubit")) return rc;
+      if (seen >> qubits[j] & 1) return fail(QSIM_EINVAL, "block qubits must differ");
+      seen |= 1ull << qubits[j];
+    }
+    return k == 4 ? launch_blockk<4>(as_stream(stream), state, n_qubits, block_dev, qubits)
+                  : launch_blockk<6>(as_stream(stream), state, n_qubits, block_dev, qubits);
+  }
   BlockArgs a;
   memset(&a, 0, sizeof a);
   a.n = n_qubits;
@@ -573,6 +778,22 @@ int qsim_apply_block_batch(void* stream, const qsim_block_rec* recs_host, const 
   hipLaunchKernelGGL(k_block5_batch, dim3((unsigned)ceil_div(waves, 4)), dim3(256), 0, as_stream(stream),
                      (const qsim_block_rec*)recs_dev, (int)n_recs, total, spw);
   return launch_status("apply_block_batch");
+}
+
+// Block-plan launcher: one MFMA block launch per entry, without a round
+// trip through the caller's language per block.
+int qsim_apply_blocks(void* stream, void* state, int n_qubits, const void* blocks_dev,
+                      const int32_t* block_qubits, int n_blocks) {
+  if (n_blocks < 0) return fail(QSIM_EINVAL, "n_blocks < 0");
+  if (n_blocks > 0 && (!blocks_dev || !block_qubits)) return fail(QSIM_EINVAL, "NULL pointer argument");
+  const float2* b = (const float2*)blocks_dev;
+  for (int i = 0; i < n_blocks; ++i) {
+    int qs[5];
+    for (int j = 0; j < 5; ++j) qs[j] = block_qubits[i * 8 + j];
+    if (int rc = qsim_apply_block_mfma(stream, state, n_qubits, b + (size_t)i * 1024, qs, 5))
+      return rc;
+  }
+  return QSIM_OK;
 }
 
 }  // extern "C"

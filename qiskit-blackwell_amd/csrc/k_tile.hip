@@ -588,6 +588,48 @@ __global__ __launch_bounds__(64) void k_qv4(const float* __restrict__ gd, const 
 
 using namespace qsim;
 
+// Host check of a program that runs on a state (or tile) of m qubits: op
+// kinds, target positions (in range, distinct), controls (in range, not on a
+// target) and coefficient ranges.  `inst` names the instance in the message.
+static int check_ops(const qsim_op* ops, uint32_t n_ops, int m, uint64_t n_coefs, int inst) {
+  for (uint32_t k = 0; k < n_ops; ++k) {
+    const qsim_op& o = ops[k];
+    int nq = 0, ncoef = 0;
+    switch (o.kind) {
+      case QSIM_OP_M1: nq = 1; ncoef = 4; break;
+      case QSIM_OP_M2: nq = 2; ncoef = 16; break;
+      case QSIM_OP_D1: nq = 1; ncoef = 2; break;
+      case QSIM_OP_D2: nq = 2; ncoef = 4; break;
+      case QSIM_OP_PH: nq = 1; ncoef = 1; break;
+      case QSIM_OP_X: nq = 1; ncoef = 0; break;
+      case QSIM_OP_SWAP: nq = 2; ncoef = 0; break;
+      case QSIM_OP_MK:
+        nq = o.nq;
+        if (nq < 1 || nq > 5) return fail(QSIM_EINVAL, "instance %d, op %u: dense block of %d qubits", inst, k, nq);
+        ncoef = 1 << (2 * nq);
+        break;
+      case QSIM_OP_DK:
+        nq = o.nq;
+        if (nq < 1 || nq > 6) return fail(QSIM_EINVAL, "instance %d, op %u: diagonal of %d qubits", inst, k, nq);
+        ncoef = 1 << nq;
+        break;
+      default: return fail(QSIM_EINVAL, "instance %d, op %u: unknown kind %d", inst, k, o.kind);
+    }
+    uint32_t tmask = 0;
+    for (int j = 0; j < nq; ++j) {
+      if (o.q[j] >= m) return fail(QSIM_EINVAL, "instance %d, op %u: qubit %d >= n_qubits %d", inst, k, o.q[j], m);
+      if (tmask & (1u << o.q[j])) return fail(QSIM_EINVAL, "instance %d, op %u: repeated qubit %d", inst, k, o.q[j]);
+      tmask |= 1u << o.q[j];
+    }
+    if ((m < 32 && (o.cmask >> m)) || (o.cmask & tmask))
+      return fail(QSIM_EINVAL, "instance %d, op %u: control mask 0x%x invalid for %d qubits", inst, k, o.cmask, m);
+    if ((uint64_t)o.coef + (uint64_t)ncoef > n_coefs)
+      return fail(QSIM_EINVAL, "instance %d, op %u: coefficients [%u, +%d) beyond the %llu passed", inst, k,
+                  o.coef, ncoef, (unsigned long long)n_coefs);
+  }
+  return QSIM_OK;
+}
+
 extern "C" {
 
 int qsim_tile_program(void* stream, void* state, int n_qubits, uint64_t tile_mask,
@@ -661,15 +703,38 @@ int qsim_apply_gates_batched_fused(void* stream, void* states, int64_t n_states,
                         targets_dev, n_gates);
 }
 
-int qsim_run_instances(void* stream, void* states_base, const void* inst_dev, int n_inst,
-                       const void* ops_dev, const void* coefs_dev, void* shots_dev,
-                       int max_qubits) {
+int qsim_run_instances(void* stream, void* states_base, uint64_t n_state_slots,
+                       const qsim_instance* inst_host, const void* inst_dev, int n_inst,
+                       const qsim_op* ops_host, const void* ops_dev, uint64_t n_ops_total,
+                       const void* coefs_dev, uint64_t n_coefs, void* shots_dev,
+                       uint64_t n_shot_slots, int max_qubits) {
+  clear_error();
   if (n_inst < 0) return fail(QSIM_EINVAL, "n_inst < 0");
   if (n_inst == 0) return QSIM_OK;
-  if (!inst_dev || !ops_dev || !coefs_dev) return fail(QSIM_EINVAL, "NULL pointer argument");
+  if (!inst_host || !inst_dev || !ops_host || !ops_dev || !coefs_dev)
+    return fail(QSIM_EINVAL, "NULL pointer argument");
   if (max_qubits < 1 || max_qubits > kTileMaxBits)
     return fail(QSIM_ERANGE, "instances hold at most %d qubits in LDS (got %d)", kTileMaxBits,
                 max_qubits);
+  for (int i = 0; i < n_inst; ++i) {
+    const qsim_instance& in = inst_host[i];
+    if (in.n_qubits < 1 || in.n_qubits > max_qubits)
+      return fail(QSIM_EINVAL, "instance %d: n_qubits = %d outside 1..max_qubits (%d)", i,
+                  in.n_qubits, max_qubits);
+    if ((uint64_t)in.op_off + in.n_ops > n_ops_total)
+      return fail(QSIM_EINVAL, "instance %d: ops [%u, %u + %u) beyond the %llu ops passed", i,
+                  in.op_off, in.op_off, in.n_ops, (unsigned long long)n_ops_total);
+    const uint64_t amps = 1ull << in.n_qubits;
+    if ((!in.init || (in.want & 1)) &&
+        (!states_base || in.state_off > n_state_slots || n_state_slots - in.state_off < amps))
+      return fail(QSIM_EINVAL, "instance %d: state [%llu, +%llu) beyond the %llu state slots", i,
+                  (unsigned long long)in.state_off, (unsigned long long)amps,
+                  (unsigned long long)n_state_slots);
+    if (in.shots && (!shots_dev || in.shot_off > n_shot_slots || n_shot_slots - in.shot_off < in.shots))
+      return fail(QSIM_EINVAL, "instance %d: shots [%llu, +%u) beyond the %llu shot slots", i,
+                  (unsigned long long)in.shot_off, in.shots, (unsigned long long)n_shot_slots);
+    if (int rc = check_ops(ops_host + in.op_off, in.n_ops, in.n_qubits, n_coefs, i)) return rc;
+  }
   const size_t lds = sizeof(float2) << max_qubits;
   hipLaunchKernelGGL(k_instances, dim3(n_inst), dim3(kTileThreads), lds, as_stream(stream),
                      (float2*)states_base, (const qsim_instance*)inst_dev, (const qsim_op*)ops_dev,

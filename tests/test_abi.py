@@ -8,6 +8,7 @@ import re
 import subprocess
 import tempfile
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -90,7 +91,7 @@ def test_meta_and_validation_without_device():
     from blackwell_kernels import _lib
     lib = _lib.load()
     assert lib.qsim_version().startswith(b"qsim-gfx950")
-    assert lib.qsim_abi_version() == 1
+    assert lib.qsim_abi_version() == 2
     c = ctypes.c_int(-1)
     assert lib.qsim_device_count(ctypes.byref(c)) == 0 and c.value >= 0
     # argument errors are reported before any HIP call
@@ -103,13 +104,17 @@ def test_meta_and_validation_without_device():
     assert lib.qsim_apply_mcx(None, ctypes.c_void_p(8), 4, ctl, 1, 2) == 1
     assert b"controls" in lib.qsim_last_error()
     assert lib.qsim_apply_block_mfma(None, ctypes.c_void_p(8), 12, ctypes.c_void_p(8),
-                                     _lib.int_array([0, 1, 2, 3]), 4) == 2
+                                     _lib.int_array([0, 1, 2, 3, 4, 5, 6]), 7) == 2
+    assert lib.qsim_apply_block_mfma(None, ctypes.c_void_p(8), 12, ctypes.c_void_p(8),
+                                     _lib.int_array([0, 1, 2, 3, 4, 5]), 6) == 2      # k = 6 needs n >= 13
+    assert lib.qsim_apply_block_mfma(None, ctypes.c_void_p(8), 13, ctypes.c_void_p(8),
+                                     _lib.int_array([0, 1, 2, 3, 4, 4]), 6) == 1      # repeated qubit
     assert lib.qsim_sample_workspace_bytes(20) > 0
     assert lib.qsim_sample_workspace_bytes(0) == -1
     assert lib.qsim_tile_program(None, ctypes.c_void_p(8), 10, 0b110, ctypes.c_void_p(8), 1,
                                  ctypes.c_void_p(8)) == 1                 # mask without qubit 0
-    assert lib.qsim_run_instances(None, None, ctypes.c_void_p(8), 1, ctypes.c_void_p(8),
-                                  ctypes.c_void_p(8), None, 15) == 2
+    v8_ = ctypes.c_void_p(8)
+    assert lib.qsim_run_instances(None, None, 0, v8_, v8_, 1, v8_, v8_, 1, v8_, 1, None, 0, 15) == 2
     # Pauli strings and noise channels
     v8 = ctypes.c_void_p(8)
     assert lib.qsim_expectation_pauli(None, v8, 4, 0, 1, 3, v8, v8) == 1
@@ -179,7 +184,7 @@ def test_kernel_path_loading(tmp_path, monkeypatch):
     try:
         bk_b = _load_kernels(str(other))
         assert os.path.abspath(bk_b.__file__).startswith(str(other))
-        assert bk_b._lib.load().qsim_abi_version() == 1
+        assert bk_b._lib.load().qsim_abi_version() == 2
         monkeypatch.setenv("BLACKWELL_KERNEL_PATH", pkg)
         bk_env = _load_kernels(None)
         assert os.path.abspath(bk_env.__file__).startswith(os.path.abspath(pkg))
@@ -187,3 +192,52 @@ def test_kernel_path_loading(tmp_path, monkeypatch):
         if str(other) in sys.path:
             sys.path.remove(str(other))
         _load_kernels(pkg)
+
+
+def test_run_instances_rejects_bad_records_without_device():
+    """qsim_run_instances checks the host copy of every instance record and
+    op before it launches: an instance larger than max_qubits, ops, states,
+    shots or coefficients beyond the buffers passed, an op qubit >= n, a
+    repeated target, a control on a target and an unknown kind are all
+    QSIM_EINVAL with the instance named (no device is touched)."""
+    from blackwell_kernels import _lib
+    lib = _lib.load()
+    v8 = ctypes.c_void_p(8)
+
+    def rec(n=4, op_off=0, n_ops=1, shots=16, state_off=0, init=1, want=1):
+        r = np.zeros(1, dtype=_lib.INSTANCE_DTYPE)
+        r["n_qubits"], r["op_off"], r["n_ops"], r["shots"] = n, op_off, n_ops, shots
+        r["state_off"], r["init"], r["want"] = state_off, init, want
+        return r
+
+    def op(kind=_lib.OP_M1, q=(1,), cmask=0, coef=0, nq=None):
+        o = np.zeros(1, dtype=_lib.OP_DTYPE)
+        o["kind"], o["nq"], o["cmask"], o["coef"] = kind, len(q) if nq is None else nq, cmask, coef
+        o["q"][0, :len(q)] = q
+        return o
+
+    def run(r, ops, n_state=16, n_coefs=64, n_shots=16, max_q=4):
+        return (lib.qsim_run_instances(None, v8, n_state, r.ctypes.data, v8, len(r), ops.ctypes.data, v8,
+                                       len(ops), v8, n_coefs, v8, n_shots, max_q),
+                lib.qsim_last_error())
+
+    good = run(rec(), op())
+    assert good[0] != 1, good          # valid records get past validation (launch fails without a GPU)
+    cases = [
+        (rec(n=5), op(), {}, b"n_qubits"),
+        (rec(n_ops=2), op(), {}, b"ops"),
+        (rec(state_off=8), op(), {}, b"state"),
+        (rec(init=0, want=0, state_off=8), op(), {}, b"state"),
+        (rec(shots=17), op(), {}, b"shots"),
+        (rec(), op(q=(4,)), {}, b"qubit 4"),
+        (rec(), op(kind=_lib.OP_M2, q=(2, 2)), {}, b"repeated"),
+        (rec(), op(q=(1,), cmask=0b10), {}, b"control"),
+        (rec(), op(q=(1,), cmask=0b10000), {}, b"control"),
+        (rec(), op(kind=_lib.OP_M2, q=(0, 1), coef=60), {}, b"coefficients"),
+        (rec(), op(kind=_lib.OP_MK, q=(0, 1, 2), coef=8), {"n_coefs": 64}, b"coefficients"),
+        (rec(), op(kind=_lib.OP_MK, q=(0, 1, 2, 3, 0, 0), nq=6), {}, b"dense block"),
+        (rec(), op(kind=42), {}, b"unknown kind"),
+    ]
+    for r, o, kw, msg in cases:
+        rc, err = run(r, o, **kw)
+        assert rc == 1 and msg in err and b"instance 0" in err, (rc, err, msg)

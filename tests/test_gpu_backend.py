@@ -318,14 +318,14 @@ def test_tune_mode_picks_a_measured_mode(sim):
     ref = oracle_sv(qc)
     check(ref, sv)
     mode = tuned.compile(qc).tuned_mode(tuned.device)
-    assert mode in ("plain", "whole", "blocks")
+    assert mode in ("plain", "whole", "blocks", "blocks/4")
     res = tuned.run(qc, shots=1000, seed=3)
     assert sum(res.get_counts().values()) == 1000
     # above 13 qubits the tuner also times tile shapes; every shape is exact
     qc = W.qft(15)
     ref = oracle_sv(qc)
     mode = tuned.compile(qc).tuned_mode(tuned.device)
-    assert mode == "plain" or mode == "blocks" or mode.startswith("tiles/")
+    assert mode == "plain" or mode.startswith("blocks") or mode.startswith("tiles/")
     for m, low in ((13, 3), (12, 4), (11, 5)):
         sv, _ = tuned.run_statevector(qc, mode=f"tiles/{m}/{low}")
         check(ref, sv)
@@ -360,3 +360,27 @@ def test_two_kernel_builds_side_by_side(tmp_path):
             sys.path.remove(str(other))
         from blackwell_backend.simulator import _load_kernels
         _load_kernels(pkg)
+
+
+def test_block_widths_and_width_tuner(sim):
+    """Circuits run as fused blocks of 4, 5 and 6 qubits match the FP64
+    oracle; the width tuner times all three and returns the fastest, and
+    every width it timed is a valid mode."""
+    from test_compiler import random_circuit
+    from blackwell_backend import workloads as W
+    for seed, n in [(4, 13), (5, 16)]:
+        qc = random_circuit(n, 160, seed)
+        ref = oracle_sv(qc)
+        for mode in ("blocks/4", "blocks", "blocks/6"):
+            sv, _ = sim.run_statevector(qc, mode=mode)
+            check(ref, sv)
+    qc = W.su4_volume(16, 16, 7)
+    ref = oracle_sv(qc)
+    cc = sim.compile(qc)
+    best, times = cc.tuned_block_width(sim.device)
+    assert set(times) == {4, 5, 6} and best in ("blocks/4", "blocks", "blocks/6")
+    assert times[int(best.split("/")[1]) if "/" in best else 5] == min(times.values())
+    sv, _ = sim.run_statevector(qc, mode=best)
+    check(ref, sv)
+    with pytest.raises(ValueError):
+        sim.run_statevector(W.qft(12), mode="blocks/6")       # k = 6 needs n >= 13

@@ -362,6 +362,26 @@ def test_block_mfma(bk, dev, n, qubits):
     assert _rel_err(ref, st.cpu().numpy()) < 8e-6
 
 
+@pytest.mark.parametrize("n,qubits", [
+    (11, [0, 1, 2, 3]), (11, [3, 1, 10, 6]), (14, [7, 12, 2, 13]), (12, [11, 8, 9, 10]),
+    (13, [0, 1, 2, 3, 4, 5]), (13, [12, 0, 7, 3, 9, 5]), (15, [14, 13, 12, 8, 9, 10]),
+    (16, [2, 15, 6, 11, 1, 8]),
+])
+def test_block_mfma_widths_4_and_6(bk, dev, n, qubits):
+    """k = 4 (v_mfma_f32_16x16x4f32) and k = 6 (two 32-row tiles of
+    v_mfma_f32_32x32x2f32) blocks from the LDS tile kernel against the FP64
+    oracle (row bit j <-> qubits[j]), for sorted, unsorted, low and high
+    qubit sets; a block of 64 terms accumulates in FP32 like k = 5."""
+    k = len(qubits)
+    psi = rand_state(n, 7 * n + sum(qubits))
+    u = rand_unitary(1 << k, 3 * n + qubits[-1]).astype(np.complex64)
+    st = to_dev(psi, dev)
+    print(f"[mfma] n={n} k={k} qubits={qubits}", flush=True)
+    bk.apply_block(st, torch.from_numpy(u).to(dev), qubits)
+    ref = O.apply_matrix(psi.astype(np.complex128), u.astype(np.complex128), qubits)
+    assert _rel_err(ref, st.cpu().numpy()) < 8e-6
+
+
 def test_qv8_qv4(bk, dev):
     m, q, g = bk.generate_qv8_circuits(37, seed=3)
     got = bk.qv8_simulate(m.to(dev), q.to(dev), 37).cpu().numpy()
@@ -420,11 +440,14 @@ def test_run_instances_with_sampling(bk, dev):
         shot_off += shots
     states = torch.zeros(st_off, dtype=torch.complex64, device=dev)
     out = torch.full((shot_off,), -1, dtype=torch.int64, device=dev)
-    inst_d = torch.from_numpy(np.concatenate(inst).view(np.uint8).copy()).to(dev)
-    ops_d = torch.from_numpy(np.concatenate(all_ops).view(np.uint8).copy()).to(dev)
+    inst_h = np.concatenate(inst)
+    ops_h = np.concatenate(all_ops)
+    inst_d = torch.from_numpy(inst_h.view(np.uint8).copy()).to(dev)
+    ops_d = torch.from_numpy(ops_h.view(np.uint8).copy()).to(dev)
     cf_d = torch.from_numpy(np.concatenate(all_cf).astype(np.complex64)).to(dev)
-    _lib.call("qsim_run_instances", _lib.stream_handle(dev), _lib.ptr(states), _lib.ptr(inst_d),
-              len(ns), _lib.ptr(ops_d), _lib.ptr(cf_d), _lib.ptr(out), max(ns))
+    _lib.call("qsim_run_instances", _lib.stream_handle(dev), _lib.ptr(states), st_off,
+              inst_h.ctypes.data, _lib.ptr(inst_d), len(ns), ops_h.ctypes.data, _lib.ptr(ops_d),
+              len(ops_h), _lib.ptr(cf_d), cf_d.numel(), _lib.ptr(out), shot_off, max(ns))
     st = states.cpu().numpy()
     sh = out.cpu().numpy()
     off = 0
@@ -468,6 +491,85 @@ def test_spmv_and_chebyshev_kernels(bk, dev):
     t2 = 2 * x - y
     assert np.allclose(tn.cpu().numpy(), t2, atol=1e-6)
     assert np.allclose(acc.cpu().numpy(), y + (0.2 + 0.1j) * t2, atol=1e-5)
+
+
+def _ragged_csr(n_rows, seed):
+    """Random complex CSR with ragged rows: empty rows, short rows and rows of
+    150-300 nonzeros (longer than a wave's 128-nonzero staging chunk), sorted
+    column indices."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    A = sp.random(n_rows, n_rows, density=0.03, random_state=seed, format="lil")
+    A = A.astype(np.complex128)
+    for r in (3, 4, 11, n_rows - 1):
+        cols = rng.choice(n_rows, size=int(rng.integers(150, 300)), replace=False)
+        A[r, cols] = rng.standard_normal(len(cols)) + 1.0
+    A = A.tocsr()
+    A.data = A.data + 1j * rng.standard_normal(A.data.shape)
+    for r in (0, 7, n_rows - 2):
+        A.data[A.indptr[r]:A.indptr[r + 1]] = 0
+    A.eliminate_zeros()
+    A.sort_indices()
+    assert A.indptr[1] == 0 and max(np.diff(A.indptr)) > 128
+    return A
+
+
+@pytest.mark.parametrize("accumulate", ["fp64", "fp32c"])
+def test_cheb_spmm_step_matches_fp64(bk, dev, accumulate):
+    """One fused Chebyshev term, qsim_cheb_spmm_step, against the same term in
+    FP64 on the host: w = scale A T_k, T_{k+1} = w (first) or 2 w - T_{k-1},
+    acc + comp += c T_{k+1}.  701 rows (odd: the last wave has one row),
+    ragged row lengths, 32 RHS (the two-rows-per-wave kernel) and 5 RHS (the
+    generic kernel).  The 32 RHS run also equals, bit for bit, the same term
+    computed as two 16-RHS halves through the generic kernel (same summation
+    order per element)."""
+    from blackwell_backend.chebyshev import ACCUMULATE
+    from blackwell_kernels._lib import call, ptr, stream_handle
+    mode = ACCUMULATE[accumulate]
+    A = _ragged_csr(701, 5)
+    N = A.shape[0]
+    A64 = A.astype(np.complex64).astype(np.complex128)
+    rp = torch.from_numpy(A.indptr.astype(np.int32)).to(dev)
+    ci = torch.from_numpy(A.indices.astype(np.int32)).to(dev)
+    va = torch.from_numpy(A.data.astype(np.complex64)).to(dev)
+    rng = np.random.default_rng(9)
+    scale, c = 0.37, 0.3 - 0.45j
+    # the bound on each element's error scales with sum_k |a_ik| |t_kj|
+    absA = abs(A64)
+    for R in (32, 5):
+        def cplx(*shape):
+            return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+        T, Tp, acc0 = cplx(N, R), cplx(N, R), cplx(N, R)
+        for first in (1, 0):
+            tpd, accd = to_dev(Tp, dev), to_dev(acc0, dev)
+            compd = torch.zeros_like(accd)
+            call("qsim_cheb_spmm_step", stream_handle(dev), ptr(rp), ptr(ci), ptr(va), N, R, scale,
+                 ptr(to_dev(T, dev)), ptr(tpd), ptr(accd), ptr(compd), c.real, c.imag, first, mode)
+            w = scale * (A64 @ T.astype(np.complex128))
+            tn_ref = w if first else 2 * w - Tp.astype(np.complex128)
+            bound = scale * (absA @ np.abs(T.astype(np.complex128))) * 2 + np.abs(Tp)
+            tn = tpd.cpu().numpy().astype(np.complex128)
+            err = np.abs(tn - tn_ref)
+            # FP64 sums: one rounding to complex64 (<= 2^-24 per component);
+            # compensated FP32: a few roundings of the row's term magnitudes
+            tol = 2.0 ** -23 * np.abs(tn_ref) + (1e-15 if accumulate == "fp64" else 4e-7) * bound
+            assert np.all(err <= tol + 1e-30), (R, first, float(np.max(err - tol)))
+            got_acc = accd.cpu().numpy().astype(np.complex128) + compd.cpu().numpy()
+            want_acc = acc0.astype(np.complex128) + c * tn
+            assert np.max(np.abs(got_acc - want_acc)) <= 1e-6 * np.max(np.abs(want_acc)), (R, first)
+            if R == 32:
+                halves = []
+                for h in range(2):
+                    sl = slice(16 * h, 16 * h + 16)
+                    tph = to_dev(np.ascontiguousarray(Tp[:, sl]), dev)
+                    acch = to_dev(np.ascontiguousarray(acc0[:, sl]), dev)
+                    comph = torch.zeros_like(acch)
+                    call("qsim_cheb_spmm_step", stream_handle(dev), ptr(rp), ptr(ci), ptr(va), N, 16,
+                         scale, ptr(to_dev(np.ascontiguousarray(T[:, sl]), dev)), ptr(tph), ptr(acch),
+                         ptr(comph), c.real, c.imag, first, mode)
+                    halves.append(tph.cpu().numpy())
+                both = np.concatenate(halves, axis=1)
+                assert np.array_equal(both.view(np.uint32), tpd.cpu().numpy().view(np.uint32)), first
 
 
 def test_block_batch_equals_per_instance_blocks(bk, dev):

@@ -78,3 +78,37 @@ def test_gate_payload_equals_circuit_path(sim):
         S.Server(sim, planner_threads=0).serve([b], realtime=False)
         assert a.lane == b.lane == "hbm"
         assert np.array_equal(a.samples, b.samples), n
+
+
+def test_served_states_match_fp64_alone_and_mixed(sim):
+    """keep_state requests in both lanes (LDS n <= 14, HBM batched and
+    unbatched) return their final statevectors: fidelity >= 1 - 1e-7 and
+    max |d amp| <= 1e-5 against the FP64 statevector of the same gates, and
+    the state of a request served alone equals, bit for bit, its state
+    served inside the mix."""
+    from blackwell_backend import serving as S, workloads as W
+    ns = [10, 14, 12, 16, 19, 11, 20, 15, 17]
+
+    def mk():
+        reqs = [S.su4_request(i, 0.0, n, 256, 31 * 1000 + i, keep_state=True)
+                for i, n in enumerate(ns[:-1])]
+        # no gate payload: the HBM lane runs it alone, compiled from its circuit
+        i, n = len(ns) - 1, ns[-1]
+        reqs.append(S.Request(i, 0.0, n, 256, 31 * 1000 + i, program=("su4", n, n, 31 * 1000 + i),
+                              keep_state=True))
+        return reqs
+    mixed = mk()
+    S.Server(sim).serve(mixed, realtime=False)
+    for r in mixed:
+        assert r.state is not None and r.state.shape == (1 << r.n_qubits,)
+        pairs, us = W.su4_arrays(r.n_qubits, r.n_qubits, r.seed)
+        psi = O.zero_state(r.n_qubits)
+        for (a, b), u in zip(pairs, us):
+            psi = O.apply_matrix(psi, np.asarray(u, dtype=np.complex128), [int(a), int(b)])
+        f = O.fidelity(psi, r.state)
+        d = float(np.max(np.abs(psi - r.state)))
+        assert f >= 1 - 1e-7 and d <= 1e-5, (r.n_qubits, r.lane, f, d)
+    for i in (0, 3, 6, 8):
+        alone = mk()[i]
+        S.Server(sim).serve([alone], realtime=False)
+        assert np.array_equal(alone.state.view(np.uint32), mixed[i].state.view(np.uint32)), ns[i]
