@@ -406,8 +406,18 @@ def c5_bench(n=22, n_rhs=32, times=(1.0, 5.0, 10.0), seed=5, precision=1e-7):
         Y55, _ = chebyshev_evolve_batch(outs[5.0][:4].contiguous(), h, 5.0, precision)
         d = (Y55.to(torch.complex128) - outs[10.0][:4].to(torch.complex128)).abs().max().item()
         res["composition_5+5_vs_10_max_abs"] = d
+    # the reference's compensated FP32 row sum (TwoProduct + Neumaier) at t = 1, for comparison
+    torch.cuda.synchronize()
+    Yc, infoc = chebyshev_evolve_batch(X, h, times[0], precision, accumulate="fp32c")
+    Kc = infoc["chebyshev_degree"]
+    res["fp32c_row_sums"] = {
+        "t": times[0], "K": Kc, "ms_per_term": round(infoc["elapsed_s"] / Kc * 1e3, 4),
+        "max_abs_diff_vs_fp64_row_sums": (Yc.to(torch.complex128) -
+                                          outs[times[0]].to(torch.complex128)).abs().max().item()}
+    del Yc
     return {"workload": f"{n}-qubit open chain ZZ + (XX+YY) + X, seed {seed}, CSR complex64 "
                         f"nnz {h.nnz}, {n_rhs} RHS, Bessel tail < {precision:g}",
+            "row_sums": "FP64 sums of exact FP32 products (accumulate='fp64')",
             "csr_build_s": round(build_s, 2), "results": res}
 
 

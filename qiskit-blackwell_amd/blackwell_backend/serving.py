@@ -21,6 +21,14 @@ LDS-lane requests are grouped into a launch as soon as the previous LDS
 launch has been handed to the device, so their work overlaps with the
 running batches and the HBM lane.
 
+Repeated circuits on the HBM lane replay as HIP graphs: a request that
+carries a circuit (no gate payload) and whose compiled circuit the server
+has already run once is executed by replaying a captured graph of state
+init + its block plan (one host call instead of one launch per block), into
+the graph's own buffer, which is sampled before the next replay on the
+same stream.  Gate-payload requests are unique by construction and run as
+batched block steps instead.
+
 Sampling in both lanes uses the same rule and the same uniforms for a given
 seed: u_k = SplitMix64(seed + (k+1) * golden) >> 11 scaled by 2^-53 and
 index = smallest i with cdf(i) > u_k * ||psi||^2.  Each instance's arithmetic
@@ -257,11 +265,13 @@ class _PinnedPool:
 
 
 class _HbmJob:
-    def __init__(self, req, sim, stream, pool=None):
+    def __init__(self, req, sim, stream, pool=None, graphs=None):
         self.req = req
         self.reqs = [req]
         self.sim = sim
         self.stream = stream
+        self.graphs = graphs          # _GraphCache, or None: no graph replay
+        self.replayed = False
         self.pool = pool if pool is not None else _PinnedPool()
         self.staging = None
         self.event = None
@@ -291,9 +301,9 @@ class _HbmJob:
             self.prepare()
         sh = ctypes.c_void_p(self.stream.cuda_stream)
         with torch.cuda.stream(self.stream):
-            self.state = torch.empty(1 << r.n_qubits, dtype=torch.complex64, device=dev)
-            call("qsim_state_init", sh, ctypes.c_void_p(self.state.data_ptr()), r.n_qubits)
             if self.blocks is not None:
+                self.state = torch.empty(1 << r.n_qubits, dtype=torch.complex64, device=dev)
+                call("qsim_state_init", sh, ctypes.c_void_p(self.state.data_ptr()), r.n_qubits)
                 bqs, bm = self.blocks
                 raw = bm.view(np.uint8).reshape(-1)
                 self.staging = self.pool.acquire(raw.size)
@@ -305,7 +315,14 @@ class _HbmJob:
                      len(bqs))
             else:
                 cc = self.sim.compile(_circuit_of(r))
-                cc.plan(cc.resolve_mode("blocks"), dev).run(self.state)
+                g = self.graphs.get(cc, dev) if self.graphs is not None else None
+                if g is not None:
+                    self.state = g.replay()       # graph buffer, read before the next replay
+                    self.replayed = True
+                else:
+                    self.state = torch.empty(1 << r.n_qubits, dtype=torch.complex64, device=dev)
+                    cc.init_state(self.state)
+                    cc.plan(cc.resolve_mode("blocks"), dev).run(self.state)
             from blackwell_kernels import sample_with_uniforms, uniforms_splitmix
             u = uniforms_splitmix(r.seed, r.shots, dev)
             self.d_out = sample_with_uniforms(self.state, u)
@@ -430,6 +447,43 @@ class _HbmBatch:
         self.staging = None
 
 
+class _GraphCache:
+    """HIP graphs of compiled circuits seen more than once on the HBM lane
+    (at most `max_graphs`, least recently used dropped; states of at most
+    `max_qubits`).  A circuit's first run is eager and only counted, so a
+    stream of distinct circuits never pays for a capture."""
+
+    def __init__(self, max_graphs=8, max_qubits=24):
+        self.max_graphs = int(max_graphs)
+        self.max_qubits = int(max_qubits)
+        self.seen = collections.Counter()
+        self.graphs = collections.OrderedDict()
+        self.captures = 0
+        self.replays = 0
+
+    def get(self, cc, device):
+        if cc.n_qubits > self.max_qubits or cc.has_stochastic or self.max_graphs <= 0:
+            return None
+        key = id(cc)
+        self.seen[key] += 1
+        g = self.graphs.get(key)
+        if g is None:
+            if self.seen[key] < 2:
+                return None
+            mode = cc.resolve_mode("blocks")
+            g = cc.graphed(mode, device)
+            if g is None:
+                return None
+            self.captures += 1
+            self.graphs[key] = g
+            while len(self.graphs) > self.max_graphs:
+                self.graphs.popitem(last=False)
+        else:
+            self.graphs.move_to_end(key)
+        self.replays += 1
+        return g
+
+
 # ---------------------------------------------------------------- server
 class Server:
     """Continuous-batching server.
@@ -444,12 +498,14 @@ class Server:
                      steps
     planner_threads  host threads that build HBM-lane block plans ahead
     plan_ahead       most queued HBM-lane circuits planned ahead of admission
+    graph_repeats    replay repeated HBM-lane circuits as HIP graphs (at most
+                     this many graphs kept; 0 turns it off)
     """
 
     def __init__(self, sim=None, budget_bytes=None, max_batch=2048, max_hbm_jobs=4,
                  device="cuda:0", lds_factory=None, hbm_factory=None, planner_threads=8,
                  plan_ahead=256, max_hbm_batch=64, hbm_batch_factory=None,
-                 max_hbm_batch_bytes=HBM_BATCH_BYTES):
+                 max_hbm_batch_bytes=HBM_BATCH_BYTES, graph_repeats=8):
         """lds_factory(reqs) / hbm_factory(req) build lane jobs with launch(),
         done() and finish(now); hbm_batch_factory(jobs) runs several HBM-lane
         jobs as one unit (None: one job per unit).  The defaults run on the
@@ -480,7 +536,9 @@ class Server:
             self.hbm_stream = torch.cuda.Stream(self.device)
             self.lds_factory = lambda reqs: _LdsBatch(reqs, self.device, self.lds_stream)
             self.pinned = _PinnedPool()
-            self.hbm_factory = lambda req: _HbmJob(req, self.sim, self.hbm_stream, self.pinned)
+            self.graphs = _GraphCache(graph_repeats)
+            self.hbm_factory = lambda req: _HbmJob(req, self.sim, self.hbm_stream, self.pinned,
+                                                   self.graphs)
             if hbm_batch_factory is None:
                 self.hbm_batch_factory = lambda jobs: _HbmBatch(jobs, self.hbm_stream, self.pinned)
         self.rejected = []

@@ -30,6 +30,10 @@ namespace qsim {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
+#ifndef QSIM_BLOCK5W_INTERLEAVE
+#define QSIM_BLOCK5W_INTERLEAVE 0
+#endif
+
 struct BlockArgs {
   int n;
   int q[5];          // block qubits, row bit j <-> q[j]
@@ -251,9 +255,19 @@ __global__ __launch_bounds__(256) void k_block5w(float2* __restrict__ st, uint64
     wof[r] = Q0 ? (uint32_t)row_off(8 * ((r >> 1) >> 1) + 4 * h + 2 * ((r >> 1) & 1), a)
                 : (uint32_t)row_off(8 * (r >> 2) + 4 * h + (r & 3), a);
   const uint64_t tw = (uint64_t)gridDim.x * 4;
-  const uint64_t gw = (uint64_t)xcd_block<uint32_t>() * 4 + (threadIdx.x >> 6);
-  const uint64_t per = (n_sets + tw - 1) / tw;
-  const uint64_t s0 = gw * per, s1 = s0 + per < n_sets ? s0 + per : n_sets;
+  uint64_t s0, s1, step;
+  if (QSIM_BLOCK5W_INTERLEAVE) {
+    // all waves advance together through one window of tw sets
+    s0 = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    s1 = n_sets;
+    step = tw;
+  } else {
+    const uint64_t gw = (uint64_t)xcd_block<uint32_t>() * 4 + (threadIdx.x >> 6);
+    const uint64_t per = (n_sets + tw - 1) / tw;
+    s0 = gw * per;
+    s1 = s0 + per < n_sets ? s0 + per : n_sets;
+    step = 1;
+  }
   if (s0 >= s1) return;
   const uint64_t gl = Q0 ? (uint64_t)(c + 32 * h) : (uint64_t)(2 * c);
   f32x4_t x[16], y[16];
@@ -263,11 +277,11 @@ __global__ __launch_bounds__(256) void k_block5w(float2* __restrict__ st, uint64
     const f32x4_t* p = reinterpret_cast<const f32x4_t*>(st + (base | rof[s]));
     x[s] = NT ? __builtin_nontemporal_load(p) : *p;
   }
-  for (uint64_t set = s0; set < s1; ++set) {
-    const bool more = set + 1 < s1;
+  for (uint64_t set = s0; set < s1; set += step) {
+    const bool more = set + step < s1;
     uint64_t nbase = 0;
     if (PF && more) {
-      nbase = group_base((set + 1) * 64 + gl, a);
+      nbase = group_base((set + step) * 64 + gl, a);
 #pragma unroll
       for (int s = 0; s < 16; ++s) {
         const f32x4_t* p = reinterpret_cast<const f32x4_t*>(st + (nbase | rof[s]));
@@ -330,7 +344,7 @@ __global__ __launch_bounds__(256) void k_block5w(float2* __restrict__ st, uint64
 #pragma unroll
       for (int s = 0; s < 16; ++s) x[s] = y[s];
     } else {
-      base = group_base((set + 1) * 64 + gl, a);
+      base = group_base((set + step) * 64 + gl, a);
 #pragma unroll
       for (int s = 0; s < 16; ++s) {
         const f32x4_t* p = reinterpret_cast<const f32x4_t*>(st + (base | rof[s]));

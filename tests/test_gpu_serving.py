@@ -112,3 +112,36 @@ def test_served_states_match_fp64_alone_and_mixed(sim):
         alone = mk()[i]
         S.Server(sim).serve([alone], realtime=False)
         assert np.array_equal(alone.state.view(np.uint32), mixed[i].state.view(np.uint32)), ns[i]
+
+
+def test_repeated_circuits_replay_as_graphs(sim):
+    """HBM-lane requests that repeat a circuit (no gate payload) replay a
+    captured HIP graph from the second occurrence on; their samples and
+    states equal, bit for bit, those of the same requests served with graph
+    replay turned off, and every result matches the FP64 statevector."""
+    from blackwell_backend import serving as S, workloads as W
+
+    def mk():
+        out = []
+        for i in range(6):
+            n, cseed = (16, 808) if i % 2 == 0 else (18, 909)
+            out.append(S.Request(i, 0.0, n, 512, 5000 + i, program=("su4", n, n, cseed), keep_state=True))
+        return out
+    with_graphs, without = mk(), mk()
+    srv = S.Server(sim)
+    srv.serve(with_graphs, realtime=False)
+    S.Server(sim, graph_repeats=0).serve(without, realtime=False)
+    assert srv.graphs.captures == 2 and srv.graphs.replays == 4, (srv.graphs.captures, srv.graphs.replays)
+    for a, b in zip(with_graphs, without):
+        assert a.lane == b.lane == "hbm"
+        assert np.array_equal(a.samples, b.samples), a.rid
+        assert np.array_equal(a.state.view(np.uint32), b.state.view(np.uint32)), a.rid
+    for r in with_graphs[:2]:
+        qc = W.su4_volume(r.n_qubits, r.n_qubits, r.program[3])
+        psi = O.run_instructions(r.n_qubits, [(nm, q, p) for nm, q, _, p in instructions_of(qc)])
+        assert O.fidelity(psi, r.state) >= 1 - 1e-7
+
+
+def instructions_of(qc):
+    from blackwell_backend.circuit import instructions
+    return instructions(qc)
