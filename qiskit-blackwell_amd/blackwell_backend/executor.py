@@ -56,6 +56,11 @@ TILE_OP_WEIGHT = {"m2": 1.4, "mk": 2.0}
 BLOCK_US_16Q = 8.2
 
 
+def block_width(mode):
+    """Fusion width of a block mode ("blocks" is 5, "blocks/<k>" is k)."""
+    return 5 if mode == "blocks" else int(mode.split("/")[1])
+
+
 def block_modes(n):
     """The MFMA block modes that can run on n qubits ("blocks" is width 5)."""
     return [f"blocks/{k}" if k != 5 else "blocks" for k, nmin in sorted(BLOCK_WIDTHS.items()) if n >= nmin]
@@ -217,7 +222,7 @@ class CompiledCircuit:
         if key not in self._tuned:
             times = {}
             best = self._time_modes(block_modes(self.n_qubits), device, times)
-            self._tuned[key] = (best, {int(m.split("/")[1]): t for m, t in times.items()})
+            self._tuned[key] = (best, {block_width(m): t for m, t in times.items()})
         return self._tuned[key]
 
     def _time_modes(self, cands, device, times=None):

@@ -142,6 +142,10 @@ constexpr int kWaves = kThreads / 64;
 #ifndef QSIM_SPMM_ORDER_SHIFT
 #define QSIM_SPMM_ORDER_SHIFT 9
 #endif
+#ifndef QSIM_SPMM32_GATHER
+#define QSIM_SPMM32_GATHER 4
+#endif
+constexpr int kGather32 = QSIM_SPMM32_GATHER;   // gathers in flight per lane
 
 typedef float f32x2_t __attribute__((ext_vector_type(2)));
 template <bool NT>
@@ -241,16 +245,16 @@ __global__ __launch_bounds__(kThreads) void k_cheb_spmm32(
     wave_sync();
     const int32_t lo = my_b > c0 ? my_b : c0, hi = my_e < cend ? my_e : cend;
     int32_t k = lo;
-    for (; k + kGather <= hi; k += kGather) {
-      float2 v[kGather], x[kGather];
+    for (; k + kGather32 <= hi; k += kGather32) {
+      float2 v[kGather32], x[kGather32];
 #pragma unroll
-      for (int u = 0; u < kGather; ++u) {
+      for (int u = 0; u < kGather32; ++u) {
         const uint32_t col = (uint32_t)s_ci[wave][k + u - c0];
         v[u] = s_v[wave][k + u - c0];
         x[u] = *reinterpret_cast<const float2*>(tcb + (col * 256u + lane_off));
       }
 #pragma unroll
-      for (int u = 0; u < kGather; ++u) sum.add(v[u], x[u]);
+      for (int u = 0; u < kGather32; ++u) sum.add(v[u], x[u]);
     }
     for (; k < hi; ++k) {
       const uint32_t col = (uint32_t)s_ci[wave][k - c0];

@@ -39,7 +39,7 @@ struct ExpTerms {
 
 __device__ __forceinline__ double wsum64(double v) {
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  for (int m = 32; m >= 1; m >>= 1) v += lane_xor(v, m);
   return v;
 }
 

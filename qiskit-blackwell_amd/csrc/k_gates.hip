@@ -311,7 +311,29 @@ __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I 
   const bool live = w0 < n_work;
   const I w = live ? w0 : n_work - 1;
   const int lane = threadIdx.x & 63;
-  if (S == 2) {
+  if (S == 3) {
+    // two stream targets and a second group of four slots 2^11 slots away
+    // (qubit 12, not a target), as mode 3 of the 1Q kernel
+    const int ps = 11;
+    int a = lo - 1, b = hi - 1, c = ps;
+    if (c < b) { const int t2 = b; b = c; c = t2; }
+    if (b < a) { const int t2 = a; a = b; b = t2; }
+    const I ml = I(1) << (lo - 1), mh = I(1) << (hi - 1), mp = I(1) << ps;
+    const I f0 = ins0(ins0(ins0(w, a), b), c);
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const I g0 = f0 + (e ? mp : I(0)), g1 = g0 + ml, g2 = g0 + mh, g3 = g1 + mh;
+      const float4 v0 = ld4<NT>(st + g0), v1 = ld4<NT>(st + g1), v2 = ld4<NT>(st + g2), v3 = ld4<NT>(st + g3);
+      float2 a0 = lo2(v0), a1 = lo2(v1), a2 = lo2(v2), a3 = lo2(v3);
+      float2 b0 = hi2(v0), b1 = hi2(v1), b2 = hi2(v2), b3 = hi2(v3);
+      mat4(g, a0, a1, a2, a3);
+      mat4(g, b0, b1, b2, b3);
+      if (live) {
+        st4p<NT>(st + g0, cat4(a0, b0), tail); st4p<NT>(st + g1, cat4(a1, b1), tail);
+        st4p<NT>(st + g2, cat4(a2, b2), tail); st4p<NT>(st + g3, cat4(a3, b3), tail);
+      }
+    }
+  } else if (S == 2) {
     const I ml = I(1) << (lo - 1), mh = I(1) << (hi - 1);
     const I f0 = ins0(ins0(w, lo - 1), hi - 1), f1 = f0 + ml, f2 = f0 + mh, f3 = f1 + mh;
     const float4 v0 = ld4<NT>(st + f0), v1 = ld4<NT>(st + f1), v2 = ld4<NT>(st + f2), v3 = ld4<NT>(st + f3);
@@ -381,6 +403,9 @@ __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I 
   }
 }
 
+#ifndef QSIM_GATE2Q_X2
+#define QSIM_GATE2Q_X2 0
+#endif
 int gate2q_impl(void* stream, void* state, int n, const float2* gdev, const M4& gv, int q0,
                 int q1) {
   if (int rc = check_n(n)) return rc;
@@ -400,6 +425,12 @@ int gate2q_impl(void* stream, void* state, int n, const float2* gdev, const M4& 
     if (S == 2) hipLaunchKernelGGL((k_gate2q<uint64_t, 2, true>), grid, block, 0, s, (float4*)state, work, lo, hi, gdev, gv, po);
   } else if (nt_for_qubits(n)) {
     const uint32_t w = (uint32_t)work;
+    if (QSIM_GATE2Q_X2 && S == 2 && lo != 12 && hi != 12 && n >= 14) {
+      const uint32_t w3 = w / 2;
+      hipLaunchKernelGGL((k_gate2q<uint32_t, 3, true>), grid1d(ceil_div(w3, kThreads)), block, 0, s,
+                         (float4*)state, w3, lo, hi, gdev, gv, po);
+      return launch_status("apply_gate_2q");
+    }
     if (S == 0) hipLaunchKernelGGL((k_gate2q<uint32_t, 0, true>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv, po);
     if (S == 1) hipLaunchKernelGGL((k_gate2q<uint32_t, 1, true>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv, po);
     if (S == 2) hipLaunchKernelGGL((k_gate2q<uint32_t, 2, true>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv, po);

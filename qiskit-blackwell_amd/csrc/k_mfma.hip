@@ -294,7 +294,7 @@ __global__ __launch_bounds__(256) void k_block5w(float2* __restrict__ st, uint64
       float ar, ai, br, bi;   // operands of column tile 1 and column tile 2
       if (Q0) {
         const float sr = h ? x[s].x : x[s].z, si = h ? x[s].y : x[s].w;
-        const float rr = __shfl_xor(sr, 32, 64), ri = __shfl_xor(si, 32, 64);
+        const float rr = lane_xor(sr, 32), ri = lane_xor(si, 32);
         ar = h ? rr : x[s].x;
         ai = h ? ri : x[s].y;
         br = h ? x[s].z : rr;

@@ -30,7 +30,7 @@ constexpr int kScanSeg = 4096;     // leaf totals per scan segment
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  for (int m = 32; m >= 1; m >>= 1) v += lane_xor(v, m);
   return v;
 }
 

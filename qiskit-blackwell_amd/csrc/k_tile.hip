@@ -355,8 +355,8 @@ __device__ __forceinline__ void xchg_apply(float2 v[4], const float2 g[4], int l
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     float2 p;
-    p.x = __shfl_xor(v[k].x, MASK, 64);
-    p.y = __shfl_xor(v[k].y, MASK, 64);
+    p.x = lane_xor(v[k].x, MASK);
+    p.y = lane_xor(v[k].y, MASK);
     v[k] = crow2(ga, v[k], gb, p);
   }
 }

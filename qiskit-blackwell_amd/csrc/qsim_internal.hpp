@@ -177,13 +177,47 @@ __device__ __forceinline__ void st4(float4* p, float4 v) {
   else *p = v;
 }
 
-// Cross-lane exchange of a float4 with lane ^ m (m < 64).
+// Value of lane (lane ^ m) for a wave-uniform mask m < 64, with every lane
+// of the wave active.  Each set bit of m is one exchange step, and the
+// steps compose because lane -> lane ^ b permutations commute:
+//   xor 1, 2  DPP quad_perm [1,0,3,2] / [2,3,0,1]   (v_mov_b32_dpp)
+//   xor 8     DPP row_ror:8 (rotation by 8 within a row of 16 is xor 8)
+//   xor 4, 16 ds_swizzle bitmask mode (and 0x1f, xor mask), no LDS memory
+//   xor 32    v_permlane32_swap between the two 32-lane halves
+// None of them needs an address register or LDS traffic, unlike the
+// ds_bpermute that __shfl_xor lowers to.  A mask known at compile time
+// folds to its steps; a kernel argument takes uniform scalar branches.
+__device__ __forceinline__ int lane_xor_i(int x, int m) {
+  if (m & 1) x = __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);
+  if (m & 2) x = __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);
+  if (m & 4) x = __builtin_amdgcn_ds_swizzle(x, 0x101F);
+  if (m & 8) x = __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);
+  if (m & 16) x = __builtin_amdgcn_ds_swizzle(x, 0x401F);
+  if (m & 32) {
+    // r[0]: this register with lanes 32..63 taken from lanes 0..31 of the
+    // other operand; r[1]: the other operand with lanes 0..31 taken from
+    // lanes 32..63 of this one.  Both operands are x.
+    const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    x = __lane_id() < 32 ? (int)r[1] : (int)r[0];
+  }
+  return x;
+}
+__device__ __forceinline__ float lane_xor(float v, int m) {
+  return __int_as_float(lane_xor_i(__float_as_int(v), m));
+}
+__device__ __forceinline__ double lane_xor(double v, int m) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const int lo = lane_xor_i((int)(uint32_t)u, m), hi = lane_xor_i((int)(uint32_t)(u >> 32), m);
+  return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// Cross-lane exchange of a float4 with lane ^ m (m < 64, wave-uniform).
 __device__ __forceinline__ float4 shfl_xor4(float4 v, int m) {
   float4 r;
-  r.x = __shfl_xor(v.x, m, 64);
-  r.y = __shfl_xor(v.y, m, 64);
-  r.z = __shfl_xor(v.z, m, 64);
-  r.w = __shfl_xor(v.w, m, 64);
+  r.x = lane_xor(v.x, m);
+  r.y = lane_xor(v.y, m);
+  r.z = lane_xor(v.z, m);
+  r.w = lane_xor(v.w, m);
   return r;
 }
 

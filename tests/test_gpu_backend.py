@@ -379,7 +379,8 @@ def test_block_widths_and_width_tuner(sim):
     cc = sim.compile(qc)
     best, times = cc.tuned_block_width(sim.device)
     assert set(times) == {4, 5, 6} and best in ("blocks/4", "blocks", "blocks/6")
-    assert times[int(best.split("/")[1]) if "/" in best else 5] == min(times.values())
+    from blackwell_backend.executor import block_width
+    assert times[block_width(best)] == min(times.values())
     sv, _ = sim.run_statevector(qc, mode=best)
     check(ref, sv)
     with pytest.raises(ValueError):
