@@ -211,6 +211,16 @@ int64_t qsim_sample_workspace_bytes(int n_qubits);
  * Replaces sample, bwk/csrc/sample_sm120.cu:73-132 (FP32 CDF there).     */
 int qsim_sample(void* stream, const void* state, int n_qubits, const void* uniforms_dev,
                 int64_t n_shots, void* workspace, void* out_dev);
+/* qsim_sample without a workspace argument and without any allocation, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  No counterpart in
+ * the reference.  The sampling scratch is a static device array of 4 MiB
+ * per GPU (held until the process ends), so n_qubits <= 26 (QSIM_ERANGE
+ * above).  Same results as qsim_sample, bit for bit.  Calls that overlap on
+ * one GPU share that scratch and race: serialise them (one stream, or
+ * events).                                                               */
+int qsim_sample_noalloc(void* stream, const void* state, int n_qubits, const void* uniforms_dev,
+                        int64_t n_shots, void* out_dev);
 /* uniforms_dev[k] = (SplitMix64(seed + (k+1) * 0x9E3779B97F4A7C15) >> 11) * 2^-53,
  * k < count (device double[count]): the stream qsim_run_instances samples
  * from in-kernel, so qsim_sample can draw the same shots for a state in HBM.

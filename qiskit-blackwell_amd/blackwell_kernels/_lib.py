@@ -60,6 +60,7 @@ PROTOTYPES = {
     "qsim_renormalize": (_i, [_vp, _vp, _i, _vp]),
     "qsim_sample_workspace_bytes": (_i64, [_i]),
     "qsim_sample": (_i, [_vp, _vp, _i, _vp, _i64, _vp, _vp]),
+    "qsim_sample_noalloc": (_i, [_vp, _vp, _i, _vp, _i64, _vp]),
     "qsim_uniforms_splitmix": (_i, [_vp, _u64, _i64, _vp]),
     "qsim_sample_batch_workspace_bytes": (_i64, [_i64, _i64]),
     "qsim_sample_batch": (_i, [_vp, _vp, _vp, _i64, _vp]),

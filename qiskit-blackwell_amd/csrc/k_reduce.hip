@@ -21,6 +21,7 @@
 // (Rule: smallest i with cdf(i) > u * total; the reference returns the
 // smallest i with cdf[i] >= u on an FP32 CDF, bwk/csrc/sample_sm120.cu:48-69.)
 #include "qsim_internal.hpp"
+#include <mutex>
 
 namespace qsim {
 
@@ -513,6 +514,48 @@ __global__ __launch_bounds__(kThreads) void k_splitmix(uint64_t seed, int64_t co
   if (k < count) out[k] = splitmix_u(seed, (uint64_t)k);
 }
 
+// ------------------------------------------------------------ no-alloc sampling
+// Scratch of qsim_sample_noalloc: the sampling workspace (leaf totals, leaf
+// prefixes, segment sums, total) of a state of up to kNoAllocMaxQubits
+// qubits, as a file-scope device array.  Every GPU the library runs on holds
+// its own copy (4 MiB) until the process ends.  The host never passes the
+// array's name to a kernel: noalloc_scratch() looks up this device's
+// address once with hipGetSymbolAddress and hands that out.
+constexpr int kNoAllocMaxQubits = 26;
+constexpr uint64_t kNoAllocLeaves = (1ull << kNoAllocMaxQubits) / kLeaf;
+constexpr uint64_t kNoAllocDoubles = 2 * kNoAllocLeaves + kNoAllocLeaves / kScanSeg + 2;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];
+
+// Device address of `scratch` on the device that owns `stream` (the current
+// device for the null stream), looked up once per device with that device
+// current.
+static int noalloc_scratch(hipStream_t stream, void** out) {
+  int dev = 0;
+  hipError_t e = stream ? hipStreamGetDevice(stream, &dev) : hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(QSIM_EHIP, "no device for the stream: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(QSIM_ERANGE, "device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int prev = 0;
+    if ((e = hipGetDevice(&prev)) != hipSuccess)
+      return fail(QSIM_EHIP, "hipGetDevice: %s", hipGetErrorString(e));
+    if (prev != dev && (e = hipSetDevice(dev)) != hipSuccess)
+      return fail(QSIM_EHIP, "hipSetDevice(%d): %s", dev, hipGetErrorString(e));
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (prev != dev) hipSetDevice(prev);
+    if (e != hipSuccess || !p)
+      return fail(QSIM_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return QSIM_OK;
+}
+
 }  // namespace qsim
 
 using namespace qsim;
@@ -633,6 +676,22 @@ int qsim_sample(void* stream, const void* state, int n_qubits, const void* unifo
                      (const double*)uniforms_dev,
                      (uint64_t)n_shots, (int64_t*)out_dev);
   return launch_status("sample: search");
+}
+
+int qsim_sample_noalloc(void* stream, const void* state, int n_qubits, const void* uniforms_dev,
+                        int64_t n_shots, void* out_dev) {
+  if (int rc = check_n(n_qubits)) return rc;
+  if (n_qubits > kNoAllocMaxQubits)
+    return fail(QSIM_ERANGE, "qsim_sample_noalloc holds scratch for at most %d qubits (got %d); "
+                "use qsim_sample with a workspace", kNoAllocMaxQubits, n_qubits);
+  if (n_shots < 0) return fail(QSIM_EINVAL, "n_shots < 0");
+  if (n_shots == 0) return QSIM_OK;
+  if (!state || !uniforms_dev || !out_dev) return fail(QSIM_EINVAL, "NULL pointer argument");
+  if (qsim_sample_workspace_bytes(n_qubits) > (int64_t)sizeof(scratch))
+    return fail(QSIM_ERANGE, "sampling workspace of %d qubits exceeds the static scratch", n_qubits);
+  void* ws = nullptr;
+  if (int rc = noalloc_scratch(as_stream(stream), &ws)) return rc;
+  return qsim_sample(stream, state, n_qubits, uniforms_dev, n_shots, ws, out_dev);
 }
 
 // Layout (doubles): tot[n_leaves] | pre[n_leaves] | totals[n_states]
