@@ -415,6 +415,21 @@ def c5_bench(n=22, n_rhs=32, times=(1.0, 5.0, 10.0), seed=5, precision=1e-7):
         "max_abs_diff_vs_fp64_row_sums": (Yc.to(torch.complex128) -
                                           outs[times[0]].to(torch.complex128)).abs().max().item()}
     del Yc
+    # the opt-in two-pass term over the CSR split at bit n/2, at t = 1
+    if n_rhs == 32:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        h.split()
+        torch.cuda.synchronize()
+        split_build_s = time.perf_counter() - t0
+        Ys, infos = chebyshev_evolve_batch(X, h, times[0], precision, split=True)
+        Ks = infos["chebyshev_degree"]
+        res["split_term"] = {
+            "t": times[0], "K": Ks, "ms_per_term": round(infos["elapsed_s"] / Ks * 1e3, 4),
+            "csr_split_s": round(split_build_s, 4),
+            "max_abs_diff_vs_one_pass": (Ys.to(torch.complex128) -
+                                         outs[times[0]].to(torch.complex128)).abs().max().item()}
+        del Ys
     return {"workload": f"{n}-qubit open chain ZZ + (XX+YY) + X, seed {seed}, CSR complex64 "
                         f"nnz {h.nnz}, {n_rhs} RHS, Bessel tail < {precision:g}",
             "row_sums": "FP64 sums of exact FP32 products (accumulate='fp64')",

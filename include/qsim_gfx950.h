@@ -420,6 +420,35 @@ int qsim_cheb_spmm_step(void* stream, const void* row_ptr, const void* col_idx, 
                         int64_t n_rows, int n_rhs, double scale,
                         const void* t_cur, void* t_prev_next, void* acc, void* comp,
                         double c_re, double c_im, int first, int accumulate);
+/* Split of a CSR of n_rows = 2^n rows at bit split_bit (1..n-1): the
+ * entries with (row ^ col) < 2^split_bit form the low CSR (rows in natural
+ * order); the others form the high CSR, whose row p holds the entries of
+ * row rot(p) = ((p & (2^(n-split_bit) - 1)) << split_bit) | (p >> (n-split_bit)).
+ * Each row keeps its entry order.  Two calls with the same workspace:
+ * _count (synchronises `stream`, returns both nonzero counts, QSIM_ERANGE
+ * above INT32_MAX), then _fill into int32 row pointers [n_rows+1] and
+ * int32 / complex64 entries of those counts.  (new)                       */
+int64_t qsim_csr_split_workspace_bytes(int64_t n_rows);
+int qsim_csr_split_count(void* stream, const void* row_ptr, const void* col_idx, int64_t n_rows,
+                         int split_bit, void* workspace, int64_t* nnz_lo, int64_t* nnz_hi);
+int qsim_csr_split_fill(void* stream, const void* row_ptr, const void* col_idx, const void* values,
+                        int64_t n_rows, int split_bit, const void* workspace, void* rp_lo,
+                        void* ci_lo, void* val_lo, void* rp_hi, void* ci_hi, void* val_hi);
+/* qsim_cheb_spmm_step (QSIM_CHEB_ACC_FP64) over a split CSR, 32 RHS,
+ * n_rows <= 2^24, in two launches: w1 = A_low * t_cur in natural row order
+ * (rounded to complex64 into w1, a [n_rows][32] complex64 buffer), then
+ * A_high * t_cur + w1 and the recurrence and accumulation of
+ * qsim_cheb_spmm_step in the rotated order, so each pass gathers rows of
+ * t_cur from a neighbourhood of 2^split_bit (2^(n-split_bit)) rows.  Same
+ * arguments and results as qsim_cheb_spmm_step up to the rounding of w1.
+ * Opt-in: on a 22-qubit chain it fetches 10 GB per term against 24 GB but
+ * takes 4.5 ms against 4.3 ms for qsim_cheb_spmm_step.
+ * (new; the reference runs one SpMV per term: spmv_sm120.cu:108-142)      */
+int qsim_cheb_spmm_step_split(void* stream, const void* rp_lo, const void* ci_lo, const void* val_lo,
+                              const void* rp_hi, const void* ci_hi, const void* val_hi, int64_t n_rows,
+                              int split_bit, int n_rhs, double scale, const void* t_cur,
+                              void* t_prev_next, void* acc, void* comp, void* w1, double c_re,
+                              double c_im, int first);
 /* acc = c * t, comp = 0 (first Chebyshev term), n elements. (new)        */
 int qsim_cheb_init_acc(void* stream, void* acc, void* comp, const void* t, int64_t n,
                        double c_re, double c_im);

@@ -15,10 +15,20 @@ T_{k+1} = 2 H~ T_k - T_{k-1} written over T_{k-1}, and the Kahan/Neumaier
 accumulation acc += c_k T_{k+1} with its compensation vector.  Any number
 of right-hand sides runs in the same launch, stored RHS-contiguous.
 
+split=True runs each term as two launches over the CSR split at bit n/2
+(HamiltonianCSR.split, qsim_cheb_spmm_step_split; 32 right-hand sides,
+FP64 row sums): the entries that connect rows differing only in the low
+half of the index bits, in natural row order, then the others in an order
+where the high index bits vary fastest.  Each pass gathers T_k rows from a
+smaller neighbourhood, at the price of one more write and read of a
+partial-sum vector.  On a 22-qubit Heisenberg chain at 32 RHS the two
+passes take longer than the one-pass term, so the one-pass term is the
+default.
+
 Row sums of the SpMM (accumulate=):
   "fp64"   every FP32 x FP32 product is exact in FP64 and is summed with
            FP64 FMAs (default: the more accurate form, and the cheaper one on
-           CDNA4, where FP64 FMA issues at the FP32 rate);
+           CDNA4, where FP64 FMA issues at the non-packed FP32 rate);
   "fp32c"  TwoProduct + Neumaier compensated FP32, the reference's
            TwoProduct/Kahan row sum (bwk/csrc/spmv_sm120.cu:26-54).
 """
@@ -63,7 +73,7 @@ def chebyshev_coefficients(t, spectral_bound, precision=1e-10):
 ACCUMULATE = {"fp64": 0, "fp32c": 1}
 
 
-def chebyshev_evolve_batch(states, hamiltonian_csr, t, precision=1e-10, accumulate="fp64"):
+def chebyshev_evolve_batch(states, hamiltonian_csr, t, precision=1e-10, accumulate="fp64", split=False):
     """Evolve R states at once.  states: complex64 [R, 2^n] (or [2^n]) on the
     device.  Returns (evolved [R, 2^n] complex64, info)."""
     if accumulate not in ACCUMULATE:
@@ -89,15 +99,28 @@ def chebyshev_evolve_batch(states, hamiltonian_csr, t, precision=1e-10, accumula
     n_el = N * R
     c0 = complex(coeffs[0])
     call("qsim_cheb_init_acc", s, ptr(acc), ptr(comp), ptr(A), n_el, c0.real, c0.imag)
-    args = (ptr(h.row_ptr), ptr(h.col_idx), ptr(h.values), N, R, scale)
-    c1 = complex(coeffs[1])
-    call("qsim_cheb_spmm_step", s, *args, ptr(A), ptr(B), ptr(acc), ptr(comp), c1.real, c1.imag, 1,
-         mode)
+    use_split = bool(split)
+    if use_split and not (mode == ACCUMULATE["fp64"] and R == 32 and 4 <= N <= (1 << 24)):
+        raise ValueError("split=True needs accumulate='fp64', 32 states and 4..2^24 rows "
+                         f"(got {accumulate!r}, {R} states, {N} rows)")
+    if use_split:
+        sb, rp_lo, ci_lo, v_lo, rp_hi, ci_hi, v_hi = h.split()
+        w1 = torch.empty_like(A)
+        sargs = (ptr(rp_lo), ptr(ci_lo), ptr(v_lo), ptr(rp_hi), ptr(ci_hi), ptr(v_hi), N, sb, R, scale)
+
+        def term(cur_, prv_, ck, first):
+            call("qsim_cheb_spmm_step_split", s, *sargs, ptr(cur_), ptr(prv_), ptr(acc), ptr(comp),
+                 ptr(w1), ck.real, ck.imag, first)
+    else:
+        args = (ptr(h.row_ptr), ptr(h.col_idx), ptr(h.values), N, R, scale)
+
+        def term(cur_, prv_, ck, first):
+            call("qsim_cheb_spmm_step", s, *args, ptr(cur_), ptr(prv_), ptr(acc), ptr(comp),
+                 ck.real, ck.imag, first, mode)
+    term(A, B, complex(coeffs[1]), 1)
     cur, prv = B, A
     for k in range(2, K + 1):
-        ck = complex(coeffs[k])
-        call("qsim_cheb_spmm_step", s, *args, ptr(cur), ptr(prv), ptr(acc), ptr(comp),
-             ck.real, ck.imag, 0, mode)
+        term(cur, prv, complex(coeffs[k]), 0)
         cur, prv = prv, cur
     call("qsim_cheb_finish", s, ptr(prv), ptr(acc), ptr(comp), n_el)
     out = prv.transpose(0, 1).contiguous()
@@ -110,11 +133,12 @@ def chebyshev_evolve_batch(states, hamiltonian_csr, t, precision=1e-10, accumula
         "spectral_bound": h.spectral_bound,
         "truncation_error_bound": tail,
         "spmv_count": K,
-        "workspace_vectors": 4,
-        "workspace_bytes": 4 * n_el * 8,
+        "workspace_vectors": 5 if use_split else 4,
+        "workspace_bytes": (5 if use_split else 4) * n_el * 8,
         "elapsed_s": elapsed,
         "n_rhs": R,
         "accumulate": accumulate,
+        "split_term": bool(use_split),
     }
     return (out[0] if single else out), info
 

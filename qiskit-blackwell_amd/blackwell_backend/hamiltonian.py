@@ -130,6 +130,38 @@ class HamiltonianCSR:
     def memory_bytes(self):
         return self.row_ptr.nbytes + self.col_idx.nbytes + self.values.nbytes
 
+    def split(self, split_bit=None):
+        """The CSR split at bit s (default n // 2) for the two-pass 32-RHS
+        Chebyshev term (qsim_csr_split_count / _fill): (split_bit, row_ptr_lo,
+        col_idx_lo, values_lo, row_ptr_hi, col_idx_hi, values_hi) device
+        tensors, the high part's rows in the rotated order.  Built on the
+        device once and cached."""
+        import ctypes
+        from blackwell_kernels import _lib
+        s_bit = int(split_bit) if split_bit is not None else self.n_qubits // 2
+        cache = getattr(self, "_split", None)
+        if cache is not None and cache[0] == s_bit:
+            return cache
+        dev = self.row_ptr.device
+        N = self.n_rows
+        ws = torch.empty(int(_lib.load().qsim_csr_split_workspace_bytes(N)), dtype=torch.uint8, device=dev)
+        st = _lib.stream_handle(dev)
+        nlo, nhi = ctypes.c_int64(0), ctypes.c_int64(0)
+        _lib.call("qsim_csr_split_count", st, _lib.ptr(self.row_ptr), _lib.ptr(self.col_idx), N, s_bit,
+                  _lib.ptr(ws), ctypes.byref(nlo), ctypes.byref(nhi))
+        rp_lo = torch.empty(N + 1, dtype=torch.int32, device=dev)
+        rp_hi = torch.empty(N + 1, dtype=torch.int32, device=dev)
+        ci_lo = torch.empty(max(nlo.value, 1), dtype=torch.int32, device=dev)
+        ci_hi = torch.empty(max(nhi.value, 1), dtype=torch.int32, device=dev)
+        v_lo = torch.empty(max(nlo.value, 1), dtype=torch.complex64, device=dev)
+        v_hi = torch.empty(max(nhi.value, 1), dtype=torch.complex64, device=dev)
+        _lib.call("qsim_csr_split_fill", st, _lib.ptr(self.row_ptr), _lib.ptr(self.col_idx),
+                  _lib.ptr(self.values), N, s_bit, _lib.ptr(ws), _lib.ptr(rp_lo), _lib.ptr(ci_lo),
+                  _lib.ptr(v_lo), _lib.ptr(rp_hi), _lib.ptr(ci_hi), _lib.ptr(v_hi))
+        self._split = (s_bit, rp_lo, ci_lo[:nlo.value], v_lo[:nlo.value], rp_hi, ci_hi[:nhi.value],
+                       v_hi[:nhi.value])
+        return self._split
+
     def __repr__(self):
         return (f"HamiltonianCSR(n_qubits={self.n_qubits}, dim={self.n_rows}, nnz={self.nnz}, "
                 f"spectral_bound={self.spectral_bound:.4f}, memory={self.memory_bytes()/1024:.1f} KB)")

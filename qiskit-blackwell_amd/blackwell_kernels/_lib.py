@@ -85,6 +85,11 @@ PROTOTYPES = {
     "qsim_chebyshev_accum": (_i, [_vp, _vp, _vp, _i64, _d, _d]),
     "qsim_chebyshev_step_accum": (_i, [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d]),
     "qsim_cheb_spmm_step": (_i, [_vp, _vp, _vp, _vp, _i64, _i, _d, _vp, _vp, _vp, _vp, _d, _d, _i, _i]),
+    "qsim_csr_split_workspace_bytes": (_i64, [_i64]),
+    "qsim_csr_split_count": (_i, [_vp, _vp, _vp, _i64, _i, _vp, _pi64, _pi64]),
+    "qsim_csr_split_fill": (_i, [_vp, _vp, _vp, _vp, _i64, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "qsim_cheb_spmm_step_split": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i, _i, _d, _vp, _vp, _vp,
+                                       _vp, _vp, _d, _d, _i]),
     "qsim_cheb_init_acc": (_i, [_vp, _vp, _vp, _vp, _i64, _d, _d]),
     "qsim_cheb_finish": (_i, [_vp, _vp, _vp, _vp, _i64]),
 }

@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kThreads) void k_step_accum(const float2* __restric
 //    bytes of the gather itself.
 //  * Row sum, ACC = kAccF64: the product of two FP32 numbers is exact in FP64
 //    (24 + 24 bits < 53), so every product is converted and accumulated with
-//    FP64 FMAs.  CDNA4 issues FP64 FMA at the FP32 rate, so this costs 8 VALU
+//    FP64 FMAs.  CDNA4 issues FP64 FMA at the non-packed FP32 rate, so this costs 8 VALU
 //    operations per complex multiply-add against ~44 for the TwoProduct +
 //    Neumaier double-float form (ACC = kAccF32c), which caps a 22-qubit,
 //    32-RHS term at ~5 ms on VALU issue alone; the FP64 sum is also the more
@@ -139,13 +139,12 @@ constexpr int kAccF64 = 0;
 constexpr int kAccF32c = 1;
 constexpr int kStageNnz = 128;      // nonzeros of a wave's two rows staged per chunk
 constexpr int kWaves = kThreads / 64;
-#ifndef QSIM_SPMM_ORDER_SHIFT
-#define QSIM_SPMM_ORDER_SHIFT 9
-#endif
-#ifndef QSIM_SPMM32_GATHER
-#define QSIM_SPMM32_GATHER 4
-#endif
-constexpr int kGather32 = QSIM_SPMM32_GATHER;   // gathers in flight per lane
+// Chunks of 2^9 rows per XCD and 4 gathers in flight per lane: measured on
+// the 22-qubit chain at 32 RHS, shifts 0, 6, 12 and 19 (the last is one
+// contiguous eighth per XCD) are within 3 % of 9, and 8 gathers in flight
+// within 1 % of 4 (2 is 3 % slower).
+constexpr int kSpmmOrderShift = 9;
+constexpr int kGather32 = 4;   // gathers in flight per lane
 
 typedef float f32x2_t __attribute__((ext_vector_type(2)));
 template <bool NT>
@@ -219,16 +218,39 @@ struct RowSum<kAccF32c> {
   }
 };
 
-template <int ACC, bool NT>
+// Split term (PASS 1 and 2).  For a matrix of N = 2^n rows whose entries
+// mostly connect rows that differ either only below bit s or only at and
+// above it (a Pauli Hamiltonian's X masks), the SpMM is done in two
+// passes, each of whose gathers stay within 2^s (2^(n-s)) neighbouring rows
+// of the order it walks, so the gathered 256-byte rows of T_k are reused
+// from L2 instead of being fetched again for every row that needs them:
+//   PASS 1  rows in natural order, the entries with (row ^ col) < 2^s:
+//           w1 = (A_low T_k) rounded to complex64;
+//   PASS 2  positions p in order, row r = rot(p) (the low n-s bits of p are
+//           the high bits of r), the other entries, stored in p order:
+//           w = w1 + A_high T_k (FP64), then the same recurrence epilogue.
+// Entries whose mask has bits on both sides of s (at most a few per row)
+// go to pass 2 and are gathered from farther away.  On a 22-qubit chain at
+// 32 RHS this cuts the bytes fetched per term from 24 GB to 10 GB, yet the
+// two passes take 4.5 ms against 4.3 ms for PASS 0: the number of gather
+// instructions and their latency, which the split leaves as they are, bound
+// both forms.  Callers opt in to it.
+__device__ __forceinline__ int64_t split_row(int64_t p, int n, int s) {
+  const int lo_bits = n - s;
+  return ((p & ((int64_t(1) << lo_bits) - 1)) << s) | (p >> lo_bits);
+}
+
+template <int ACC, bool NT, int PASS>
 __global__ __launch_bounds__(kThreads) void k_cheb_spmm32(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, const float2* __restrict__ val,
     int64_t n_rows, double scale, const float2* __restrict__ tc, float2* __restrict__ tpn,
-    float2* __restrict__ acc, float2* __restrict__ comp, float2 c, int first, int order_shift) {
+    float2* __restrict__ acc, float2* __restrict__ comp, float2* __restrict__ w1, float2 c,
+    int first, int order_shift, int n_bits, int split) {
   __shared__ int32_t s_ci[kWaves][kStageNnz];
   __shared__ float2 s_v[kWaves][kStageNnz];
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63, h = lane >> 5, j = lane & 31;
-  const int64_t row0 = (spmm_block(order_shift) * kWaves + wave) * 2;
+  const int64_t row0 = (spmm_block(order_shift) * kWaves + wave) * 2;   // CSR row (position)
   if (row0 >= n_rows) return;
   const bool has1 = row0 + 1 < n_rows;
   const int32_t b0 = rp[row0], b1 = rp[row0 + 1], b2 = has1 ? rp[row0 + 2] : b1;
@@ -263,7 +285,18 @@ __global__ __launch_bounds__(kThreads) void k_cheb_spmm32(
     wave_sync();
   }
   if (h && !has1) return;
-  const int64_t e_idx = (row0 + h) * 32 + j;
+  if constexpr (PASS == 1) {
+    // partial row sum of the low entries, rounded once
+    st2<NT>(w1 + (row0 + h) * 32 + j, make_float2((float)sum.r, (float)sum.i));
+    return;
+  }
+  const int64_t row = PASS == 2 ? split_row(row0 + h, n_bits, split) : row0 + h;
+  const int64_t e_idx = row * 32 + j;
+  if constexpr (PASS == 2) {
+    const float2 w = ld2<NT>(w1 + e_idx);
+    sum.r += (double)w.x;
+    sum.i += (double)w.y;
+  }
   const float2 tp = first ? make_float2(0.f, 0.f) : ld2<NT>(tpn + e_idx);
   const float2 tn = sum.next(scale, tp, first != 0);
   st2<NT>(tpn + e_idx, tn);
@@ -273,6 +306,117 @@ __global__ __launch_bounds__(kThreads) void k_cheb_spmm32(
   neu_add(a.y, k2.y, v.y);
   st2<NT>(acc + e_idx, a);
   st2<NT>(comp + e_idx, k2);
+}
+
+// ---------------------------------------------------------------- CSR split
+// Splits a CSR of N = 2^n rows at bit s: entries with (row ^ col) < 2^s go
+// to the low CSR (rows in natural order), the others to the high CSR, whose
+// row p is row rot(p) of the input (split_row).  Both keep each row's entry
+// order.  One row (position) per thread: k_split_count sums the per-block
+// counts of both outputs, k_split_scan (one workgroup) turns them into block
+// offsets, k_split_fill writes row pointers, columns and values.
+constexpr int kSplitThreads = 256;
+
+__device__ __forceinline__ int64_t blk_exscan(int64_t v, int64_t* total) {
+  __shared__ int64_t wsum[kSplitThreads / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int64_t incl = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int64_t y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  int64_t base = 0, all = 0;
+  for (int k = 0; k < kSplitThreads / 64; ++k) {
+    if (k < w) base += wsum[k];
+    all += wsum[k];
+  }
+  __syncthreads();
+  *total = all;
+  return base + incl - v;
+}
+
+__device__ __forceinline__ void split_counts(const int32_t* __restrict__ rp,
+                                             const int32_t* __restrict__ ci, int64_t i, int n,
+                                             int s, int& nlo, int& nhi) {
+  nlo = 0;
+  for (int32_t k = rp[i]; k < rp[i + 1]; ++k) nlo += ((uint32_t)(ci[k] ^ (int32_t)i) >> s) == 0;
+  const int64_t r = split_row(i, n, s);
+  nhi = 0;
+  for (int32_t k = rp[r]; k < rp[r + 1]; ++k) nhi += ((uint32_t)(ci[k] ^ (int32_t)r) >> s) != 0;
+}
+
+__global__ __launch_bounds__(kSplitThreads) void k_split_count(const int32_t* __restrict__ rp,
+                                                               const int32_t* __restrict__ ci,
+                                                               int64_t n_rows, int n, int s,
+                                                               int64_t* __restrict__ bsum) {
+  const int64_t i = (int64_t)blockIdx.x * kSplitThreads + threadIdx.x;
+  int nlo = 0, nhi = 0;
+  if (i < n_rows) split_counts(rp, ci, i, n, s, nlo, nhi);
+  int64_t tl, th;
+  blk_exscan(nlo, &tl);
+  blk_exscan(nhi, &th);
+  if (threadIdx.x == 0) {
+    bsum[2 * blockIdx.x] = tl;
+    bsum[2 * blockIdx.x + 1] = th;
+  }
+}
+
+__global__ __launch_bounds__(kSplitThreads) void k_split_scan(uint64_t n_blocks,
+                                                              int64_t* __restrict__ bsum,
+                                                              int64_t* __restrict__ nnz2) {
+  int64_t carry[2] = {0, 0};
+  for (uint64_t base = 0; base < n_blocks; base += kSplitThreads) {
+    const uint64_t b = base + threadIdx.x;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int64_t v = b < n_blocks ? bsum[2 * b + side] : 0;
+      int64_t total;
+      const int64_t pre = blk_exscan(v, &total);
+      if (b < n_blocks) bsum[2 * b + side] = carry[side] + pre;
+      carry[side] += total;
+    }
+  }
+  if (threadIdx.x == 0) {
+    nnz2[0] = carry[0];
+    nnz2[1] = carry[1];
+  }
+}
+
+__global__ __launch_bounds__(kSplitThreads) void k_split_fill(
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, const float2* __restrict__ val,
+    int64_t n_rows, int n, int s, const int64_t* __restrict__ boff, int32_t* __restrict__ rp_lo,
+    int32_t* __restrict__ ci_lo, float2* __restrict__ val_lo, int32_t* __restrict__ rp_hi,
+    int32_t* __restrict__ ci_hi, float2* __restrict__ val_hi) {
+  const int64_t i = (int64_t)blockIdx.x * kSplitThreads + threadIdx.x;
+  int nlo = 0, nhi = 0;
+  if (i < n_rows) split_counts(rp, ci, i, n, s, nlo, nhi);
+  int64_t tl, th;
+  const int64_t olo = boff[2 * blockIdx.x] + blk_exscan(nlo, &tl);
+  const int64_t ohi = boff[2 * blockIdx.x + 1] + blk_exscan(nhi, &th);
+  if (i >= n_rows) return;
+  rp_lo[i] = (int32_t)olo;
+  rp_hi[i] = (int32_t)ohi;
+  if (i == n_rows - 1) {
+    rp_lo[n_rows] = (int32_t)(olo + nlo);
+    rp_hi[n_rows] = (int32_t)(ohi + nhi);
+  }
+  int64_t o = olo;
+  for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
+    if (((uint32_t)(ci[k] ^ (int32_t)i) >> s) == 0) {
+      ci_lo[o] = ci[k];
+      val_lo[o] = val[k];
+      ++o;
+    }
+  const int64_t r = split_row(i, n, s);
+  o = ohi;
+  for (int32_t k = rp[r]; k < rp[r + 1]; ++k)
+    if (((uint32_t)(ci[k] ^ (int32_t)r) >> s) != 0) {
+      ci_hi[o] = ci[k];
+      val_hi[o] = val[k];
+      ++o;
+    }
 }
 
 // Fused batched Chebyshev term for any number of right-hand sides (and for
@@ -335,6 +479,20 @@ __global__ __launch_bounds__(kThreads) void k_cheb_finish(float2* __restrict__ o
   if (i >= n) return;
   const float2 a = acc[i], k = comp[i];
   out[i] = make_float2(a.x + k.x, a.y + k.y);
+}
+
+// Order shift for a grid of `blocks` 32-RHS workgroups (8 rows each): the
+// requested shift when the chunks tile the grid exactly, else 0.
+static int spmm_order(int64_t blocks, int shift) {
+  const int64_t chunk = shift > 3 ? int64_t(1) << (shift - 3) : 1;
+  return blocks % (8 * chunk) == 0 ? shift : 0;
+}
+
+static int log2_exact(int64_t v) {
+  if (v <= 0 || (v & (v - 1))) return -1;
+  int n = 0;
+  while ((int64_t(1) << n) < v) ++n;
+  return n;
 }
 
 static inline float2 f2(double re, double im) { return make_float2((float)re, (float)im); }
@@ -404,15 +562,14 @@ int qsim_cheb_spmm_step(void* stream, const void* row_ptr, const void* col_idx, 
   // the 32-RHS kernel addresses T_k with 32-bit byte offsets (<= 2^24 rows)
   if (n_rhs == 32 && n_rows <= (int64_t(1) << 24)) {
     const int64_t blocks = (int64_t)ceil_div((uint64_t)n_rows, 2 * kWaves);
-    const int chunk_blocks = QSIM_SPMM_ORDER_SHIFT > 3 ? 1 << (QSIM_SPMM_ORDER_SHIFT - 3) : 1;
-    const int order = blocks % (8 * (int64_t)chunk_blocks) == 0 ? QSIM_SPMM_ORDER_SHIFT : 0;
+    const int order = spmm_order(blocks, kSpmmOrderShift);
     const bool nt = nt_for_bytes((uint64_t)n_rows * 256u);
-    auto* kern = f64 ? (nt ? k_cheb_spmm32<kAccF64, true> : k_cheb_spmm32<kAccF64, false>)
-                     : (nt ? k_cheb_spmm32<kAccF32c, true> : k_cheb_spmm32<kAccF32c, false>);
+    auto* kern = f64 ? (nt ? k_cheb_spmm32<kAccF64, true, 0> : k_cheb_spmm32<kAccF64, false, 0>)
+                     : (nt ? k_cheb_spmm32<kAccF32c, true, 0> : k_cheb_spmm32<kAccF32c, false, 0>);
     hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kThreads), 0, s, (const int32_t*)row_ptr,
                        (const int32_t*)col_idx, (const float2*)values, n_rows, scale,
-                       (const float2*)t_cur, (float2*)t_prev_next, (float2*)acc, (float2*)comp, cf,
-                       first, order);
+                       (const float2*)t_cur, (float2*)t_prev_next, (float2*)acc, (float2*)comp,
+                       (float2*)nullptr, cf, first, order, 0, 0);
     return launch_status("cheb_spmm_step");
   }
   const int64_t total = n_rows * (int64_t)n_rhs;
@@ -441,6 +598,93 @@ int qsim_cheb_finish(void* stream, void* out, const void* acc, const void* comp,
   hipLaunchKernelGGL(k_cheb_finish, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0,
                      as_stream(stream), (float2*)out, (const float2*)acc, (const float2*)comp, n);
   return launch_status("cheb_finish");
+}
+
+int64_t qsim_csr_split_workspace_bytes(int64_t n_rows) {
+  if (n_rows < 1) return -1;
+  const int64_t blocks = (int64_t)ceil_div((uint64_t)n_rows, kSplitThreads);
+  return (2 * blocks + 2) * (int64_t)sizeof(int64_t);
+}
+
+static int check_split(int64_t n_rows, int split_bit, int& n) {
+  n = log2_exact(n_rows);
+  if (n < 2 || n_rows > (int64_t(1) << 30))
+    return fail(QSIM_EINVAL, "the CSR split needs 2^n rows, 4 <= rows <= 2^30 (got %lld)", (long long)n_rows);
+  if (split_bit < 1 || split_bit >= n)
+    return fail(QSIM_EINVAL, "split_bit %d outside 1..%d", split_bit, n - 1);
+  return QSIM_OK;
+}
+
+int qsim_csr_split_count(void* stream, const void* row_ptr, const void* col_idx, int64_t n_rows,
+                         int split_bit, void* workspace, int64_t* nnz_lo, int64_t* nnz_hi) {
+  int n = 0;
+  if (int rc = check_split(n_rows, split_bit, n)) return rc;
+  if (!row_ptr || !col_idx || !workspace || !nnz_lo || !nnz_hi) return fail(QSIM_EINVAL, "NULL pointer argument");
+  hipStream_t s = as_stream(stream);
+  const uint64_t blocks = ceil_div((uint64_t)n_rows, kSplitThreads);
+  int64_t* bsum = (int64_t*)workspace;
+  int64_t* tot = bsum + 2 * blocks;
+  hipLaunchKernelGGL(k_split_count, dim3((unsigned)blocks), dim3(kSplitThreads), 0, s,
+                     (const int32_t*)row_ptr, (const int32_t*)col_idx, n_rows, n, split_bit, bsum);
+  if (int rc = launch_status("csr_split: count")) return rc;
+  hipLaunchKernelGGL(k_split_scan, dim3(1), dim3(kSplitThreads), 0, s, blocks, bsum, tot);
+  if (int rc = launch_status("csr_split: scan")) return rc;
+  int64_t h[2] = {0, 0};
+  hipError_t e = hipMemcpyAsync(h, tot, sizeof h, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(QSIM_EHIP, "csr_split: %s", hipGetErrorString(e));
+  if (h[0] > INT32_MAX || h[1] > INT32_MAX) return fail(QSIM_ERANGE, "split CSR beyond int32 nonzeros");
+  *nnz_lo = h[0];
+  *nnz_hi = h[1];
+  return QSIM_OK;
+}
+
+int qsim_csr_split_fill(void* stream, const void* row_ptr, const void* col_idx, const void* values,
+                        int64_t n_rows, int split_bit, const void* workspace, void* rp_lo,
+                        void* ci_lo, void* val_lo, void* rp_hi, void* ci_hi, void* val_hi) {
+  int n = 0;
+  if (int rc = check_split(n_rows, split_bit, n)) return rc;
+  if (!row_ptr || !col_idx || !values || !workspace || !rp_lo || !ci_lo || !val_lo || !rp_hi ||
+      !ci_hi || !val_hi)
+    return fail(QSIM_EINVAL, "NULL pointer argument");
+  const uint64_t blocks = ceil_div((uint64_t)n_rows, kSplitThreads);
+  hipLaunchKernelGGL(k_split_fill, dim3((unsigned)blocks), dim3(kSplitThreads), 0, as_stream(stream),
+                     (const int32_t*)row_ptr, (const int32_t*)col_idx, (const float2*)values, n_rows, n,
+                     split_bit, (const int64_t*)workspace, (int32_t*)rp_lo, (int32_t*)ci_lo,
+                     (float2*)val_lo, (int32_t*)rp_hi, (int32_t*)ci_hi, (float2*)val_hi);
+  return launch_status("csr_split: fill");
+}
+
+int qsim_cheb_spmm_step_split(void* stream, const void* rp_lo, const void* ci_lo, const void* val_lo,
+                              const void* rp_hi, const void* ci_hi, const void* val_hi, int64_t n_rows,
+                              int split_bit, int n_rhs, double scale, const void* t_cur,
+                              void* t_prev_next, void* acc, void* comp, void* w1, double c_re,
+                              double c_im, int first) {
+  int n = 0;
+  if (int rc = check_split(n_rows, split_bit, n)) return rc;
+  if (n_rhs != 32) return fail(QSIM_EINVAL, "the split term runs 32 right-hand sides (got %d)", n_rhs);
+  if (n_rows > (int64_t(1) << 24)) return fail(QSIM_ERANGE, "the split term handles at most 2^24 rows");
+  if (!rp_lo || !ci_lo || !val_lo || !rp_hi || !ci_hi || !val_hi || !t_cur || !t_prev_next || !acc ||
+      !comp || !w1)
+    return fail(QSIM_EINVAL, "NULL pointer argument");
+  hipStream_t s = as_stream(stream);
+  const float2 cf = f2(c_re, c_im);
+  const int64_t blocks = (int64_t)ceil_div((uint64_t)n_rows, 2 * kWaves);
+  // each XCD walks one contiguous eighth of the rows (positions) in both passes
+  const int order = spmm_order(blocks, n >= 3 ? n - 3 : 0);
+  const bool nt = nt_for_bytes((uint64_t)n_rows * 256u);
+  auto* k1 = nt ? k_cheb_spmm32<kAccF64, true, 1> : k_cheb_spmm32<kAccF64, false, 1>;
+  auto* k2 = nt ? k_cheb_spmm32<kAccF64, true, 2> : k_cheb_spmm32<kAccF64, false, 2>;
+  hipLaunchKernelGGL(k1, dim3((unsigned)blocks), dim3(kThreads), 0, s, (const int32_t*)rp_lo,
+                     (const int32_t*)ci_lo, (const float2*)val_lo, n_rows, scale, (const float2*)t_cur,
+                     (float2*)t_prev_next, (float2*)acc, (float2*)comp, (float2*)w1, cf, first, order,
+                     n, split_bit);
+  if (int rc = launch_status("cheb_spmm_step_split: low pass")) return rc;
+  hipLaunchKernelGGL(k2, dim3((unsigned)blocks), dim3(kThreads), 0, s, (const int32_t*)rp_hi,
+                     (const int32_t*)ci_hi, (const float2*)val_hi, n_rows, scale, (const float2*)t_cur,
+                     (float2*)t_prev_next, (float2*)acc, (float2*)comp, (float2*)w1, cf, first, order,
+                     n, split_bit);
+  return launch_status("cheb_spmm_step_split: high pass");
 }
 
 }  // extern "C"

@@ -155,7 +155,8 @@ __global__ __launch_bounds__(kThreads) void k_gate1q(float4* __restrict__ st, I 
     // (n_work is a power of two larger than 2^(t-1)).
     const float4 v = ld4<NT>(st + w);
     const int m = 1 << (t - 1);
-    const float4 p = shfl_xor4(v, m);
+    float4 p;
+    with_lane_bit(m, [&](auto M) { p = xor4c<decltype(M)::value>(v); });
     const bool low = (threadIdx.x & m) == 0;
     const float2 ga = low ? g[0] : g[3], gb = low ? g[1] : g[2];
     const float4 r = cat4(crow2(ga, lo2(v), gb, lo2(p)), crow2(ga, hi2(v), gb, hi2(p)));
@@ -311,29 +312,7 @@ __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I 
   const bool live = w0 < n_work;
   const I w = live ? w0 : n_work - 1;
   const int lane = threadIdx.x & 63;
-  if (S == 3) {
-    // two stream targets and a second group of four slots 2^11 slots away
-    // (qubit 12, not a target), as mode 3 of the 1Q kernel
-    const int ps = 11;
-    int a = lo - 1, b = hi - 1, c = ps;
-    if (c < b) { const int t2 = b; b = c; c = t2; }
-    if (b < a) { const int t2 = a; a = b; b = t2; }
-    const I ml = I(1) << (lo - 1), mh = I(1) << (hi - 1), mp = I(1) << ps;
-    const I f0 = ins0(ins0(ins0(w, a), b), c);
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const I g0 = f0 + (e ? mp : I(0)), g1 = g0 + ml, g2 = g0 + mh, g3 = g1 + mh;
-      const float4 v0 = ld4<NT>(st + g0), v1 = ld4<NT>(st + g1), v2 = ld4<NT>(st + g2), v3 = ld4<NT>(st + g3);
-      float2 a0 = lo2(v0), a1 = lo2(v1), a2 = lo2(v2), a3 = lo2(v3);
-      float2 b0 = hi2(v0), b1 = hi2(v1), b2 = hi2(v2), b3 = hi2(v3);
-      mat4(g, a0, a1, a2, a3);
-      mat4(g, b0, b1, b2, b3);
-      if (live) {
-        st4p<NT>(st + g0, cat4(a0, b0), tail); st4p<NT>(st + g1, cat4(a1, b1), tail);
-        st4p<NT>(st + g2, cat4(a2, b2), tail); st4p<NT>(st + g3, cat4(a3, b3), tail);
-      }
-    }
-  } else if (S == 2) {
+  if (S == 2) {
     const I ml = I(1) << (lo - 1), mh = I(1) << (hi - 1);
     const I f0 = ins0(ins0(w, lo - 1), hi - 1), f1 = f0 + ml, f2 = f0 + mh, f3 = f1 + mh;
     const float4 v0 = ld4<NT>(st + f0), v1 = ld4<NT>(st + f1), v2 = ld4<NT>(st + f2), v3 = ld4<NT>(st + f3);
@@ -361,7 +340,11 @@ __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I 
       // permuted to match once per lane, so no per-amplitude select remains.
       const int m = 1 << (lo - 1);
       const bool bl = (lane & m) != 0;
-      const float4 p0 = shfl_xor4(v0, m), p1 = shfl_xor4(v1, m);
+      float4 p0, p1;
+      with_lane_bit(m, [&](auto M) {
+        p0 = xor4c<decltype(M)::value>(v0);
+        p1 = xor4c<decltype(M)::value>(v1);
+      });
       float2 ha[4], hb[4];
       perm_rows2(g, bl, ha, hb);
       float2 y[2][2];
@@ -383,7 +366,7 @@ __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I 
     float4 o;
     if (lo == 0) {
       // x = (lo2(v), hi2(v), lo2(p), hi2(p)) have columns k ^ 2*bh
-      const float4 p = shfl_xor4(v, mh);
+      const float4 p = shfl_xor4_lds(v, mh);
       float2 ha[4], hb[4];
       perm_rows0(g, bh, ha, hb);
       o = cat4(crow4(ha, lo2(v), hi2(v), lo2(p), hi2(p)),
@@ -393,7 +376,7 @@ __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I 
       // r = 2*bh + bl the row of the amplitudes this lane holds
       const int ml = 1 << (lo - 1);
       const bool bl = (lane & ml) != 0;
-      const float4 pl = shfl_xor4(v, ml), ph = shfl_xor4(v, mh), pb = shfl_xor4(v, ml | mh);
+      const float4 pl = shfl_xor4_lds(v, ml), ph = shfl_xor4_lds(v, mh), pb = shfl_xor4_lds(v, ml | mh);
       float2 h[4];
       perm_row(g, bh, bl, h);
       o = cat4(crow4(h, lo2(v), lo2(pl), lo2(ph), lo2(pb)),
@@ -403,9 +386,6 @@ __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I 
   }
 }
 
-#ifndef QSIM_GATE2Q_X2
-#define QSIM_GATE2Q_X2 0
-#endif
 int gate2q_impl(void* stream, void* state, int n, const float2* gdev, const M4& gv, int q0,
                 int q1) {
   if (int rc = check_n(n)) return rc;
@@ -425,12 +405,6 @@ int gate2q_impl(void* stream, void* state, int n, const float2* gdev, const M4& 
     if (S == 2) hipLaunchKernelGGL((k_gate2q<uint64_t, 2, true>), grid, block, 0, s, (float4*)state, work, lo, hi, gdev, gv, po);
   } else if (nt_for_qubits(n)) {
     const uint32_t w = (uint32_t)work;
-    if (QSIM_GATE2Q_X2 && S == 2 && lo != 12 && hi != 12 && n >= 14) {
-      const uint32_t w3 = w / 2;
-      hipLaunchKernelGGL((k_gate2q<uint32_t, 3, true>), grid1d(ceil_div(w3, kThreads)), block, 0, s,
-                         (float4*)state, w3, lo, hi, gdev, gv, po);
-      return launch_status("apply_gate_2q");
-    }
     if (S == 0) hipLaunchKernelGGL((k_gate2q<uint32_t, 0, true>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv, po);
     if (S == 1) hipLaunchKernelGGL((k_gate2q<uint32_t, 1, true>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv, po);
     if (S == 2) hipLaunchKernelGGL((k_gate2q<uint32_t, 2, true>), grid, block, 0, s, (float4*)state, w, lo, hi, gdev, gv, po);
@@ -758,7 +732,8 @@ __global__ __launch_bounds__(kThreads) void k_mc1(float4* __restrict__ st, I n_i
     const bool ld = FULL || act;
     const float4 v = ld ? ld4<NT>(st + s0) : make_float4(0.f, 0.f, 0.f, 0.f);
     const int m = 1 << (a.t - 1);
-    const float4 p = shfl_xor4(v, m);
+    float4 p;
+    with_lane_bit(m, [&](auto M) { p = xor4c<decltype(M)::value>(v); });
     if (!ld) return;
     float4 r;
     if (OP == 1) {

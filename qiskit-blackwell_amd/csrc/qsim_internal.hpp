@@ -14,6 +14,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <atomic>
+#include <type_traits>
 #include <stdint.h>
 #include <stdarg.h>
 #include "qsim_gfx950.h"
@@ -211,14 +212,52 @@ __device__ __forceinline__ double lane_xor(double v, int m) {
   return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
 }
 
-// Cross-lane exchange of a float4 with lane ^ m (m < 64, wave-uniform).
-__device__ __forceinline__ float4 shfl_xor4(float4 v, int m) {
+// Cross-lane exchange of a float4 with lane ^ M for a compile-time mask.
+template <int M>
+__device__ __forceinline__ float4 xor4c(float4 v) {
   float4 r;
-  r.x = lane_xor(v.x, m);
-  r.y = lane_xor(v.y, m);
-  r.z = lane_xor(v.z, m);
-  r.w = lane_xor(v.w, m);
+  r.x = lane_xor(v.x, M);
+  r.y = lane_xor(v.y, M);
+  r.z = lane_xor(v.z, M);
+  r.w = lane_xor(v.w, M);
   return r;
+}
+// Calls f(std::integral_constant<int, M>) for the wave-uniform single-bit
+// lane mask m (1, 2, ..., 32) known only at run time: one uniform branch
+// selects a body in which the exchanges have compile-time masks, so every
+// exchange inside it is straight-line DPP / swizzle / permlane code.
+template <class F>
+__device__ __forceinline__ void with_lane_bit(int m, F&& f) {
+  switch (m) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    default: f(std::integral_constant<int, 32>{}); break;
+  }
+}
+// Cross-lane exchange of a float4 through ds_bpermute (the LDS crossbar,
+// no LDS memory): one address for the four components, and no VALU work.
+// Used where the VALU is the scarcer pipe: the 4x4 gate with both targets
+// below 7 exchanges three times per slot, and there DPP moves measured
+// 2.5-5 % slower at 30 qubits.
+__device__ __forceinline__ float4 shfl_xor4_lds(float4 v, int m) {
+  float4 r;
+  r.x = __shfl_xor(v.x, m, 64);
+  r.y = __shfl_xor(v.y, m, 64);
+  r.z = __shfl_xor(v.z, m, 64);
+  r.w = __shfl_xor(v.w, m, 64);
+  return r;
+}
+// Cross-lane exchange of a float4 with lane ^ m, m < 64 wave-uniform and
+// known only at run time: a single-bit mask takes one uniform branch; a
+// mask of several bits is exchanged one bit at a time.
+__device__ __forceinline__ float4 shfl_xor4(float4 v, int m) {
+#pragma unroll
+  for (int b = 1; b < 64; b <<= 1)
+    if (m & b) with_lane_bit(b, [&](auto M) { v = xor4c<decltype(M)::value>(v); });
+  return v;
 }
 
 // Flat block index of a 1D grid laid out as grid1d() does (gridDim.y rows of

@@ -385,3 +385,35 @@ def test_block_widths_and_width_tuner(sim):
     check(ref, sv)
     with pytest.raises(ValueError):
         sim.run_statevector(W.qft(12), mode="blocks/6")       # k = 6 needs n >= 13
+
+
+def test_chebyshev_split_term_16q_matches_fp64():
+    """At 16 qubits and 32 RHS, split=True runs each Chebyshev term in two
+    passes (HamiltonianCSR.split); the result matches the FP64 Chebyshev oracle
+    (fidelity >= 1 - 1e-7, max |d amp| <= 1e-5 max |a|) and the one-pass term
+    within 1e-6."""
+    from blackwell_backend import workloads as W
+    from blackwell_backend.chebyshev import chebyshev_evolve_batch
+    from blackwell_backend.hamiltonian import HamiltonianCSR
+    from oracle import hamiltonian as OH
+    n, R = 16, 32
+    op = W.heisenberg_chain(n, 11)
+    h = HamiltonianCSR.from_sparse_pauli_op(op, device="cuda:0")
+    rng = np.random.default_rng(12)
+    X = (rng.standard_normal((R, 1 << n)) + 1j * rng.standard_normal((R, 1 << n))).astype(np.complex64)
+    X /= np.linalg.norm(X, axis=1, keepdims=True).astype(np.float32)
+    Xd = torch.from_numpy(X).cuda()
+    Y, info = chebyshev_evolve_batch(Xd, h, 2.0, 1e-8, split=True)
+    assert info["split_term"]
+    Y1, info1 = chebyshev_evolve_batch(Xd, h, 2.0, 1e-8)
+    assert not info1["split_term"]
+    with pytest.raises(ValueError):
+        chebyshev_evolve_batch(Xd[:4], h, 2.0, 1e-8, split=True)
+    got, one = Y.cpu().numpy(), Y1.cpu().numpy()
+    assert np.max(np.abs(got - one)) <= 1e-6
+    ref, _ = OH.chebyshev_evolve_c(op.labels, op.coeffs, n, X[:4].astype(np.complex128), 2.0,
+                                   h.spectral_bound, precision=1e-13)
+    for k in range(4):
+        f = O.fidelity(ref[k], got[k])
+        d = float(np.max(np.abs(ref[k] - got[k])))
+        assert f >= 1 - 1e-7 and d <= 1e-5 * np.max(np.abs(ref[k])), (k, f, d)

@@ -572,6 +572,88 @@ def test_cheb_spmm_step_matches_fp64(bk, dev, accumulate):
                 assert np.array_equal(both.view(np.uint32), tpd.cpu().numpy().view(np.uint32)), first
 
 
+def test_csr_split_and_split_term(bk, dev):
+    """qsim_csr_split_count / _fill split a ragged CSR of 2^10 rows at bit 4:
+    the low part holds exactly the entries with (row ^ col) < 16 of each
+    row, the high part (row p = rot(p)) the others, both in the original
+    entry order.  The two-pass term on that split (32 RHS) equals the
+    one-pass FP64 term up to the complex64 rounding of the low partial sum,
+    for the first term and a later one."""
+    import ctypes
+    import scipy.sparse as sp
+    from blackwell_kernels._lib import call, ptr, stream_handle
+    n, sb, N = 10, 4, 1 << 10
+    rng = np.random.default_rng(21)
+    A = sp.random(N, N, density=0.01, random_state=3, format="lil").astype(np.complex128)
+    for r in range(0, N, 37):             # add entries near the diagonal (low masks)
+        for d in (1, 3, 6, 15):
+            A[r, r ^ d] = rng.standard_normal() + 1j
+    A = A.tocsr()
+    A.data = A.data + 1j * rng.standard_normal(A.data.shape)
+    A.sort_indices()
+    rp = torch.from_numpy(A.indptr.astype(np.int32)).to(dev)
+    ci = torch.from_numpy(A.indices.astype(np.int32)).to(dev)
+    va = torch.from_numpy(A.data.astype(np.complex64)).to(dev)
+    lib = bk._lib.load()
+    ws = torch.empty(int(lib.qsim_csr_split_workspace_bytes(N)), dtype=torch.uint8, device=dev)
+    nlo, nhi = ctypes.c_int64(0), ctypes.c_int64(0)
+    st = stream_handle(dev)
+    call("qsim_csr_split_count", st, ptr(rp), ptr(ci), N, sb, ptr(ws), ctypes.byref(nlo), ctypes.byref(nhi))
+    assert nlo.value + nhi.value == A.nnz
+    rp_lo = torch.empty(N + 1, dtype=torch.int32, device=dev)
+    rp_hi = torch.empty(N + 1, dtype=torch.int32, device=dev)
+    ci_lo = torch.empty(nlo.value, dtype=torch.int32, device=dev)
+    ci_hi = torch.empty(nhi.value, dtype=torch.int32, device=dev)
+    v_lo = torch.empty(nlo.value, dtype=torch.complex64, device=dev)
+    v_hi = torch.empty(nhi.value, dtype=torch.complex64, device=dev)
+    call("qsim_csr_split_fill", st, ptr(rp), ptr(ci), ptr(va), N, sb, ptr(ws), ptr(rp_lo), ptr(ci_lo),
+         ptr(v_lo), ptr(rp_hi), ptr(ci_hi), ptr(v_hi))
+    h_lo = (rp_lo.cpu().numpy(), ci_lo.cpu().numpy(), v_lo.cpu().numpy())
+    h_hi = (rp_hi.cpu().numpy(), ci_hi.cpu().numpy(), v_hi.cpu().numpy())
+    a32 = A.data.astype(np.complex64)
+
+    def rot(p):
+        return ((p & ((1 << (n - sb)) - 1)) << sb) | (p >> (n - sb))
+    for r in range(N):
+        cols = A.indices[A.indptr[r]:A.indptr[r + 1]]
+        vals = a32[A.indptr[r]:A.indptr[r + 1]]
+        low = ((cols ^ r) >> sb) == 0
+        got_c = h_lo[1][h_lo[0][r]:h_lo[0][r + 1]]
+        got_v = h_lo[2][h_lo[0][r]:h_lo[0][r + 1]]
+        assert np.array_equal(got_c, cols[low]) and np.array_equal(got_v.view(np.uint32), vals[low].view(np.uint32)), r
+    for p in range(N):
+        r = rot(p)
+        cols = A.indices[A.indptr[r]:A.indptr[r + 1]]
+        vals = a32[A.indptr[r]:A.indptr[r + 1]]
+        high = ((cols ^ r) >> sb) != 0
+        got_c = h_hi[1][h_hi[0][p]:h_hi[0][p + 1]]
+        got_v = h_hi[2][h_hi[0][p]:h_hi[0][p + 1]]
+        assert np.array_equal(got_c, cols[high]) and np.array_equal(got_v.view(np.uint32), vals[high].view(np.uint32)), p
+    R, scale, c = 32, 0.21, 0.4 + 0.25j
+    A64 = A.astype(np.complex64).astype(np.complex128)
+    bound = scale * (abs(A64) @ np.ones(N))
+    for first in (1, 0):
+        T = (rng.standard_normal((N, R)) + 1j * rng.standard_normal((N, R))).astype(np.complex64)
+        Tp = (rng.standard_normal((N, R)) + 1j * rng.standard_normal((N, R))).astype(np.complex64)
+        outs = []
+        for split in (0, 1):
+            tpd, acc = to_dev(Tp, dev), torch.zeros(N, R, dtype=torch.complex64, device=dev)
+            comp = torch.zeros_like(acc)
+            if split:
+                w1 = torch.empty_like(acc)
+                call("qsim_cheb_spmm_step_split", st, ptr(rp_lo), ptr(ci_lo), ptr(v_lo), ptr(rp_hi),
+                     ptr(ci_hi), ptr(v_hi), N, sb, R, scale, ptr(to_dev(T, dev)), ptr(tpd), ptr(acc),
+                     ptr(comp), ptr(w1), c.real, c.imag, first)
+            else:
+                call("qsim_cheb_spmm_step", st, ptr(rp), ptr(ci), ptr(va), N, R, scale, ptr(to_dev(T, dev)),
+                     ptr(tpd), ptr(acc), ptr(comp), c.real, c.imag, first, 0)
+            outs.append((tpd.cpu().numpy().astype(np.complex128), (acc + comp).cpu().numpy()))
+        tmax = np.max(np.abs(T))
+        tol = 2.0 ** -22 * (2 * bound[:, None] * tmax + np.abs(outs[0][0])) + 1e-30
+        assert np.all(np.abs(outs[0][0] - outs[1][0]) <= tol), first
+        assert np.max(np.abs(outs[0][1] - outs[1][1])) <= 1e-6 * np.max(np.abs(outs[0][1])), first
+
+
 def test_block_batch_equals_per_instance_blocks(bk, dev):
     """Batched block steps (qsim_apply_block_batch) give bitwise the results
     of applying each instance's blocks alone (qsim_apply_block_mfma, whose
