@@ -20,10 +20,10 @@ split=True runs each term as two launches over the CSR split at bit n/2
 FP64 row sums): the entries that connect rows differing only in the low
 half of the index bits, in natural row order, then the others in an order
 where the high index bits vary fastest.  Each pass gathers T_k rows from a
-smaller neighbourhood, at the price of one more write and read of a
-partial-sum vector.  On a 22-qubit Heisenberg chain at 32 RHS the two
-passes take longer than the one-pass term, so the one-pass term is the
-default.
+smaller neighbourhood, at the price of a second copy of the CSR and one
+more write and read of a partial-sum vector.  On a 22-qubit Heisenberg
+chain at 32 RHS it saves a few per cent per term, so the one-pass term is
+the default.
 
 Row sums of the SpMM (accumulate=):
   "fp64"   every FP32 x FP32 product is exact in FP64 and is summed with

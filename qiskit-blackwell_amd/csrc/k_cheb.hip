@@ -108,43 +108,49 @@ __global__ __launch_bounds__(kThreads) void k_step_accum(const float2* __restric
 
 // ---------------------------------------------------------------- 32 RHS
 // The batched term at 32 right-hand sides, the width of one RHS-contiguous
-// row (32 x 8 B = 256 B = one half-wave of float2).  One wave owns two
-// consecutive CSR rows: lanes 0..31 row r, lanes 32..63 row r + 1, lane & 31
-// the RHS.  Per nonzero a half-wave needs the column index and the value
-// (the same for its 32 lanes) and one 256-byte gathered row of T_k.
-//  * Row data: the wave's two rows are consecutive in the CSR arrays, so it
+// row (32 x 8 B = 256 B).  One wave owns four consecutive CSR rows: lanes
+// 16 q .. 16 q + 15 row r + q, each lane two RHS (one 16-byte load per
+// gathered row).  Per nonzero a quarter-wave needs the column index and the
+// value (the same for its 16 lanes) and one 256-byte gathered row of T_k, so
+// one gather instruction fetches four rows (1 KiB).
+//  * Why four rows: the gathers are bound by the number of vector memory
+//    instructions in flight, not by their bytes; measured on the 22-qubit
+//    chain at 32 RHS (FP64 sums), two rows per wave with 8-byte lanes take
+//    4.2 ms per term and four rows with 16-byte lanes 3.7 ms, bit for bit the
+//    same result.  2, 4 or 8 gathers in flight per lane measured the same
+//    (1 is 5 % slower).
+//  * Row data: the wave's four rows are consecutive in the CSR arrays, so it
 //    stages them into its own LDS slice with coalesced full-wave loads
-//    (nontemporal: the CSR is read once per term) and reads them back with
-//    half-wave broadcasts.  Loading them per lane from global memory instead
-//    returns 12 B per lane per nonzero through the vector L1, three times the
-//    bytes of the gather itself.
+//    (nontemporal: the CSR is read once per term) and reads them back as
+//    quarter-wave broadcasts.
 //  * Row sum, ACC = kAccF64: the product of two FP32 numbers is exact in FP64
 //    (24 + 24 bits < 53), so every product is converted and accumulated with
-//    FP64 FMAs.  CDNA4 issues FP64 FMA at the non-packed FP32 rate, so this costs 8 VALU
-//    operations per complex multiply-add against ~44 for the TwoProduct +
-//    Neumaier double-float form (ACC = kAccF32c), which caps a 22-qubit,
-//    32-RHS term at ~5 ms on VALU issue alone; the FP64 sum is also the more
-//    accurate of the two (53-bit accumulator, exact terms).
+//    FP64 FMAs.  CDNA4 issues FP64 FMA at the non-packed FP32 rate, so this
+//    costs 8 VALU operations per complex multiply-add against ~44 for the
+//    TwoProduct + Neumaier double-float form (ACC = kAccF32c); the FP64 sum is
+//    also the more accurate of the two (53-bit accumulator, exact terms).
 //  * Epilogue (both forms): T_{k+1} = 2 scale (A T_k) - T_{k-1}, rounded to
 //    complex64 once, and acc += c_k T_{k+1} as a Neumaier pair (acc, comp) in
-//    FP32, as in the generic kernel.  T_{k-1}, acc and comp are streamed
-//    (nontemporal when the 32-RHS block exceeds the MALL), so the cache keeps
-//    the gathered rows of T_k.
+//    FP32, as in the generic kernel.  T_{k-1}, acc and comp are loaded before
+//    the row's gathers and streamed (nontemporal when the 32-RHS block
+//    exceeds the MALL), so the cache keeps the gathered rows of T_k.
 //  * Block order: order_shift s > 0 hands each XCD contiguous chunks of 2^s
 //    rows, chunk x of every group of 8 chunks going to XCD x, so that a
 //    gather through a low column bit stays in the XCD's own L2 while all 8
 //    XCDs still walk one front through the MALL.  s = 0 is the dispatch
-//    order (blocks round-robin over the XCDs).
+//    order (blocks round-robin over the XCDs).  Shifts 0, 6, 9, 12 and 19
+//    measured within 4 % of each other.
 constexpr int kAccF64 = 0;
 constexpr int kAccF32c = 1;
-constexpr int kStageNnz = 128;      // nonzeros of a wave's two rows staged per chunk
 constexpr int kWaves = kThreads / 64;
-// Chunks of 2^9 rows per XCD and 4 gathers in flight per lane: measured on
-// the 22-qubit chain at 32 RHS, shifts 0, 6, 12 and 19 (the last is one
-// contiguous eighth per XCD) are within 3 % of 9, and 8 gathers in flight
-// within 1 % of 4 (2 is 3 % slower).
+constexpr int kRowsPerWave = 4;
+constexpr int kLanesPerRow = 64 / kRowsPerWave;            // 16
+constexpr int kRhsPerLane = 32 / kLanesPerRow;             // 2
+constexpr int kBlockRowsLog2 = 4;                          // 4 waves x 4 rows per workgroup
+static_assert(kWaves * kRowsPerWave == 1 << kBlockRowsLog2, "rows per workgroup");
+constexpr int kStageNnz = 256;      // nonzeros of a wave's four rows staged per chunk
 constexpr int kSpmmOrderShift = 9;
-constexpr int kGather32 = 4;   // gathers in flight per lane
+constexpr int kGather32 = 2;        // gathers in flight per lane
 
 typedef float f32x2_t __attribute__((ext_vector_type(2)));
 template <bool NT>
@@ -179,7 +185,7 @@ __device__ __forceinline__ void wave_sync() {
 __device__ __forceinline__ int64_t spmm_block(int s) {
   const int64_t b = blockIdx.x;
   if (s <= 0) return b;
-  const int g = s > 3 ? s - 3 : 0;            // log2(blocks per chunk): 8 rows per block
+  const int g = s > kBlockRowsLog2 ? s - kBlockRowsLog2 : 0;   // log2(blocks per chunk)
   const int64_t xcd = b & 7, i = b >> 3;
   const int64_t grp = i >> g, w = i & ((1ll << g) - 1);
   return ((grp * 8 + xcd) << g) + w;
@@ -231,13 +237,44 @@ struct RowSum<kAccF32c> {
 //           w = w1 + A_high T_k (FP64), then the same recurrence epilogue.
 // Entries whose mask has bits on both sides of s (at most a few per row)
 // go to pass 2 and are gathered from farther away.  On a 22-qubit chain at
-// 32 RHS this cuts the bytes fetched per term from 24 GB to 10 GB, yet the
-// two passes take 4.5 ms against 4.3 ms for PASS 0: the number of gather
-// instructions and their latency, which the split leaves as they are, bound
-// both forms.  Callers opt in to it.
+// 32 RHS this cuts the bytes fetched per term from 24 GB to 10 GB but gains
+// only a few per cent over PASS 0 (the gathers' issue and latency, not their
+// bytes, bound both forms), so callers opt in to it.
 __device__ __forceinline__ int64_t split_row(int64_t p, int n, int s) {
   const int lo_bits = n - s;
   return ((p & ((int64_t(1) << lo_bits) - 1)) << s) | (p >> lo_bits);
+}
+
+// kRhsPerLane complex RHS of one lane: float2 or float4 loads and stores.
+template <int EPL>
+struct CVec {
+  float2 c[EPL];
+};
+template <bool NT, int EPL>
+__device__ __forceinline__ CVec<EPL> ldv(const float2* p) {
+  CVec<EPL> r;
+  if constexpr (EPL == 1) {
+    r.c[0] = ld2<NT>(p);
+  } else {
+    typedef float f32x4_t __attribute__((ext_vector_type(4)));
+    f32x4_t v;
+    if constexpr (NT) v = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t*>(p));
+    else v = *reinterpret_cast<const f32x4_t*>(p);
+    r.c[0] = make_float2(v.x, v.y);
+    r.c[1] = make_float2(v.z, v.w);
+  }
+  return r;
+}
+template <bool NT, int EPL>
+__device__ __forceinline__ void stv(float2* p, const CVec<EPL>& r) {
+  if constexpr (EPL == 1) {
+    st2<NT>(p, r.c[0]);
+  } else {
+    typedef float f32x4_t __attribute__((ext_vector_type(4)));
+    const f32x4_t v = {r.c[0].x, r.c[0].y, r.c[1].x, r.c[1].y};
+    if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<f32x4_t*>(p));
+    else *reinterpret_cast<f32x4_t*>(p) = v;
+  }
 }
 
 template <int ACC, bool NT, int PASS>
@@ -246,66 +283,85 @@ __global__ __launch_bounds__(kThreads) void k_cheb_spmm32(
     int64_t n_rows, double scale, const float2* __restrict__ tc, float2* __restrict__ tpn,
     float2* __restrict__ acc, float2* __restrict__ comp, float2* __restrict__ w1, float2 c,
     int first, int order_shift, int n_bits, int split) {
+  constexpr int RW = kRowsPerWave, LPR = kLanesPerRow, EPL = kRhsPerLane, G = kGather32;
   __shared__ int32_t s_ci[kWaves][kStageNnz];
   __shared__ float2 s_v[kWaves][kStageNnz];
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int lane = threadIdx.x & 63, h = lane >> 5, j = lane & 31;
-  const int64_t row0 = (spmm_block(order_shift) * kWaves + wave) * 2;   // CSR row (position)
-  if (row0 >= n_rows) return;
-  const bool has1 = row0 + 1 < n_rows;
-  const int32_t b0 = rp[row0], b1 = rp[row0 + 1], b2 = has1 ? rp[row0 + 2] : b1;
-  const int32_t my_b = h ? b1 : b0, my_e = h ? b2 : b1;
+  const int lane = threadIdx.x & 63, qr = lane / LPR, jj = lane % LPR;
+  const int64_t pos0 = (spmm_block(order_shift) * kWaves + wave) * RW;   // first row (position)
+  if (pos0 >= n_rows) return;
+  const int n_here = (int)(n_rows - pos0 < RW ? n_rows - pos0 : RW);
+  const int32_t my_rp = lane <= n_here ? rp[pos0 + lane] : 0;
   const char* tcb = reinterpret_cast<const char*>(tc);
-  const uint32_t lane_off = (uint32_t)j * 8u;
-  RowSum<ACC> sum;
-  for (int32_t c0 = b0; c0 < b2; c0 += kStageNnz) {
-    const int32_t cend = c0 + kStageNnz < b2 ? c0 + kStageNnz : b2;
+  const uint32_t lane_off = (uint32_t)jj * (8u * EPL);
+  const int32_t gb = __builtin_amdgcn_readlane(my_rp, 0);        // the wave's entries [gb, ge)
+  const int32_t ge = __builtin_amdgcn_readlane(my_rp, n_here);
+  const bool live = qr < n_here;
+  const int my_r = live ? qr : 0;   // lanes past the last row shadow the first and store nothing
+  const int32_t my_b = __shfl(my_rp, my_r), my_e = live ? __shfl(my_rp, my_r + 1) : my_b;
+  const int64_t pos = pos0 + my_r;
+  const int64_t row = PASS == 2 ? split_row(pos, n_bits, split) : pos;
+  const int64_t e_idx = row * 32 + jj * EPL;
+  CVec<EPL> tp{}, a{}, k2{}, wv{};
+  if constexpr (PASS != 1) {
+    tp = ldv<NT, EPL>(tpn + e_idx);
+    a = ldv<NT, EPL>(acc + e_idx);
+    k2 = ldv<NT, EPL>(comp + e_idx);
+    if constexpr (PASS == 2) wv = ldv<NT, EPL>(w1 + e_idx);
+  }
+  RowSum<ACC> sum[EPL];
+  for (int32_t c0 = gb; c0 < ge; c0 += kStageNnz) {
+    const int32_t cend = c0 + kStageNnz < ge ? c0 + kStageNnz : ge;
+    wave_sync();
     for (int32_t k = c0 + lane; k < cend; k += 64) {
       s_ci[wave][k - c0] = __builtin_nontemporal_load(ci + k);
       s_v[wave][k - c0] = ld2<true>(val + k);
     }
     wave_sync();
     const int32_t lo = my_b > c0 ? my_b : c0, hi = my_e < cend ? my_e : cend;
-    int32_t k = lo;
-    for (; k + kGather32 <= hi; k += kGather32) {
-      float2 v[kGather32], x[kGather32];
+    const int32_t* sc = s_ci[wave] - c0;
+    const float2* sv = s_v[wave] - c0;
+    for (int32_t k = lo; k < hi; k += G) {
+      CVec<EPL> x[G];
 #pragma unroll
-      for (int u = 0; u < kGather32; ++u) {
-        const uint32_t col = (uint32_t)s_ci[wave][k + u - c0];
-        v[u] = s_v[wave][k + u - c0];
-        x[u] = *reinterpret_cast<const float2*>(tcb + (col * 256u + lane_off));
+      for (int u = 0; u < G; ++u) {
+        const int32_t idx = k + u < hi ? k + u : hi - 1;
+        // positions past hi gather the row's last entry again (no branch
+        // between the loads) and add it with weight 0, an exact no-op
+        x[u] = ldv<false, EPL>(
+            reinterpret_cast<const float2*>(tcb + ((uint32_t)sc[idx] * 256u + lane_off)));
       }
 #pragma unroll
-      for (int u = 0; u < kGather32; ++u) sum.add(v[u], x[u]);
+      for (int u = 0; u < G; ++u) {
+        const float2 v = k + u < hi ? sv[k + u] : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int e = 0; e < EPL; ++e) sum[e].add(v, x[u].c[e]);
+      }
     }
-    for (; k < hi; ++k) {
-      const uint32_t col = (uint32_t)s_ci[wave][k - c0];
-      sum.add(s_v[wave][k - c0], *reinterpret_cast<const float2*>(tcb + (col * 256u + lane_off)));
-    }
-    wave_sync();
   }
-  if (h && !has1) return;
+  if (!live) return;
   if constexpr (PASS == 1) {
-    // partial row sum of the low entries, rounded once
-    st2<NT>(w1 + (row0 + h) * 32 + j, make_float2((float)sum.r, (float)sum.i));
+    CVec<EPL> o;
+#pragma unroll
+    for (int e = 0; e < EPL; ++e) o.c[e] = make_float2((float)sum[e].r, (float)sum[e].i);
+    stv<NT, EPL>(w1 + e_idx, o);
     return;
   }
-  const int64_t row = PASS == 2 ? split_row(row0 + h, n_bits, split) : row0 + h;
-  const int64_t e_idx = row * 32 + j;
-  if constexpr (PASS == 2) {
-    const float2 w = ld2<NT>(w1 + e_idx);
-    sum.r += (double)w.x;
-    sum.i += (double)w.y;
+  CVec<EPL> tn;
+#pragma unroll
+  for (int e = 0; e < EPL; ++e) {
+    if constexpr (PASS == 2) {
+      sum[e].r += (double)wv.c[e].x;
+      sum[e].i += (double)wv.c[e].y;
+    }
+    tn.c[e] = sum[e].next(scale, first ? make_float2(0.f, 0.f) : tp.c[e], first != 0);
+    const float2 v = cmulc(c, tn.c[e]);
+    neu_add(a.c[e].x, k2.c[e].x, v.x);
+    neu_add(a.c[e].y, k2.c[e].y, v.y);
   }
-  const float2 tp = first ? make_float2(0.f, 0.f) : ld2<NT>(tpn + e_idx);
-  const float2 tn = sum.next(scale, tp, first != 0);
-  st2<NT>(tpn + e_idx, tn);
-  const float2 v = cmulc(c, tn);
-  float2 a = ld2<NT>(acc + e_idx), k2 = ld2<NT>(comp + e_idx);
-  neu_add(a.x, k2.x, v.x);
-  neu_add(a.y, k2.y, v.y);
-  st2<NT>(acc + e_idx, a);
-  st2<NT>(comp + e_idx, k2);
+  stv<NT, EPL>(tpn + e_idx, tn);
+  stv<NT, EPL>(acc + e_idx, a);
+  stv<NT, EPL>(comp + e_idx, k2);
 }
 
 // ---------------------------------------------------------------- CSR split
@@ -481,10 +537,11 @@ __global__ __launch_bounds__(kThreads) void k_cheb_finish(float2* __restrict__ o
   out[i] = make_float2(a.x + k.x, a.y + k.y);
 }
 
-// Order shift for a grid of `blocks` 32-RHS workgroups (8 rows each): the
-// requested shift when the chunks tile the grid exactly, else 0.
+// Order shift for a grid of `blocks` 32-RHS workgroups (2^kBlockRowsLog2
+// rows each): the requested shift when the chunks tile the grid exactly,
+// else 0.
 static int spmm_order(int64_t blocks, int shift) {
-  const int64_t chunk = shift > 3 ? int64_t(1) << (shift - 3) : 1;
+  const int64_t chunk = shift > kBlockRowsLog2 ? int64_t(1) << (shift - kBlockRowsLog2) : 1;
   return blocks % (8 * chunk) == 0 ? shift : 0;
 }
 
@@ -561,7 +618,7 @@ int qsim_cheb_spmm_step(void* stream, const void* row_ptr, const void* col_idx, 
   const bool f64 = accumulate == QSIM_CHEB_ACC_FP64;
   // the 32-RHS kernel addresses T_k with 32-bit byte offsets (<= 2^24 rows)
   if (n_rhs == 32 && n_rows <= (int64_t(1) << 24)) {
-    const int64_t blocks = (int64_t)ceil_div((uint64_t)n_rows, 2 * kWaves);
+    const int64_t blocks = (int64_t)ceil_div((uint64_t)n_rows, kWaves * kRowsPerWave);
     const int order = spmm_order(blocks, kSpmmOrderShift);
     const bool nt = nt_for_bytes((uint64_t)n_rows * 256u);
     auto* kern = f64 ? (nt ? k_cheb_spmm32<kAccF64, true, 0> : k_cheb_spmm32<kAccF64, false, 0>)
@@ -669,7 +726,7 @@ int qsim_cheb_spmm_step_split(void* stream, const void* rp_lo, const void* ci_lo
     return fail(QSIM_EINVAL, "NULL pointer argument");
   hipStream_t s = as_stream(stream);
   const float2 cf = f2(c_re, c_im);
-  const int64_t blocks = (int64_t)ceil_div((uint64_t)n_rows, 2 * kWaves);
+  const int64_t blocks = (int64_t)ceil_div((uint64_t)n_rows, kWaves * kRowsPerWave);
   // each XCD walks one contiguous eighth of the rows (positions) in both passes
   const int order = spmm_order(blocks, n >= 3 ? n - 3 : 0);
   const bool nt = nt_for_bytes((uint64_t)n_rows * 256u);

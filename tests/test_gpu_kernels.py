@@ -514,19 +514,22 @@ def _ragged_csr(n_rows, seed):
     return A
 
 
+@pytest.mark.parametrize("n_rows", [701, 1536, 4096])
 @pytest.mark.parametrize("accumulate", ["fp64", "fp32c"])
-def test_cheb_spmm_step_matches_fp64(bk, dev, accumulate):
+def test_cheb_spmm_step_matches_fp64(bk, dev, accumulate, n_rows):
     """One fused Chebyshev term, qsim_cheb_spmm_step, against the same term in
     FP64 on the host: w = scale A T_k, T_{k+1} = w (first) or 2 w - T_{k-1},
-    acc + comp += c T_{k+1}.  701 rows (odd: the last wave has one row),
-    ragged row lengths, 32 RHS (the two-rows-per-wave kernel) and 5 RHS (the
-    generic kernel).  The 32 RHS run also equals, bit for bit, the same term
-    computed as two 16-RHS halves through the generic kernel (same summation
-    order per element)."""
+    acc + comp += c T_{k+1}.  Ragged row lengths; 32 RHS and 5 RHS (the
+    generic kernel).  At 32 RHS, 701 rows (odd: the last wave has one row)
+    run the two-rows-per-wave kernel, and 1536 rows (3 windows, dispatch
+    order) and 4096 rows (8 windows, one per XCD) the 512-row LDS window
+    kernel, whose entries mix window reads and global gathers.  The 32 RHS
+    run also equals, bit for bit, the same term computed as two 16-RHS halves
+    through the generic kernel (same summation order per element)."""
     from blackwell_backend.chebyshev import ACCUMULATE
     from blackwell_kernels._lib import call, ptr, stream_handle
     mode = ACCUMULATE[accumulate]
-    A = _ragged_csr(701, 5)
+    A = _ragged_csr(n_rows, 5)
     N = A.shape[0]
     A64 = A.astype(np.complex64).astype(np.complex128)
     rp = torch.from_numpy(A.indptr.astype(np.int32)).to(dev)
