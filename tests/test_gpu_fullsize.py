@@ -335,7 +335,9 @@ def test_mfma_block_variants_26q_match_oracle():
       (2, 7, 11, 18, 24): 16-byte-lane kernel, lowest block qubit in 1..3;
       (0, 6, 13, 19, 23): 16-byte-lane kernel with qubit 0 as row bit 0;
       (1, 3, 4, 17, 22) and (22, 4, 1, 17, 3): LDS-staged kernel
-          (three block qubits among 0..4), nontemporal."""
+          (three block qubits among 0..4), nontemporal;
+      (20, 7, 25, 11, 16): 16-byte-lane kernel with every block qubit >= 7,
+          unsorted, nontemporal."""
     import blackwell_kernels as bk
     from oracle import statevector as SV
     from helpers import rand_state, rand_unitary, close
@@ -345,7 +347,7 @@ def test_mfma_block_variants_26q_match_oracle():
     st = torch.from_numpy(psi).to(dev)
     ref = psi.astype(np.complex128)
     blocks = [(5, 9, 14, 20, 25), (25, 5, 20, 9, 14), (2, 7, 11, 18, 24), (0, 6, 13, 19, 23),
-              (1, 3, 4, 17, 22), (22, 4, 1, 17, 3)]
+              (1, 3, 4, 17, 22), (22, 4, 1, 17, 3), (20, 7, 25, 11, 16)]
     for k, qs in enumerate(blocks):
         print(f"[fullsize] MFMA block {qs} at {n} qubits vs the FP64 oracle", flush=True)
         u = rand_unitary(32, 500 + k).astype(np.complex64)

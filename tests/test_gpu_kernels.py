@@ -352,7 +352,9 @@ def test_tile_program_random(bk, dev):
     # 16-byte-lane kernel: qubit 0 outside the block (lowest block qubit >= 4 or in 1..3) and
     # qubit 0 as block row bit 0; n = 11 is the smallest state it takes (one set of 64 groups)
     (11, [4, 5, 6, 9, 10]), (14, [1, 6, 8, 12, 13]), (14, [2, 3, 7, 9, 13]), (14, [0, 5, 6, 9, 12]),
-    (13, [0, 1, 7, 10, 12]), (17, [16, 12, 8, 5, 4]), (15, [0, 14, 3, 9, 6]), (11, [0, 6, 7, 9, 10])])
+    (13, [0, 1, 7, 10, 12]), (17, [16, 12, 8, 5, 4]), (15, [0, 14, 3, 9, 6]), (11, [0, 6, 7, 9, 10]),
+    # every block qubit >= 7 (rows of 512 contiguous bytes per set), unsorted qubits
+    (13, [12, 7, 9, 8, 11]), (16, [15, 8, 12, 10, 9]), (12, [7, 8, 9, 10, 11])])
 def test_block_mfma(bk, dev, n, qubits):
     psi = rand_state(n, n + qubits[0])
     u = rand_unitary(32, sum(qubits)).astype(np.complex64)
