@@ -42,6 +42,7 @@ single-launch (bwk/csrc/qv8_sim_sm120.cu:58-151).
 import collections
 import ctypes
 import time
+import weakref
 
 import numpy as np
 import torch
@@ -456,7 +457,10 @@ class _GraphCache:
     def __init__(self, max_graphs=8, max_qubits=24):
         self.max_graphs = int(max_graphs)
         self.max_qubits = int(max_qubits)
-        self.seen = collections.Counter()
+        # run counts die with their circuits, so a long stream of distinct
+        # circuits leaves nothing behind
+        self.seen = weakref.WeakKeyDictionary()
+        # id(cc) -> graph; the graph holds cc, so the id stays cc's while cached
         self.graphs = collections.OrderedDict()
         self.captures = 0
         self.replays = 0
@@ -464,11 +468,13 @@ class _GraphCache:
     def get(self, cc, device):
         if cc.n_qubits > self.max_qubits or cc.has_stochastic or self.max_graphs <= 0:
             return None
+        self.seen[cc] = self.seen.get(cc, 0) + 1
         key = id(cc)
-        self.seen[key] += 1
         g = self.graphs.get(key)
+        if g is not None and g.cc is not cc:
+            g = None
         if g is None:
-            if self.seen[key] < 2:
+            if self.seen[cc] < 2:
                 return None
             mode = cc.resolve_mode("blocks")
             g = cc.graphed(mode, device)
@@ -476,8 +482,14 @@ class _GraphCache:
                 return None
             self.captures += 1
             self.graphs[key] = g
-            while len(self.graphs) > self.max_graphs:
-                self.graphs.popitem(last=False)
+            if len(self.graphs) > self.max_graphs:
+                # a dropped graph's private memory goes back to the allocator:
+                # let its replays finish first
+                if torch.device(device).type == "cuda":
+                    torch.cuda.synchronize(device)
+                while len(self.graphs) > self.max_graphs:
+                    _, old = self.graphs.popitem(last=False)
+                    old.cc._graphs.clear()
         else:
             self.graphs.move_to_end(key)
         self.replays += 1

@@ -157,3 +157,49 @@ def test_hbm_lane_batches_capped_by_state_bytes():
     for b in batches:
         assert len(b) == 1 or sum(8 << n for n in b) <= cap
     assert [20] in batches                                   # larger than the cap: alone
+
+
+def test_graph_cache_policy():
+    """_GraphCache: a circuit's first run is not captured, its second is; at
+    most max_graphs graphs stay cached (least recently used dropped, and the
+    circuit's own graph slot cleared with it); circuits that are too large or
+    measure mid-circuit are never captured; run counts go away with their
+    circuits."""
+    import gc
+    from blackwell_backend.serving import _GraphCache
+
+    class FakeGraph:
+        def __init__(self, cc):
+            self.cc = cc
+
+    class FakeCC:
+        def __init__(self, n=10, stochastic=False):
+            self.n_qubits = n
+            self.has_stochastic = stochastic
+            self._graphs = {}
+
+        def resolve_mode(self, mode):
+            return mode
+
+        def graphed(self, mode, device):
+            return self._graphs.setdefault((mode, device), FakeGraph(self))
+
+    cache = _GraphCache(max_graphs=2, max_qubits=20)
+    a, b, c = FakeCC(), FakeCC(), FakeCC()
+    assert cache.get(a, "cpu") is None                  # first run: eager
+    ga = cache.get(a, "cpu")
+    assert ga is not None and ga.cc is a and cache.get(a, "cpu") is ga
+    assert cache.get(b, "cpu") is None and cache.get(b, "cpu").cc is b
+    cache.get(a, "cpu")                                  # a most recently used
+    assert cache.get(c, "cpu") is None and cache.get(c, "cpu").cc is c
+    assert set(id(x) for x in (a, c)) == set(cache.graphs) and b._graphs == {}
+    assert cache.captures == 3
+    big, noisy = FakeCC(n=21), FakeCC(stochastic=True)
+    for cc in (big, noisy):
+        assert cache.get(cc, "cpu") is None and cache.get(cc, "cpu") is None
+    d = FakeCC()
+    assert cache.get(d, "cpu") is None
+    n_seen = len(cache.seen)
+    del d
+    gc.collect()
+    assert len(cache.seen) == n_seen - 1
