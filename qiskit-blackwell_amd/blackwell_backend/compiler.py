@@ -293,30 +293,34 @@ def _mask(qs):
     return m
 
 
-def plan_blocks(ops, n, k=5):
+def plan_blocks(ops, n, k=5, effort=1):
     """Pack unitary ops into dense blocks of min(k, n) qubits with the
-    library's host planner (qsim_plan_blocks, FP64): a greedy scan in
-    circuit order where an op joins the current block if none of its qubits
-    was touched by an earlier op left out, and the union stays <= k.
-    measure/reset split the list into independently planned segments.
-    Returns a list of Block | Op (ops wider than k pass through)."""
+    library's host planner (qsim_plan_blocks_search, FP64): an op joins a
+    block if its qubits lie in the block's qubit set and none of them was
+    touched by an earlier op left out; effort 1 (the default) also searches
+    the block qubit sets around each seed op, effort 0 is the greedy scan
+    alone.  measure/reset split the
+    list into independently planned segments.  Returns a list of Block | Op
+    (ops wider than k pass through)."""
     out = []
     seg = []
     for op in ops:
         if op.kind in ("measure", "reset"):
-            out.extend(_plan_segment(seg, n, k))
+            out.extend(_plan_segment(seg, n, k, effort))
             seg = []
             out.append(op)
         else:
             seg.append(op)
-    out.extend(_plan_segment(seg, n, k))
+    out.extend(_plan_segment(seg, n, k, effort))
     return out
 
 
-def plan_blocks_arrays(n, op_nq, op_qubits, op_mats, k=5):
+def plan_blocks_arrays(n, op_nq, op_qubits, op_mats, k=5, effort=1):
     """The native planner on flat arrays: op_nq int32 [m], op_qubits int32
     [m, 8] (-1 padded), op_mats complex128 (each op's 2^nq x 2^nq matrix,
-    row bit j <-> op_qubits[j], concatenated).  Returns (refs, block_qubits
+    row bit j <-> op_qubits[j], concatenated); effort as plan_blocks (the
+    serving lane plans gate payloads with the same default, so a payload and
+    its circuit get the same blocks).  Returns (refs, block_qubits
     int32 [B, 8], block_mats complex128 [B, D, D]): refs[e] >= 0 is block
     refs[e], refs[e] < 0 is op -1 - refs[e] passed through.  Pure host code;
     the library call releases the GIL, so plans can be built on worker
@@ -335,14 +339,14 @@ def plan_blocks_arrays(n, op_nq, op_qubits, op_mats, k=5):
     bm = np.empty((max(m, 1), D, D), dtype=np.complex128)
     n_out = ctypes.c_int(0)
     vp = lambda a: ctypes.c_void_p(a.ctypes.data)
-    _lib.call("qsim_plan_blocks", n, k, m, vp(nq), vp(qb), vp(mv.view(np.float64)), vp(refs),
-              vp(bqs), vp(bm), ctypes.byref(n_out))
+    _lib.call("qsim_plan_blocks_search", n, k, effort, m, vp(nq), vp(qb), vp(mv.view(np.float64)),
+              vp(refs), vp(bqs), vp(bm), ctypes.byref(n_out))
     refs = refs[:n_out.value]
     nb = int((refs >= 0).sum())
     return refs, bqs[:nb], bm[:nb]
 
 
-def _plan_segment(ops, n, k):
+def _plan_segment(ops, n, k, effort=1):
     if not ops:
         return []
     m = len(ops)
@@ -354,7 +358,7 @@ def _plan_segment(ops, n, k):
         nq[i] = len(qs)
         qb[i, :len(qs)] = qs
         mats.append(np.asarray(op.full_matrix(), dtype=np.complex128).reshape(-1))
-    refs, bqs, bm = plan_blocks_arrays(n, nq, qb, np.concatenate(mats), k)
+    refs, bqs, bm = plan_blocks_arrays(n, nq, qb, np.concatenate(mats), k, effort)
     res = []
     for e in refs:
         if e >= 0:

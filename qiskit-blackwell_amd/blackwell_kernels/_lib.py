@@ -51,6 +51,7 @@ PROTOTYPES = {
     "qsim_tile_program": (_i, [_vp, _vp, _i, _u64, _vp, _i, _vp]),
     "qsim_apply_block_mfma": (_i, [_vp, _vp, _i, _vp, _pint, _i]),
     "qsim_plan_blocks": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _pint]),
+    "qsim_plan_blocks_search": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _pint]),
     "qsim_apply_blocks": (_i, [_vp, _vp, _i, _vp, _vp, _i]),
     "qsim_apply_block_batch": (_i, [_vp, _vp, _vp, _i64]),
     "qsim_reduce_workspace_bytes": (_i64, [_i]),

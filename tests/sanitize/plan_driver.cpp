@@ -1,4 +1,5 @@
-// Host driver for qsim_plan_blocks (csrc/host_plan.cpp) built with the host
+// Host driver for qsim_plan_blocks and qsim_plan_blocks_search
+// (csrc/host_plan.cpp) built with the host
 // sanitizers by tests/test_planner_sanitizers.py.  For random circuits of
 // 1..3-qubit ops (some wider than the block size, so they pass through) it
 // checks that the plan, replayed in FP64 on a random state (each block as a
@@ -82,8 +83,13 @@ int main() {
     std::vector<int32_t> ref(m > 0 ? m : 1), bq(8 * (size_t)(m > 0 ? m : 1));
     std::vector<double> bm((size_t)(m > 0 ? m : 1) * D * D * 2);
     int n_out = -1;
-    const int rc = qsim_plan_blocks(n, k, m, nq.data(), qs.data(), mats.empty() ? nullptr : mats.data(),
-                                    ref.data(), bq.data(), bm.data(), &n_out);
+    // odd trials plan with the qubit-set search (effort 1), even ones greedily
+    const int rc = trial & 1
+                       ? qsim_plan_blocks_search(n, k, 1, m, nq.data(), qs.data(),
+                                                 mats.empty() ? nullptr : mats.data(), ref.data(),
+                                                 bq.data(), bm.data(), &n_out)
+                       : qsim_plan_blocks(n, k, m, nq.data(), qs.data(), mats.empty() ? nullptr : mats.data(),
+                                          ref.data(), bq.data(), bm.data(), &n_out);
     if (rc != QSIM_OK || n_out < 0 || n_out > m) {
       std::printf("trial %d: plan failed rc=%d n_out=%d (%s)\n", trial, rc, n_out, qsim::g_msg);
       return 1;
@@ -146,6 +152,11 @@ int main() {
   }
   if (qsim_plan_blocks(4, 5, 1, nq2, q2, u2, out, bqs, bm1, nullptr) == QSIM_OK) {
     std::printf("NULL n_out accepted\n");
+    return 1;
+  }
+  int32_t nq3[1] = {1}, q3[8] = {0, -1, -1, -1, -1, -1, -1, -1};
+  if (qsim_plan_blocks_search(4, 5, 2, 1, nq3, q3, u2, out, bqs, bm1, &no) == QSIM_OK) {
+    std::printf("effort 2 accepted\n");
     return 1;
   }
   std::printf("planner ok: %d random plans replayed exactly (%d passthrough entries), malformed input rejected\n",
