@@ -4,9 +4,10 @@ one MI355X.
 Workload (one step): the 30-qubit random-SU(4) volume circuit, depth 30
 (450 Haar 4x4 gates on seeded random pairs, complex64 state of 8 GiB), run
 from |0...0> through the plain per-gate kernels.  value = 450 * 2^30 /
-seconds per step.  The same circuit also runs as fused 5-qubit blocks on
-FP32 MFMA ("fused"); both forms are checked against an FP64 CPU statevector
-of the same circuit family at 22 qubits ("parity").
+seconds per step.  The same circuit also runs as fused blocks on FP32 MFMA
+("fused"), at the fusion width (4, 5 or 6 qubits) the tuner measures fastest
+for it; the plain and 5-qubit-block forms are checked against an FP64 CPU
+statevector of the same circuit family at 22 qubits ("parity").
 
 Extra figures in the same JSON line:
   roofline        dominant kernel (dense 2Q gate), 16 B per amplitude update,
@@ -39,7 +40,7 @@ import torch  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per amplitude update that k_gate2q moves, measured with two
 # rocprofv3 --pmc runs of this script (FETCH_SIZE, then WRITE_SIZE, both scaled
-# by the counts of a 2 GiB device-to-device copy run first with
+# by the counts of a device-to-device copy of the state run first with
 # --pmc-calibration): one read and one write of the state, nothing more.
 GATE2Q_TRAFFIC_B_PER_AMP = 16.0
 GATE2Q_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, per k_gate2q dispatch of this "
@@ -846,18 +847,26 @@ def main():
                              "read and one write of the state per launch"},
     }
     if not args.no_fused:
-        log("[bench] fused 5-qubit MFMA blocks")
-        st, ftotal, fwall, fkern, flaunch = run_steps(sim, qc, "blocks", args.steps, args.warmup, stream)
+        # the fusion width the tuner picks for this circuit (blocks of 4, 5 or 6
+        # qubits, each timed from |0...0> with HIP events), then the timed steps
+        from blackwell_backend.executor import block_width
+        log("[bench] fused MFMA blocks: width tuner")
+        fmode, width_ms = sim.compile(qc).tuned_block_width(sim.device)
+        k = block_width(fmode)
+        log(f"[bench] fused {k}-qubit MFMA blocks (tuner ms per run: "
+            + ", ".join(f"k={w}: {t:.1f}" for w, t in sorted(width_ms.items())) + ")")
+        st, ftotal, fwall, fkern, flaunch = run_steps(sim, qc, fmode, args.steps, args.warmup, stream)
         del st
         fused_value = updates / (ftotal / args.steps)
         blk_s = float(np.median(fkern)) / flaunch
         blk_bytes = 16.0 * (1 << n)
-        blk_flops = 256.0 * (1 << n)
+        blk_flops = 8.0 * (1 << k) * (1 << n)
         log(f"[bench] fused: {ftotal / args.steps * 1e3:.1f} ms/step, {fused_value:.3e} gate-amp-updates/s,"
             f" {flaunch} blocks")
         line["fused"] = {"value": fused_value, "unit": "gate-equivalent amp-updates/s",
                          "ms_per_step": ftotal / args.steps * 1e3, "blocks": flaunch,
-                         "block_qubits": 5,
+                         "block_qubits": k,
+                         "tuner_ms_per_run_by_width": {str(w): round(t, 3) for w, t in sorted(width_ms.items())},
                          "speedup_vs_plain": fused_value / value,
                          "block_GB_per_s": blk_bytes / blk_s / 1e9,
                          "block_mfma_TFLOP_per_s": blk_flops / blk_s / 1e12,
