@@ -416,6 +416,18 @@ def c5_bench(n=22, n_rhs=32, times=(1.0, 5.0, 10.0), seed=5, precision=1e-7):
         "max_abs_diff_vs_fp64_row_sums": (Yc.to(torch.complex128) -
                                           outs[times[0]].to(torch.complex128)).abs().max().item()}
     del Yc
+    # one right-hand side (the single-state chebyshev_evolve / run_hamiltonian
+    # path) at t = 1: the first vector alone
+    torch.cuda.synchronize()
+    Y1, info1 = chebyshev_evolve_batch(X[0].contiguous(), h, times[0], precision)
+    K1 = info1["chebyshev_degree"]
+    per1 = 12.0 * h.nnz + 4.0 * (N + 1) + 5.0 * 8.0 * N
+    res["single_rhs"] = {
+        "t": times[0], "K": K1, "ms_per_term": round(info1["elapsed_s"] / K1 * 1e3, 4),
+        "GB_per_s": round(per1 * K1 / info1["elapsed_s"] / 1e9, 1),
+        "max_abs_diff_vs_batched": (Y1.to(torch.complex128) -
+                                    outs[times[0]][0].to(torch.complex128)).abs().max().item()}
+    del Y1
     # the opt-in two-pass term over the CSR split at bit n/2, at t = 1
     if n_rhs == 32:
         torch.cuda.synchronize()

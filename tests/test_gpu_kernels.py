@@ -521,8 +521,8 @@ def _ragged_csr(n_rows, seed):
 def test_cheb_spmm_step_matches_fp64(bk, dev, accumulate, n_rows):
     """One fused Chebyshev term, qsim_cheb_spmm_step, against the same term in
     FP64 on the host: w = scale A T_k, T_{k+1} = w (first) or 2 w - T_{k-1},
-    acc + comp += c T_{k+1}.  Ragged row lengths; 32 RHS and 5 RHS (the
-    generic kernel).  At 32 RHS, 701 rows (odd: the last wave has one row)
+    acc + comp += c T_{k+1}.  Ragged row lengths; 32 RHS, 5 RHS (the
+    generic kernel) and 1 RHS (16 lanes per row, merged partial sums).  At 32 RHS, 701 rows (odd: the last wave has one row)
     run the two-rows-per-wave kernel, and 1536 rows (3 windows, dispatch
     order) and 4096 rows (8 windows, one per XCD) the 512-row LDS window
     kernel, whose entries mix window reads and global gathers.  The 32 RHS
@@ -541,7 +541,7 @@ def test_cheb_spmm_step_matches_fp64(bk, dev, accumulate, n_rows):
     scale, c = 0.37, 0.3 - 0.45j
     # the bound on each element's error scales with sum_k |a_ik| |t_kj|
     absA = abs(A64)
-    for R in (32, 5):
+    for R in (32, 5, 1):
         def cplx(*shape):
             return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
         T, Tp, acc0 = cplx(N, R), cplx(N, R), cplx(N, R)
