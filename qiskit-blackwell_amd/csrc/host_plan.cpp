@@ -15,9 +15,9 @@
 //    takes 100 blocks (120 with the first op as the only seed).
 //  * effort 1: in addition, every S made of the seed's qubits and up to
 //    k - |seed| of the qubits that the next kSearchWindow ops connect to the
-//    seed is scored.  The depth-30 SU(4) circuit on 30 qubits then takes 108
-//    blocks instead of 119 (28 qubits: 96 instead of 100), for a few tens of
-//    milliseconds of host time.
+//    seed is scored.  The depth-30 SU(4) circuit on 30 qubits then takes 110
+//    blocks instead of 119 (28 qubits, depth 28: 93 instead of 100), for a
+//    few tens of milliseconds of host time.
 // The block matrix is the FP64 product of the members in circuit order, each
 // applied to the rows of the running 2^k x 2^k matrix.  Blocks are padded with
 // the lowest unused qubits to exactly min(k, n) qubits.  An op wider than k
