@@ -150,7 +150,7 @@ def _materialise(req):
     if req.gates is not None:
         return gates_program(*req.gates)
     p = req.program
-    if p is not None and isinstance(p, tuple) and p and p[0] == "su4":
+    if isinstance(p, tuple) and p and isinstance(p[0], str) and p[0] == "su4":
         return su4_program(p[1], p[2], p[3])
     if p is not None:
         return p
@@ -161,9 +161,39 @@ def _circuit_of(req):
     if req.circuit is not None:
         return req.circuit
     p = req.program
-    if isinstance(p, tuple) and p and p[0] == "su4":
+    if isinstance(p, tuple) and p and isinstance(p[0], str) and p[0] == "su4":
         return W.su4_volume(p[1], p[2], p[3])
     raise ValueError("request has neither a circuit nor a rebuildable program")
+
+
+def _batch_programs(reqs):
+    """Per-request op and coefficient counts, and the batch's concatenated
+    ops (coefficient offsets global) and complex64 coefficients.  Requests
+    that all carry gate payloads are stacked without per-request programs;
+    the result is the same as concatenating their gates_program()s."""
+    if all(r.gates is not None for r in reqs):
+        pairs = [np.asarray(r.gates[0]) for r in reqs]
+        n_ops = np.array([len(pp) for pp in pairs], dtype=np.int64)
+        total = int(n_ops.sum())
+        ops = np.zeros(total, dtype=OP_DTYPE)
+        if total:
+            pq = np.concatenate(pairs)
+            ops["kind"] = _lib.OP_M2
+            ops["nq"] = 2
+            ops["q"][:, 0] = pq[:, 0]
+            ops["q"][:, 1] = pq[:, 1]
+            ops["coef"] = np.arange(total, dtype=np.uint32) * 16
+            cf = np.concatenate([np.asarray(r.gates[1]).reshape(-1) for r in reqs]).astype(np.complex64)
+        else:
+            cf = np.zeros(0, np.complex64)
+        return n_ops, 16 * n_ops, ops, cf
+    progs = [_materialise(r) for r in reqs]
+    n_ops = np.array([len(o) for o, _ in progs], dtype=np.int64)
+    n_cf = np.array([len(c) for _, c in progs], dtype=np.int64)
+    ops = np.concatenate([o for o, _ in progs])       # a new array: the programs stay as built
+    ops["coef"] += np.repeat(np.concatenate([[0], np.cumsum(n_cf)[:-1]]), n_ops).astype(np.uint32)
+    cf = np.concatenate([np.asarray(c, dtype=np.complex64) for _, c in progs])
+    return n_ops, n_cf, ops, cf
 
 
 # ---------------------------------------------------------------- LDS lane
@@ -175,35 +205,35 @@ class _LdsBatch:
         self.event = None
 
     def launch(self):
-        inst = np.zeros(len(self.reqs), dtype=INSTANCE_DTYPE)
-        all_ops, all_cf = [], []
-        op_off = cf_off = shot_off = st_off = 0
-        max_n = 1
-        self.state_off = []
-        for i, r in enumerate(self.reqs):
-            ops, cf = _materialise(r)
-            ops = ops.copy()
-            ops["coef"] += cf_off
-            inst[i]["seed"] = r.seed
-            inst[i]["shot_off"] = shot_off
-            inst[i]["op_off"] = op_off
-            inst[i]["n_ops"] = len(ops)
-            inst[i]["shots"] = r.shots
-            inst[i]["n_qubits"] = r.n_qubits
-            inst[i]["init"] = 1
-            inst[i]["want"] = 1 if r.keep_state else 0
-            inst[i]["state_off"] = st_off
-            self.state_off.append(st_off)
-            if r.keep_state:
-                st_off += 1 << r.n_qubits
-            all_ops.append(ops)
-            all_cf.append(np.asarray(cf, dtype=np.complex64))
-            op_off += len(ops)
-            cf_off += len(cf)
-            shot_off += r.shots
-            max_n = max(max_n, r.n_qubits)
-        blob_ops = np.concatenate(all_ops).view(np.uint8)
-        blob_cf = np.concatenate(all_cf) if cf_off else np.zeros(1, np.complex64)
+        # the batch's records are assembled column-wise: per-request Python
+        # work is only the (ops, coefs) of each circuit
+        reqs = self.reqs
+        n_ops, n_cf, ops_all, blob_cf = _batch_programs(reqs)
+        shots = np.array([r.shots for r in reqs], dtype=np.int64)
+        nq = np.array([r.n_qubits for r in reqs], dtype=np.int64)
+        want = np.array([r.keep_state for r in reqs], dtype=bool)
+        op_off = np.concatenate([[0], np.cumsum(n_ops)])
+        cf_off = np.concatenate([[0], np.cumsum(n_cf)])
+        shot_offs = np.concatenate([[0], np.cumsum(shots)])
+        st_sizes = np.where(want, np.left_shift(1, nq), 0)
+        st_offs = np.concatenate([[0], np.cumsum(st_sizes)])
+        inst = np.zeros(len(reqs), dtype=INSTANCE_DTYPE)
+        inst["seed"] = np.array([r.seed for r in reqs], dtype=np.uint64)
+        inst["shot_off"] = shot_offs[:-1]
+        inst["op_off"] = op_off[:-1]
+        inst["n_ops"] = n_ops
+        inst["shots"] = shots
+        inst["n_qubits"] = nq
+        inst["init"] = 1
+        inst["want"] = want
+        inst["state_off"] = st_offs[:-1]
+        self.state_off = [int(o) for o in st_offs[:-1]]
+        op_off, cf_off, shot_off, st_off = int(op_off[-1]), int(cf_off[-1]), int(shot_offs[-1]), int(st_offs[-1])
+        max_n = max(1, int(nq.max()))
+        blob_ops = ops_all.view(np.uint8)
+        if not cf_off:
+            blob_cf = np.zeros(1, np.complex64)
+        self.shot_offs = shot_offs
         with torch.cuda.stream(self.stream):
             self.d_inst = torch.from_numpy(inst.view(np.uint8).copy()).to(self.device)
             self.d_ops = torch.from_numpy(blob_ops.copy()).to(self.device)
@@ -234,14 +264,13 @@ class _LdsBatch:
 
     def finish(self, now):
         self.event.synchronize()
-        out = self.h_out.numpy()
-        off = 0
-        for r, so in zip(self.reqs, self.state_off):
-            r.samples = out[off:off + r.shots].copy()
+        # one copy out of the pinned buffer; each request's samples are a view of it
+        out = self.h_out.numpy().copy()
+        for r, so, off in zip(self.reqs, self.state_off, self.shot_offs[:-1]):
+            r.samples = out[int(off):int(off) + r.shots]
             if r.keep_state:
                 r.state = self.h_states.numpy()[so:so + (1 << r.n_qubits)].copy()
             r.t_done = now
-            off += r.shots
         self.d_states = self.h_states = None
 
 
@@ -436,9 +465,9 @@ class _HbmBatch:
 
     def finish(self, now):
         self.event.synchronize()
-        out = self.h_out.numpy()
+        out = self.h_out.numpy().copy()
         for i, (r, so) in enumerate(zip(self.reqs, self.shot_off[:-1])):
-            r.samples = out[int(so):int(so) + r.shots].copy()
+            r.samples = out[int(so):int(so) + r.shots]
             if i in self.h_states:
                 r.state = self.h_states[i].numpy().copy()
             r.t_done = now

@@ -203,3 +203,31 @@ def test_graph_cache_policy():
     del d
     gc.collect()
     assert len(cache.seen) == n_seen - 1
+
+
+def test_batch_programs_stacked_equals_per_request_programs():
+    """The LDS lane's stacked assembly of gate-payload requests gives byte for
+    byte the ops and coefficients of concatenating each request's
+    gates_program() with its coefficient offset, and programs passed as
+    (ops, coefs) tuples or su4 tags go through the per-request path."""
+    from blackwell_backend import serving as S
+    rs = [S.su4_request(i, 0.0, n, 64, 1000 + i) for i, n in enumerate([10, 12, 14, 11, 13, 10])]
+    n_ops, n_cf, ops, cf = S._batch_programs(rs)
+    want_ops, want_cf, off = [], [], 0
+    for r in rs:
+        o, c = S.gates_program(*r.gates)
+        o = o.copy()
+        o["coef"] += off
+        off += len(c)
+        want_ops.append(o)
+        want_cf.append(np.asarray(c, dtype=np.complex64))
+    assert ops.tobytes() == np.concatenate(want_ops).tobytes()
+    assert cf.tobytes() == np.concatenate(want_cf).tobytes()
+    assert list(n_ops) == [len(r.gates[0]) for r in rs] and list(n_cf) == [16 * len(r.gates[0]) for r in rs]
+    as_prog = [S.Request(r.rid, 0.0, r.n_qubits, r.shots, r.seed, program=S.gates_program(*r.gates))
+               for r in rs]
+    as_tag = [S.Request(r.rid, 0.0, r.n_qubits, r.shots, r.seed, program=("su4", r.n_qubits, r.n_qubits, r.seed))
+              for r in rs]
+    for alt in (as_prog, as_tag):
+        a_ops, a_cf, o2, c2 = S._batch_programs(alt)
+        assert o2.tobytes() == ops.tobytes() and c2.tobytes() == cf.tobytes()
