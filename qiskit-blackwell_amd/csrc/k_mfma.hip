@@ -119,12 +119,6 @@ __global__ __launch_bounds__(64 * W) void k_block5(float2* __restrict__ st, uint
 // does not depend on what else is in the batch.  The loads of a wave's next
 // set are issued before the current set's MFMA chains when both sets belong
 // to one record.  Row offsets are 32-bit (n <= 32, checked on the host).
-#ifndef QSIM_BATCH_3M
-#define QSIM_BATCH_3M 0
-#endif
-#ifndef QSIM_BATCH_SPW
-#define QSIM_BATCH_SPW 0
-#endif
 __global__ __launch_bounds__(256) void k_block5_batch(const qsim_block_rec* __restrict__ recs,
                                                       int n_recs, uint64_t total_sets, int spw) {
   const int lane = threadIdx.x & 63;
@@ -199,22 +193,6 @@ __global__ __launch_bounds__(256) void k_block5_batch(const qsim_block_rec* __re
       for (int s = 0; s < 16; ++s) y[s] = st[nbase | rof[s]];
     }
     f32x16 accr = {0.f}, acci = {0.f};
-#if QSIM_BATCH_3M
-    {
-      f32x16 p1 = {0.f}, p2 = {0.f}, p3 = {0.f};
-#pragma unroll
-      for (int s = 0; s < 16; ++s) {
-        p1 = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[s], x[s].x, p1, 0, 0, 0);
-        p2 = __builtin_amdgcn_mfma_f32_32x32x2f32(ui[s], x[s].y, p2, 0, 0, 0);
-        p3 = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[s] + ui[s], x[s].x + x[s].y, p3, 0, 0, 0);
-      }
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        accr[r] = p1[r] - p2[r];
-        acci[r] = p3[r] - p1[r] - p2[r];
-      }
-    }
-#else
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
       accr = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[s], x[s].x, accr, 0, 0, 0);
@@ -222,7 +200,6 @@ __global__ __launch_bounds__(256) void k_block5_batch(const qsim_block_rec* __re
       acci = __builtin_amdgcn_mfma_f32_32x32x2f32(ui[s], x[s].x, acci, 0, 0, 0);
       acci = __builtin_amdgcn_mfma_f32_32x32x2f32(ur[s], x[s].y, acci, 0, 0, 0);
     }
-#endif
     // register r of lane (h, c) holds row i = 8 (r / 4) + 4 h + r % 4 of group c
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
@@ -814,7 +791,7 @@ int qsim_apply_block_batch(void* stream, const qsim_block_rec* recs_host, const 
                   (long long)i, (unsigned long long)r.set_begin, (unsigned long long)total);
     total += 1ull << (r.n_qubits - 10);
   }
-  const int spw = QSIM_BATCH_SPW > 0 ? QSIM_BATCH_SPW : (total < 2048 ? 1 : (total < 16384 ? 2 : 4));
+  const int spw = total < 2048 ? 1 : (total < 16384 ? 2 : 4);
   const uint64_t waves = ceil_div(total, spw);
   if (ceil_div(waves, 4) > 0x7fffffffull) return fail(QSIM_ERANGE, "batch too large (%llu sets)",
                                                      (unsigned long long)total);
