@@ -180,8 +180,8 @@ int qsim_plan_blocks(int n_qubits, int k, int n_ops, const int32_t* op_nq, const
                      double* block_mats, int* n_out);
 /* qsim_plan_blocks with a search effort: 0 is qsim_plan_blocks; 1 also
  * scores, for each seed op, every block qubit set made of the seed's qubits
- * and up to k - |seed| of the qubits that the next 40 ops connect to it, and
- * keeps the set that absorbs the most ops (fewer blocks for a few tens of
+ * and up to k - |seed| of the qubits that the next 30 ops connect to it, and
+ * keeps the set that absorbs the most ops (fewer blocks for a few
  * milliseconds of host time on a few hundred ops).  Same outputs.  (new)  */
 int qsim_plan_blocks_search(int n_qubits, int k, int effort, int n_ops, const int32_t* op_nq,
                             const int32_t* op_qubits, const double* op_mats, int32_t* out_ref,

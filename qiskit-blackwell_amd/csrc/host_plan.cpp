@@ -15,9 +15,10 @@
 //    takes 100 blocks (120 with the first op as the only seed).
 //  * effort 1: in addition, every S made of the seed's qubits and up to
 //    k - |seed| of the qubits that the next kSearchWindow ops connect to the
-//    seed is scored.  The depth-30 SU(4) circuit on 30 qubits then takes 110
-//    blocks instead of 119 (28 qubits, depth 28: 93 instead of 100), for a
-//    few tens of milliseconds of host time.
+//    seed is scored.  The depth-30 SU(4) circuit on 30 qubits then takes 108
+//    blocks instead of 119 (28 qubits, depth 28: 92 instead of 100), for about
+//    5 ms of host time.  A 40-op window gave 110 and 93 blocks in twice the
+//    time, a 20-op window 112 and 96.
 // The block matrix is the FP64 product of the members in circuit order, each
 // applied to the rows of the running 2^k x 2^k matrix.  Blocks are padded with
 // the lowest unused qubits to exactly min(k, n) qubits.  An op wider than k
@@ -32,7 +33,7 @@
 namespace qsim {
 
 constexpr int kSeeds = 32;
-constexpr int kSearchWindow = 40;     // ops after the seed whose qubits may join S (effort 1)
+constexpr int kSearchWindow = 30;     // ops after the seed whose qubits may join S (effort 1)
 constexpr int kSearchQubits = 16;     // at most this many candidate qubits per seed
 
 // Ops of rem (in order) that the qubit set S absorbs: an op joins when its
@@ -61,8 +62,59 @@ static int absorb_set(const std::vector<int>& rem, const std::vector<uint64_t>& 
 //
 // rows of (re, im) (row bit b <-> block bit b) <- u (on block bits pos[0..kq))
 // times those rows; u is row-major complex (re, im interleaved).
+// One- and two-qubit ops (the common case) go through local row buffers, so
+// the column loops vectorise without alias checks (with an AVX2 clone where
+// the host has it); the operation order is that of the general form below
+// (each product rounded, then summed over t from 0.0), so both give the same
+// bits.
+template <int KQ>
+__attribute__((target_clones("avx2", "default")))
+static void apply_rows_small(double* re, double* im, int D, const double* u, const int* pos) {
+  constexpr int d = 1 << KQ;
+  constexpr int kMaxD = 1 << QSIM_MAX_BLOCK_QUBITS;
+  double ur[d][d], ui[d][d];
+  for (int s = 0; s < d; ++s)
+    for (int t = 0; t < d; ++t) {
+      ur[s][t] = u[2 * (s * d + t)];
+      ui[s][t] = u[2 * (s * d + t) + 1];
+    }
+  uint32_t pm = 0;
+  for (int j = 0; j < KQ; ++j) pm |= 1u << pos[j];
+  double xr[d][kMaxD], xi[d][kMaxD], yr[d][kMaxD], yi[d][kMaxD];
+  for (int base = 0; base < D; ++base) {
+    if (base & pm) continue;
+    double* pr[d];
+    double* pi[d];
+    for (int s = 0; s < d; ++s) {
+      int r = base;
+      for (int j = 0; j < KQ; ++j)
+        if (s >> j & 1) r |= 1 << pos[j];
+      pr[s] = re + (size_t)r * D;
+      pi[s] = im + (size_t)r * D;
+      memcpy(xr[s], pr[s], sizeof(double) * D);
+      memcpy(xi[s], pi[s], sizeof(double) * D);
+    }
+    for (int s = 0; s < d; ++s)
+      for (int c = 0; c < D; ++c) {
+        double ar = 0.0, ai = 0.0;
+        for (int t = 0; t < d; ++t) {
+          ar += ur[s][t] * xr[t][c] - ui[s][t] * xi[t][c];
+          ai += ur[s][t] * xi[t][c] + ui[s][t] * xr[t][c];
+        }
+        yr[s][c] = ar;
+        yi[s][c] = ai;
+      }
+    for (int s = 0; s < d; ++s) {
+      memcpy(pr[s], yr[s], sizeof(double) * D);
+      memcpy(pi[s], yi[s], sizeof(double) * D);
+    }
+  }
+}
+
 static void apply_rows(double* re, double* im, int D, const double* u, const int* pos, int kq,
                        double* tre, double* tim) {
+  if (kq == 1) return apply_rows_small<1>(re, im, D, u, pos);
+  if (kq == 2) return apply_rows_small<2>(re, im, D, u, pos);
   const int d = 1 << kq;
   uint32_t pm = 0;
   for (int j = 0; j < kq; ++j) pm |= 1u << pos[j];
