@@ -419,7 +419,7 @@ def c5_bench(n=22, n_rhs=32, times=(1.0, 5.0, 10.0), seed=5, precision=1e-7):
     # one right-hand side (the single-state chebyshev_evolve / run_hamiltonian
     # path) at t = 1: the first vector alone
     torch.cuda.synchronize()
-    Y1, info1 = chebyshev_evolve_batch(X[0].contiguous(), h, times[0], precision)
+    Y1, info1 = chebyshev_evolve_batch(X[0].contiguous(), h, times[0], precision, matrix_free=False)
     K1 = info1["chebyshev_degree"]
     per1 = 12.0 * h.nnz + 4.0 * (N + 1) + 5.0 * 8.0 * N
     res["single_rhs"] = {
@@ -427,7 +427,17 @@ def c5_bench(n=22, n_rhs=32, times=(1.0, 5.0, 10.0), seed=5, precision=1e-7):
         "GB_per_s": round(per1 * K1 / info1["elapsed_s"] / 1e9, 1),
         "max_abs_diff_vs_batched": (Y1.to(torch.complex128) -
                                     outs[times[0]][0].to(torch.complex128)).abs().max().item()}
-    del Y1
+    # the same vector with matrix-free terms (the default for one state):
+    # the algorithmic bytes are the five state vectors alone
+    torch.cuda.synchronize()
+    Ym, infom = chebyshev_evolve_batch(X[0].contiguous(), h, times[0], precision, matrix_free=True)
+    Km = infom["chebyshev_degree"]
+    perm = 5.0 * 8.0 * N
+    res["single_rhs_matrix_free"] = {
+        "t": times[0], "K": Km, "ms_per_term": round(infom["elapsed_s"] / Km * 1e3, 4),
+        "GB_per_s": round(perm * Km / infom["elapsed_s"] / 1e9, 1),
+        "max_abs_diff_vs_csr_terms": (Ym.to(torch.complex128) - Y1.to(torch.complex128)).abs().max().item()}
+    del Y1, Ym
     # the opt-in two-pass term over the CSR split at bit n/2, at t = 1
     if n_rhs == 32:
         torch.cuda.synchronize()

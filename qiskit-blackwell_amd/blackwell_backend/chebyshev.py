@@ -25,6 +25,12 @@ more write and read of a partial-sum vector.  On a 22-qubit Heisenberg
 chain at 32 RHS it saves a few per cent per term, so the one-pass term is
 the default.
 
+A Hamiltonian built from a Pauli sum keeps its grouped Pauli tables, and a
+single state with FP64 row sums runs each term matrix-free by default
+(qsim_cheb_pauli_step): every entry is recomputed from the tables as the
+CSR builder computes it, so no index or value stream is read, and the CSR
+stays for batches, fp32c row sums and the split term.
+
 Row sums of the SpMM (accumulate=):
   "fp64"   every FP32 x FP32 product is exact in FP64 and is summed with
            FP64 FMAs (default: the more accurate form, and the cheaper one on
@@ -73,9 +79,15 @@ def chebyshev_coefficients(t, spectral_bound, precision=1e-10):
 ACCUMULATE = {"fp64": 0, "fp32c": 1}
 
 
-def chebyshev_evolve_batch(states, hamiltonian_csr, t, precision=1e-10, accumulate="fp64", split=False):
+def chebyshev_evolve_batch(states, hamiltonian_csr, t, precision=1e-10, accumulate="fp64", split=False,
+                           matrix_free=None):
     """Evolve R states at once.  states: complex64 [R, 2^n] (or [2^n]) on the
-    device.  Returns (evolved [R, 2^n] complex64, info)."""
+    device.  Returns (evolved [R, 2^n] complex64, info).
+
+    matrix_free: run each term from the Pauli tables the Hamiltonian was
+    built from (qsim_cheb_pauli_step) instead of the CSR; None (default)
+    does so for one state with accumulate="fp64" when the Hamiltonian
+    carries its Pauli tables."""
     if accumulate not in ACCUMULATE:
         raise ValueError(f"accumulate must be one of {sorted(ACCUMULATE)}, got {accumulate!r}")
     mode = ACCUMULATE[accumulate]
@@ -90,8 +102,12 @@ def chebyshev_evolve_batch(states, hamiltonian_csr, t, precision=1e-10, accumula
     t_start = time.perf_counter()
     coeffs, K, tail = chebyshev_coefficients(t, h.spectral_bound, precision)
     scale = 1.0 / h.spectral_bound
-    # RHS-contiguous buffers [N, R]
+    # RHS-contiguous buffers [N, R]; A is always a copy (for one state the
+    # transpose is already contiguous and would alias the caller's tensor,
+    # which the recurrence overwrites)
     A = X.transpose(0, 1).contiguous()
+    if A.data_ptr() == X.data_ptr():
+        A = A.clone()
     B = torch.empty_like(A)
     acc = torch.empty_like(A)
     comp = torch.empty_like(A)
@@ -103,7 +119,19 @@ def chebyshev_evolve_batch(states, hamiltonian_csr, t, precision=1e-10, accumula
     if use_split and not (mode == ACCUMULATE["fp64"] and R == 32 and 4 <= N <= (1 << 24)):
         raise ValueError("split=True needs accumulate='fp64', 32 states and 4..2^24 rows "
                          f"(got {accumulate!r}, {R} states, {N} rows)")
-    if use_split:
+    pauli = getattr(h, "pauli", None)
+    mf_ok = pauli is not None and R == 1 and mode == ACCUMULATE["fp64"] and not use_split
+    use_mf = mf_ok if matrix_free is None else bool(matrix_free)
+    if use_mf and not mf_ok:
+        raise ValueError("matrix_free=True needs a Hamiltonian built from a Pauli sum, one state, "
+                         f"accumulate='fp64' and split=False (got {R} states, {accumulate!r})")
+    if use_mf:
+        pargs = (h.n_qubits,) + pauli.args() + (scale,)
+
+        def term(cur_, prv_, ck, first):
+            call("qsim_cheb_pauli_step", s, *pargs, ptr(cur_), ptr(prv_), ptr(acc), ptr(comp),
+                 ck.real, ck.imag, first)
+    elif use_split:
         sb, rp_lo, ci_lo, v_lo, rp_hi, ci_hi, v_hi = h.split()
         w1 = torch.empty_like(A)
         sargs = (ptr(rp_lo), ptr(ci_lo), ptr(v_lo), ptr(rp_hi), ptr(ci_hi), ptr(v_hi), N, sb, R, scale)
@@ -139,11 +167,13 @@ def chebyshev_evolve_batch(states, hamiltonian_csr, t, precision=1e-10, accumula
         "n_rhs": R,
         "accumulate": accumulate,
         "split_term": bool(use_split),
+        "matrix_free": bool(use_mf),
     }
     return (out[0] if single else out), info
 
 
-def chebyshev_evolve(state, hamiltonian_csr, t, precision=1e-10, bk=None, accumulate="fp64"):
+def chebyshev_evolve(state, hamiltonian_csr, t, precision=1e-10, bk=None, accumulate="fp64",
+                     matrix_free=None):
     """exp(-iHt)|state> for one state [2^n]; returns (result, info)
     (chebyshev.py:81-172).  `bk` is accepted for signature parity."""
-    return chebyshev_evolve_batch(state, hamiltonian_csr, t, precision, accumulate)
+    return chebyshev_evolve_batch(state, hamiltonian_csr, t, precision, accumulate, matrix_free=matrix_free)

@@ -85,20 +85,36 @@ def pauli_groups(labels, coeffs):
             np.array(coef, dtype=np.complex128))
 
 
-def pauli_to_csr_device(labels, coeffs, n, device):
+class PauliTables:
+    """pauli_groups() on a device: n_groups and the tensors xm (uint64 as
+    int64), toff (int32), sm (uint64 as int64), coef (FP64 pairs), the
+    layout qsim_pauli_csr_count and qsim_cheb_pauli_step take."""
+
+    def __init__(self, labels, coeffs, device):
+        xm, toff, sm, coef = pauli_groups(labels, coeffs)
+        dev = torch.device(device)
+        self.n_groups = len(xm)
+        self.xm = torch.from_numpy(xm.view(np.int64) if xm.size else np.zeros(1, np.int64)).to(dev)
+        self.toff = torch.from_numpy(toff).to(dev)
+        self.sm = torch.from_numpy(sm.view(np.int64) if sm.size else np.zeros(1, np.int64)).to(dev)
+        self.coef = torch.from_numpy(coef.view(np.float64) if coef.size else np.zeros(2)).to(dev)
+
+    def args(self):
+        from blackwell_kernels import _lib
+        return (self.n_groups, _lib.ptr(self.xm), _lib.ptr(self.toff), _lib.ptr(self.sm), _lib.ptr(self.coef))
+
+
+def pauli_to_csr_device(labels, coeffs, n, device, tables=None):
     """(row_ptr int32 [2^n+1], col_idx int32 [nnz], values complex64 [nnz])
-    as device tensors, built by the library from pauli_groups."""
+    as device tensors, built by the library from pauli_groups (or from
+    `tables`, a PauliTables of the same terms)."""
     import ctypes
     from blackwell_kernels import _lib
-    xm, toff, sm, coef = pauli_groups(labels, coeffs)
+    pt = tables if tables is not None else PauliTables(labels, coeffs, device)
     dev = torch.device(device)
-    d_xm = torch.from_numpy(xm.view(np.int64)).to(dev)
-    d_toff = torch.from_numpy(toff).to(dev)
-    d_sm = torch.from_numpy(sm.view(np.int64) if sm.size else np.zeros(1, np.int64)).to(dev)
-    d_cf = torch.from_numpy(coef.view(np.float64) if coef.size else np.zeros(2)).to(dev)
     ws = torch.empty(int(_lib.load().qsim_pauli_csr_workspace_bytes(n)), dtype=torch.uint8, device=dev)
     s = _lib.stream_handle(dev)
-    args = (n, len(xm), _lib.ptr(d_xm), _lib.ptr(d_toff), _lib.ptr(d_sm), _lib.ptr(d_cf), _lib.ptr(ws))
+    args = (n,) + pt.args() + (_lib.ptr(ws),)
     nnz = ctypes.c_int64(0)
     _lib.call("qsim_pauli_csr_count", s, *args, ctypes.byref(nnz))
     row_ptr = torch.empty((1 << n) + 1, dtype=torch.int32, device=dev)
@@ -112,7 +128,10 @@ def pauli_to_csr_device(labels, coeffs, n, device):
 class HamiltonianCSR:
     """CSR Hamiltonian on a device: row_ptr/col_idx int32, values complex64."""
 
-    def __init__(self, row_ptr, col_idx, values, n_qubits, spectral_bound):
+    def __init__(self, row_ptr, col_idx, values, n_qubits, spectral_bound, pauli=None):
+        # pauli: the PauliTables the CSR was built from, when it came from a
+        # Pauli sum (single-state Chebyshev terms then run matrix-free)
+        self.pauli = pauli
         self.row_ptr = row_ptr
         self.col_idx = col_idx
         self.values = values
@@ -124,8 +143,9 @@ class HamiltonianCSR:
     @staticmethod
     def from_sparse_pauli_op(spo, device="cuda"):
         labels, coeffs, n = _labels_and_coeffs(spo)
-        rp, ci, v = pauli_to_csr_device(labels, coeffs, n, device)
-        return HamiltonianCSR(rp, ci, v, n, float(np.sum(np.abs(coeffs))))
+        pt = PauliTables(labels, coeffs, device)
+        rp, ci, v = pauli_to_csr_device(labels, coeffs, n, device, tables=pt)
+        return HamiltonianCSR(rp, ci, v, n, float(np.sum(np.abs(coeffs))), pauli=pt)
 
     def memory_bytes(self):
         return self.row_ptr.nbytes + self.col_idx.nbytes + self.values.nbytes

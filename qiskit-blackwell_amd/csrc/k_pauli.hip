@@ -23,30 +23,13 @@ namespace {
 
 constexpr int kCsrThreads = 256;
 
-__device__ __forceinline__ bool group_value(uint64_t row, int t0, int t1,
-                                            const uint64_t* __restrict__ sm,
-                                            const double2* __restrict__ coef, double& re,
-                                            double& im) {
-  re = 0.0;
-  im = 0.0;
-  for (int t = t0; t < t1; ++t) {
-    const double s = (__popcll(row & sm[t]) & 1) ? -1.0 : 1.0;
-    const double2 c = coef[t];
-    const double xr = c.x * s - c.y * 0.0;
-    const double xi = c.x * 0.0 + c.y * s;
-    re += xr;
-    im += xi;
-  }
-  return re != 0.0 || im != 0.0;
-}
-
 __device__ __forceinline__ int row_count(uint64_t row, int n_groups, const int* __restrict__ toff,
                                          const uint64_t* __restrict__ sm,
                                          const double2* __restrict__ coef) {
   int c = 0;
   for (int g = 0; g < n_groups; ++g) {
     double re, im;
-    c += group_value(row, toff[g], toff[g + 1], sm, coef, re, im) ? 1 : 0;
+    c += pauli_group_value<uint64_t>(row, toff[g], toff[g + 1], sm, coef, re, im) ? 1 : 0;
   }
   return c;
 }
@@ -113,7 +96,7 @@ k_csr_fill(uint64_t n_rows, int n_groups, const uint64_t* __restrict__ xm,
   row_ptr[row] = (int32_t)off;
   for (int g = 0; g < n_groups; ++g) {
     double re, im;
-    if (group_value(row, toff[g], toff[g + 1], sm, coef, re, im)) {
+    if (pauli_group_value<uint64_t>(row, toff[g], toff[g + 1], sm, coef, re, im)) {
       col_idx[off] = (int32_t)(row ^ xm[g]);
       values[off] = make_float2((float)re, (float)im);
       ++off;

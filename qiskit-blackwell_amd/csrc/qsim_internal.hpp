@@ -126,6 +126,29 @@ __device__ __forceinline__ float2 crow4(const float2* g, float2 a0, float2 a1,
 }
 
 // Compensated complex product a*d.
+// Entry of one X-mask group of a Pauli sum in row `row`: sum over the
+// group's terms t in [t0, t1) of coef[t] (-1)^popcount(row & sm[t]), in FP64
+// and term order (each term enters with its sign flipped or not, which is
+// exact); false when the sum is exactly zero.  The device CSR builder and the
+// matrix-free Chebyshev term both evaluate entries with this function, so
+// they see the same values.  R is the row index type (uint32_t when every
+// row index fits in 32 bits).
+template <typename R>
+__device__ __forceinline__ bool pauli_group_value(R row, int t0, int t1, const uint64_t* __restrict__ sm,
+                                                  const double2* __restrict__ coef, double& re,
+                                                  double& im) {
+  re = 0.0;
+  im = 0.0;
+  for (int t = t0; t < t1; ++t) {
+    const bool odd = sizeof(R) == 4 ? (__popc((uint32_t)row & (uint32_t)sm[t]) & 1)
+                                    : (__popcll((uint64_t)row & sm[t]) & 1);
+    const double2 c = coef[t];
+    re += odd ? -c.x : c.x;
+    im += odd ? -c.y : c.y;
+  }
+  return re != 0.0 || im != 0.0;
+}
+
 __device__ __forceinline__ float2 cmulc(float2 a, float2 d) {
   float p0, e0, p1, e1;
   float2 r;

@@ -577,6 +577,109 @@ def test_cheb_spmm_step_matches_fp64(bk, dev, accumulate, n_rows):
                 assert np.array_equal(both.view(np.uint32), tpd.cpu().numpy().view(np.uint32)), first
 
 
+def _rand_pauli_sum(n, n_terms, seed):
+    """Random Pauli labels over I/X/Y/Z with real coefficients; every X mask
+    appears with several Z/Y patterns (groups of more than one term), one
+    term is the identity, and one pair of terms cancels on half the rows."""
+    rng = np.random.default_rng(seed)
+    labels, coeffs = [], []
+    for _ in range(n_terms):
+        lab = "".join(rng.choice(list("IXYZ"), n, p=[0.55, 0.15, 0.15, 0.15]))
+        labels.append(lab)
+        coeffs.append(float(rng.standard_normal()))
+        # a partner with the same X mask and a different Z/Y pattern
+        twin = "".join({"X": "Y", "Y": "X", "Z": "I", "I": "Z"}[ch] if rng.random() < 0.3 else ch for ch in lab)
+        labels.append(twin)
+        coeffs.append(float(rng.standard_normal()))
+    labels.append("I" * n)
+    coeffs.append(0.75)
+    labels += ["X" + "I" * (n - 1), "X" + "Z" + "I" * (n - 2)]   # 0.5 X (1 + Z): zero where bit n-2 is set
+    coeffs += [0.5, 0.5]
+    return labels, coeffs
+
+
+@pytest.mark.parametrize("n", [6, 11, 14])
+def test_cheb_pauli_step_matches_csr_and_fp64(bk, dev, n):
+    """qsim_cheb_pauli_step (matrix-free, one state) against the same term in
+    FP64 on the host over the oracle's Kronecker-built matrix, and against the
+    CSR term (qsim_cheb_spmm_step, 1 RHS, FP64 row sums) on the device CSR of
+    the same Pauli sum: equal up to the order of the FP64 row sum (one
+    rounding to complex64), for the first term and a later one."""
+    from oracle import hamiltonian as OH
+    from blackwell_backend.hamiltonian import HamiltonianCSR, SparsePauliOp
+    from blackwell_backend.chebyshev import ACCUMULATE
+    from blackwell_kernels._lib import call, ptr, stream_handle
+    labels, coeffs = _rand_pauli_sum(n, 3 * n, 100 + n)
+    h = HamiltonianCSR.from_sparse_pauli_op(SparsePauliOp.from_list(list(zip(labels, coeffs))), device=dev)
+    assert h.pauli is not None and h.pauli.n_groups > 1
+    A64 = OH.pauli_sparse(labels, coeffs)
+    A64 = A64.astype(np.complex64).astype(np.complex128)     # the entries rounded as the device stores them
+    absA = abs(A64)
+    N = 1 << n
+    rng = np.random.default_rng(n)
+    scale, c = 0.21, 0.3 - 0.45j
+    cplx = lambda: (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(np.complex64)
+    T, Tp, acc0 = cplx(), cplx(), cplx()
+    s = stream_handle(dev)
+    for first in (1, 0):
+        outs = {}
+        for how in ("pauli", "csr"):
+            tpd, accd = to_dev(Tp, dev), to_dev(acc0, dev)
+            compd = torch.zeros_like(accd)
+            if how == "pauli":
+                call("qsim_cheb_pauli_step", s, n, *h.pauli.args(), scale, ptr(to_dev(T, dev)), ptr(tpd),
+                     ptr(accd), ptr(compd), c.real, c.imag, first)
+            else:
+                call("qsim_cheb_spmm_step", s, ptr(h.row_ptr), ptr(h.col_idx), ptr(h.values), N, 1, scale,
+                     ptr(to_dev(T, dev)), ptr(tpd), ptr(accd), ptr(compd), c.real, c.imag, first,
+                     ACCUMULATE["fp64"])
+            outs[how] = (tpd.cpu().numpy().astype(np.complex128),
+                         accd.cpu().numpy().astype(np.complex128) + compd.cpu().numpy())
+        w = scale * (A64 @ T.astype(np.complex128))
+        tn_ref = w if first else 2 * w - Tp.astype(np.complex128)
+        bound = scale * (absA @ np.abs(T.astype(np.complex128))) * 2 + np.abs(Tp)
+        tol = 2.0 ** -23 * np.abs(tn_ref) + 1e-15 * bound + 1e-30
+        for how, (tn, acc) in outs.items():
+            assert np.all(np.abs(tn - tn_ref) <= tol), (how, first, float(np.max(np.abs(tn - tn_ref) - tol)))
+            want_acc = acc0.astype(np.complex128) + c * tn
+            assert np.max(np.abs(acc - want_acc)) <= 1e-6 * np.max(np.abs(want_acc)), (how, first)
+        d = np.abs(outs["pauli"][0] - outs["csr"][0])
+        assert np.all(d <= 2.0 ** -22 * np.abs(outs["csr"][0]) + 1e-15 * bound + 1e-30), (first, float(d.max()))
+
+
+def test_chebyshev_matrix_free_equals_csr_evolution(bk, dev):
+    """exp(-iHt)|psi> on a 12-qubit chain with the matrix-free terms (the
+    default for one state) and with the CSR terms: the same result to 1e-6,
+    and both against FP64 expm_multiply; the batch path and fp32c row sums
+    stay on the CSR, and matrix_free=True is refused where it cannot run.
+    The caller's state is not modified (for one state the RHS-contiguous
+    transpose is a view of it, so the evolution works on a copy)."""
+    from oracle import hamiltonian as OH
+    from blackwell_backend import workloads as W
+    from blackwell_backend.hamiltonian import HamiltonianCSR
+    from blackwell_backend.chebyshev import chebyshev_evolve, chebyshev_evolve_batch
+    spo = W.heisenberg_chain(12, seed=4)
+    h = HamiltonianCSR.from_sparse_pauli_op(spo, device=dev)
+    rng = np.random.default_rng(3)
+    psi = rng.standard_normal(1 << 12) + 1j * rng.standard_normal(1 << 12)
+    psi = (psi / np.linalg.norm(psi)).astype(np.complex64)
+    x = to_dev(psi, dev)
+    a, ia = chebyshev_evolve(x, h, 2.0, 1e-9)
+    assert np.array_equal(x.cpu().numpy(), psi)          # the input state is left as it was
+    b, ib = chebyshev_evolve(x, h, 2.0, 1e-9, matrix_free=False)
+    assert ia["matrix_free"] and not ib["matrix_free"]
+    a, b = a.cpu().numpy(), b.cpu().numpy()
+    assert np.max(np.abs(a - b)) < 1e-6
+    ref = OH.expm_reference(OH.pauli_sparse(spo.labels, spo.coeffs), psi.astype(np.complex128), 2.0)
+    assert np.max(np.abs(a - ref)) < 1e-5
+    _, ibatch = chebyshev_evolve_batch(torch.stack([x, x]), h, 0.5, 1e-8)
+    assert not ibatch["matrix_free"]
+    _, ic = chebyshev_evolve(x, h, 0.5, 1e-8, accumulate="fp32c")
+    assert not ic["matrix_free"]
+    with pytest.raises(ValueError):
+        chebyshev_evolve_batch(torch.stack([x, x]), h, 0.5, 1e-8, matrix_free=True)
+
+
 def test_csr_split_and_split_term(bk, dev):
     """qsim_csr_split_count / _fill split a ragged CSR of 2^10 rows at bit 4:
     the low part holds exactly the entries with (row ^ col) < 16 of each
