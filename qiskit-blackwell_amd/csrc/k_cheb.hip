@@ -597,10 +597,12 @@ __global__ __launch_bounds__(kThreads) void k_cheb_spmv1(const int32_t* __restri
 // are uniform across the wave, so the tables come in through scalar loads.
 // No index or value stream is read: per term the only HBM traffic is the
 // five state vectors, and the gathers of T_k hit the caches (T_k is 32 MiB
-// at 22 qubits).  On the 22-qubit chain a term takes 0.26 ms against 0.70 ms
-// for the CSR term; walking XCD-contiguous row ranges measured the same, and
-// a cheaper sign evaluation changed nothing, so neither the L2 reach nor the
-// entry arithmetic bounds it.
+// at 22 qubits).  The kernel is VALU-bound (VALUBusy 99 % at 22 qubits): each
+// term of an entry costs an AND, a popcount, one shift-or that puts the
+// parity into the sign of 1.0, and two FP64 FMAs; with that form a
+// 22-qubit chain term takes 0.20 ms against 0.26 ms with per-term selects
+// and 0.70 ms for the CSR term.  Walking XCD-contiguous row ranges
+// measured the same.
 constexpr int kPauliRun = 8;   // 4: same time, 16: 3 % slower, 1: 10 % slower (22-qubit chain)
 template <typename R, bool NT>
 __global__ __launch_bounds__(kThreads) void k_cheb_pauli1(int n_groups, const uint64_t* __restrict__ xm,

@@ -140,11 +140,14 @@ __device__ __forceinline__ bool pauli_group_value(R row, int t0, int t1, const u
   re = 0.0;
   im = 0.0;
   for (int t = t0; t < t1; ++t) {
-    const bool odd = sizeof(R) == 4 ? (__popc((uint32_t)row & (uint32_t)sm[t]) & 1)
-                                    : (__popcll((uint64_t)row & sm[t]) & 1);
+    const uint32_t pc = sizeof(R) == 4 ? (uint32_t)__popc((uint32_t)row & (uint32_t)sm[t])
+                                       : (uint32_t)__popcll((uint64_t)row & sm[t]);
+    // +-1.0 with the parity of pc in the sign bit; c * (+-1) is exact, so
+    // each FMA rounds exactly as re +- c would
+    const double sgn = __hiloint2double((int)((pc << 31) | 0x3ff00000u), 0);
     const double2 c = coef[t];
-    re += odd ? -c.x : c.x;
-    im += odd ? -c.y : c.y;
+    re = __builtin_fma(c.x, sgn, re);
+    im = __builtin_fma(c.y, sgn, im);
   }
   return re != 0.0 || im != 0.0;
 }
