@@ -597,13 +597,38 @@ __global__ __launch_bounds__(kThreads) void k_cheb_spmv1(const int32_t* __restri
 // are uniform across the wave, so the tables come in through scalar loads.
 // No index or value stream is read: per term the only HBM traffic is the
 // five state vectors, and the gathers of T_k hit the caches (T_k is 32 MiB
-// at 22 qubits).  The kernel is VALU-bound (VALUBusy 99 % at 22 qubits): each
-// term of an entry costs an AND, a popcount, one shift-or that puts the
-// parity into the sign of 1.0, and two FP64 FMAs; with that form a
-// 22-qubit chain term takes 0.20 ms against 0.26 ms with per-term selects
-// and 0.70 ms for the CSR term.  Walking XCD-contiguous row ranges
-// measured the same.
+// at 22 qubits).  Each term of an entry costs an AND, a popcount, one
+// shift-or that puts the parity into the sign of 1.0, and two FP64 FMAs
+// (per-term selects of +-c took 18 instructions); a 22-qubit chain term
+// takes 0.16 ms against 0.70 ms for the CSR term.  Walking XCD-contiguous
+// row ranges measured the same.
 constexpr int kPauliRun = 8;   // 4: same time, 16: 3 % slower, 1: 10 % slower (22-qubit chain)
+// rows per thread, kThreads apart: the scalar table loads and the loop
+// control serve two rows (22-qubit chain: 1 row 0.206 ms, 2 rows 0.164 ms,
+// 4 rows 0.182 ms per term, same bits)
+constexpr int kPauliRows = 2;
+// Entries of one group g for RR rows (uniform group, per-row values).
+template <typename R, int RR>
+__device__ __forceinline__ void pauli_group_rows(const R (&r)[RR], int t0, int t1,
+                                                 const uint64_t* __restrict__ sm,
+                                                 const double2* __restrict__ coef, double (&re)[RR],
+                                                 double (&im)[RR]) {
+#pragma unroll
+  for (int q = 0; q < RR; ++q) re[q] = im[q] = 0.0;
+  for (int t = t0; t < t1; ++t) {
+    const uint64_t m = sm[t];
+    const double2 c = coef[t];
+#pragma unroll
+    for (int q = 0; q < RR; ++q) {
+      const uint32_t pc = sizeof(R) == 4 ? (uint32_t)__popc((uint32_t)r[q] & (uint32_t)m)
+                                         : (uint32_t)__popcll((uint64_t)r[q] & m);
+      const double sgn = __hiloint2double((int)((pc << 31) | 0x3ff00000u), 0);
+      re[q] = __builtin_fma(c.x, sgn, re[q]);
+      im[q] = __builtin_fma(c.y, sgn, im[q]);
+    }
+  }
+}
+
 template <typename R, bool NT>
 __global__ __launch_bounds__(kThreads) void k_cheb_pauli1(int n_groups, const uint64_t* __restrict__ xm,
                                                           const int* __restrict__ toff,
@@ -615,39 +640,55 @@ __global__ __launch_bounds__(kThreads) void k_cheb_pauli1(int n_groups, const ui
                                                           float2* __restrict__ acc,
                                                           float2* __restrict__ comp, float2 c,
                                                           int first) {
-  const int64_t row = gblock<int64_t>() * kThreads + threadIdx.x;
-  if (row >= n_rows) return;
-  const R r = (R)row, rmask = (R)(n_rows - 1);   // n_rows = 2^n: every column stays in range
-  RowSum<kAccF64> sum;
+  constexpr int RR = kPauliRows;
+  const int64_t row0 = gblock<int64_t>() * (kThreads * RR) + threadIdx.x;
+  if (row0 >= n_rows) return;
+  const R rmask = (R)(n_rows - 1);        // n_rows = 2^n: every column stays in range
+  R r[RR];
+#pragma unroll
+  for (int q = 0; q < RR; ++q) r[q] = (R)(row0 + q * kThreads) & rmask;   // rows past the end are not stored
+  RowSum<kAccF64> sum[RR];
   // groups in runs of kPauliRun: the run's gathers are issued together (an
   // entry that turns out to be zero is gathered and not used), then the
   // values are formed and added in group order
   int g = 0;
   for (; g + kPauliRun <= n_groups; g += kPauliRun) {
-    float2 x[kPauliRun];
+    float2 x[kPauliRun][RR];
 #pragma unroll
-    for (int j = 0; j < kPauliRun; ++j) x[j] = tc[(r ^ (R)xm[g + j]) & rmask];
+    for (int j = 0; j < kPauliRun; ++j)
+#pragma unroll
+      for (int q = 0; q < RR; ++q) x[j][q] = tc[(r[q] ^ (R)xm[g + j]) & rmask];
 #pragma unroll
     for (int j = 0; j < kPauliRun; ++j) {
-      double re, im;
-      if (pauli_group_value<R>(r, toff[g + j], toff[g + j + 1], sm, coef, re, im))
-        sum.add(make_float2((float)re, (float)im), x[j]);
+      double re[RR], im[RR];
+      pauli_group_rows<R, RR>(r, toff[g + j], toff[g + j + 1], sm, coef, re, im);
+#pragma unroll
+      for (int q = 0; q < RR; ++q)
+        if (re[q] != 0.0 || im[q] != 0.0) sum[q].add(make_float2((float)re[q], (float)im[q]), x[j][q]);
     }
   }
   for (; g < n_groups; ++g) {
-    double re, im;
-    if (!pauli_group_value<R>(r, toff[g], toff[g + 1], sm, coef, re, im)) continue;
-    sum.add(make_float2((float)re, (float)im), tc[(r ^ (R)xm[g]) & rmask]);
+    double re[RR], im[RR];
+    pauli_group_rows<R, RR>(r, toff[g], toff[g + 1], sm, coef, re, im);
+#pragma unroll
+    for (int q = 0; q < RR; ++q)
+      if (re[q] != 0.0 || im[q] != 0.0)
+        sum[q].add(make_float2((float)re[q], (float)im[q]), tc[(r[q] ^ (R)xm[g]) & rmask]);
   }
-  const float2 tp = first ? make_float2(0.f, 0.f) : ld2<NT>(tpn + row);
-  const float2 tn = sum.next(scale, tp, first != 0);
-  st2<NT>(tpn + row, tn);
-  const float2 v = cmulc(c, tn);
-  float2 a = ld2<NT>(acc + row), k2 = ld2<NT>(comp + row);
-  neu_add(a.x, k2.x, v.x);
-  neu_add(a.y, k2.y, v.y);
-  st2<NT>(acc + row, a);
-  st2<NT>(comp + row, k2);
+#pragma unroll
+  for (int q = 0; q < RR; ++q) {
+    const int64_t row = row0 + q * kThreads;
+    if (row >= n_rows) break;
+    const float2 tp = first ? make_float2(0.f, 0.f) : ld2<NT>(tpn + row);
+    const float2 tn = sum[q].next(scale, tp, first != 0);
+    st2<NT>(tpn + row, tn);
+    const float2 v = cmulc(c, tn);
+    float2 a = ld2<NT>(acc + row), k2 = ld2<NT>(comp + row);
+    neu_add(a.x, k2.x, v.x);
+    neu_add(a.y, k2.y, v.y);
+    st2<NT>(acc + row, a);
+    st2<NT>(comp + row, k2);
+  }
 }
 
 __global__ __launch_bounds__(kThreads) void k_cheb_init(float2* __restrict__ acc,
@@ -795,7 +836,7 @@ int qsim_cheb_pauli_step(void* stream, int n_qubits, int n_groups, const void* x
   const bool nt = nt_for_bytes((uint64_t)n_rows * 8u);
   auto* kern = n_qubits > 32 ? (nt ? k_cheb_pauli1<uint64_t, true> : k_cheb_pauli1<uint64_t, false>)
                              : (nt ? k_cheb_pauli1<uint32_t, true> : k_cheb_pauli1<uint32_t, false>);
-  hipLaunchKernelGGL(kern, grid1d(ceil_div((uint64_t)n_rows, kThreads)), dim3(kThreads), 0,
+  hipLaunchKernelGGL(kern, grid1d(ceil_div((uint64_t)n_rows, kThreads * kPauliRows)), dim3(kThreads), 0,
                      as_stream(stream), n_groups, (const uint64_t*)xm, (const int*)toff,
                      (const uint64_t*)sm, (const double2*)coef, n_rows, scale, (const float2*)t_cur,
                      (float2*)t_prev_next, (float2*)acc, (float2*)comp, f2(c_re, c_im), first);
