@@ -127,13 +127,7 @@ __device__ __forceinline__ void load_m2(const float2* gdev, const M2& gv, float2
 }
 
 // MODE 0: t = 0, MODE 1: t = 1..6 (cross-lane), MODE 2: t >= 7 (streams).
-// MODE 3: t >= 7 with a second pair 2^11 slots away (bit 11 of the slot
-// index, qubit 12), so that a wave's accesses cover four
-// regions {0, 32 KiB} x {0, partner distance} instead of two: at 30 qubits
-// a far partner alone (t = 21..29) runs at 71-72 % of the HBM peak, with the
-// extra pair at 75-76 % (measured on MI355X).
-// n_work = float4 slots (modes 0, 1), slot pairs (mode 2) or pairs of pairs
-// (mode 3).  Batched states
+// n_work = float4 slots (modes 0, 1) or slot pairs (mode 2).  Batched states
 // need no extra index math: the flat slot index has the state number above
 // bit n-2 and t < n keeps every pair inside one state.
 template <typename I, int MODE, bool NT>
@@ -161,27 +155,13 @@ __global__ __launch_bounds__(kThreads) void k_gate1q(float4* __restrict__ st, I 
     const float2 ga = low ? g[0] : g[3], gb = low ? g[1] : g[2];
     const float4 r = cat4(crow2(ga, lo2(v), gb, lo2(p)), crow2(ga, hi2(v), gb, hi2(p)));
     if (live) st4p<NT>(st + w, r, tail);
-  } else if (MODE == 2) {
+  } else {
     const int s = t - 1;
     const I f0 = ins0(w, s), f1 = f0 + (I(1) << s);
     const float4 v0 = ld4<NT>(st + f0), v1 = ld4<NT>(st + f1);
     const float4 r0 = cat4(crow2(g[0], lo2(v0), g[1], lo2(v1)), crow2(g[0], hi2(v0), g[1], hi2(v1)));
     const float4 r1 = cat4(crow2(g[2], lo2(v0), g[3], lo2(v1)), crow2(g[2], hi2(v0), g[3], hi2(v1)));
     if (live) { st4p<NT>(st + f0, r0, tail); st4p<NT>(st + f1, r1, tail); }
-  } else {
-    const int s = t - 1, ps = 11;
-    const I f0 = ins0(ins0(w, s < ps ? s : ps), s < ps ? ps : s);
-    const I ms = I(1) << s, mp = I(1) << ps;
-    const float4 v0 = ld4<NT>(st + f0), v1 = ld4<NT>(st + f0 + ms);
-    const float4 v2 = ld4<NT>(st + f0 + mp), v3 = ld4<NT>(st + f0 + mp + ms);
-    const float4 r0 = cat4(crow2(g[0], lo2(v0), g[1], lo2(v1)), crow2(g[0], hi2(v0), g[1], hi2(v1)));
-    const float4 r1 = cat4(crow2(g[2], lo2(v0), g[3], lo2(v1)), crow2(g[2], hi2(v0), g[3], hi2(v1)));
-    const float4 r2 = cat4(crow2(g[0], lo2(v2), g[1], lo2(v3)), crow2(g[0], hi2(v2), g[1], hi2(v3)));
-    const float4 r3 = cat4(crow2(g[2], lo2(v2), g[3], lo2(v3)), crow2(g[2], hi2(v2), g[3], hi2(v3)));
-    if (live) {
-      st4p<NT>(st + f0, r0, tail); st4p<NT>(st + f0 + ms, r1, tail);
-      st4p<NT>(st + f0 + mp, r2, tail); st4p<NT>(st + f0 + mp + ms, r3, tail);
-    }
   }
 }
 
@@ -215,31 +195,27 @@ int gate1q_impl(void* stream, void* states, uint64_t n_states, int n, const floa
     return launch_status("apply_gate");
   }
   const uint64_t slots = (1ull << n) / 2 * n_states;
-  // mode 3 for targets >= 7 other than 12 and 13 (whose partners already sit
-  // 32-64 KiB away) on states that stream from HBM; MALL-resident states
-  // measured 1-3 % slower with it
-  int mode = t == 0 ? 0 : (t < 7 ? 1 : 2);
-  if (mode == 2 && t != 12 && t != 13 && nt_for_bytes(slots * 16)) mode = 3;
-  const uint64_t work = mode == 3 ? slots / 4 : (mode == 2 ? slots / 2 : slots);
+  // a second pair 32 KiB away per thread for far targets measured 2-6
+  // points slower at 30 qubits (t = 19: 74 % against 81 % of the HBM peak),
+  // so every stream target runs one pair per thread
+  const int mode = t == 0 ? 0 : (t < 7 ? 1 : 2);
+  const uint64_t work = mode == 2 ? slots / 2 : slots;
   const PassOrder po = nt_for_bytes(slots * 16) ? next_pass(slots * 16) : PassOrder{0, 0};
   const dim3 grid = grid1d(ceil_div(work, kThreads)), block(kThreads);
   if (slots > 0xffffffffull) {
     if (mode == 0) hipLaunchKernelGGL((k_gate1q<uint64_t, 0, true>), grid, block, 0, s, (float4*)states, work, t, gdev, gv, po);
     if (mode == 1) hipLaunchKernelGGL((k_gate1q<uint64_t, 1, true>), grid, block, 0, s, (float4*)states, work, t, gdev, gv, po);
     if (mode == 2) hipLaunchKernelGGL((k_gate1q<uint64_t, 2, true>), grid, block, 0, s, (float4*)states, work, t, gdev, gv, po);
-    if (mode == 3) hipLaunchKernelGGL((k_gate1q<uint64_t, 3, true>), grid, block, 0, s, (float4*)states, work, t, gdev, gv, po);
   } else if (nt_for_bytes(slots * 16)) {
     const uint32_t w = (uint32_t)work;
     if (mode == 0) hipLaunchKernelGGL((k_gate1q<uint32_t, 0, true>), grid, block, 0, s, (float4*)states, w, t, gdev, gv, po);
     if (mode == 1) hipLaunchKernelGGL((k_gate1q<uint32_t, 1, true>), grid, block, 0, s, (float4*)states, w, t, gdev, gv, po);
     if (mode == 2) hipLaunchKernelGGL((k_gate1q<uint32_t, 2, true>), grid, block, 0, s, (float4*)states, w, t, gdev, gv, po);
-    if (mode == 3) hipLaunchKernelGGL((k_gate1q<uint32_t, 3, true>), grid, block, 0, s, (float4*)states, w, t, gdev, gv, po);
   } else {
     const uint32_t w = (uint32_t)work;
     if (mode == 0) hipLaunchKernelGGL((k_gate1q<uint32_t, 0, false>), grid, block, 0, s, (float4*)states, w, t, gdev, gv, po);
     if (mode == 1) hipLaunchKernelGGL((k_gate1q<uint32_t, 1, false>), grid, block, 0, s, (float4*)states, w, t, gdev, gv, po);
     if (mode == 2) hipLaunchKernelGGL((k_gate1q<uint32_t, 2, false>), grid, block, 0, s, (float4*)states, w, t, gdev, gv, po);
-    if (mode == 3) hipLaunchKernelGGL((k_gate1q<uint32_t, 3, false>), grid, block, 0, s, (float4*)states, w, t, gdev, gv, po);
   }
   return launch_status("apply_gate");
 }
