@@ -200,8 +200,8 @@ def test_33_qubit_state_64bit_paths():
 
 def test_hbm_streaming_paths_26q_match_oracle():
     """At 26 qubits (512 MiB) the state is larger than the 256 MB MALL, so the
-    kernels take their nontemporal forms, and 1Q stream targets other than
-    12 and 13 take the form with a second pair 32 KiB away.  A sequence
+    kernels take their nontemporal forms and the pass order across
+    launches (alternating direction, cached stores on each pass's tail).  A sequence
     covering those paths is compared with the FP64 oracle on the same
     complex64 input (compensated FP32 bar, relative to max |a|), and the
     permutation gates in it are checked bit for bit on their own."""
