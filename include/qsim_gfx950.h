@@ -463,13 +463,16 @@ int qsim_cheb_spmm_step_split(void* stream, const void* rp_lo, const void* ci_lo
  * values of qsim_pauli_csr_fill (FP64 in term order, rounded to complex64,
  * exact zeros skipped) at columns i ^ xm[g], summed in group order in FP64;
  * the result equals the CSR term's up to the order of that FP64 sum.  Reads
- * no index or value stream, only the state vectors.
+ * no index or value stream, only the state vectors.  gconst (device, 2
+ * doubles per group, or NULL) holds each group's complex64-rounded entry
+ * when no term of the group has a Y|Z mask (the entry is then the same in
+ * every row), else NaN in its real part.
  * (new; the reference builds the CSR and runs one SpMV per term:
  *  blackwell_backend/hamiltonian.py:44-69, bwk/csrc/spmv_sm120.cu:108-142) */
 int qsim_cheb_pauli_step(void* stream, int n_qubits, int n_groups, const void* xm, const void* toff,
-                         const void* sm, const void* coef, double scale, const void* t_cur,
-                         void* t_prev_next, void* acc, void* comp, double c_re, double c_im,
-                         int first);
+                         const void* sm, const void* coef, const void* gconst, double scale,
+                         const void* t_cur, void* t_prev_next, void* acc, void* comp, double c_re,
+                         double c_im, int first);
 /* acc = c * t, comp = 0 (first Chebyshev term), n elements. (new)        */
 int qsim_cheb_init_acc(void* stream, void* acc, void* comp, const void* t, int64_t n,
                        double c_re, double c_im);

@@ -126,7 +126,7 @@ def chebyshev_evolve_batch(states, hamiltonian_csr, t, precision=1e-10, accumula
         raise ValueError("matrix_free=True needs a Hamiltonian built from a Pauli sum, one state, "
                          f"accumulate='fp64' and split=False (got {R} states, {accumulate!r})")
     if use_mf:
-        pargs = (h.n_qubits,) + pauli.args() + (scale,)
+        pargs = (h.n_qubits,) + pauli.step_args() + (scale,)
 
         def term(cur_, prv_, ck, first):
             call("qsim_cheb_pauli_step", s, *pargs, ptr(cur_), ptr(prv_), ptr(acc), ptr(comp),

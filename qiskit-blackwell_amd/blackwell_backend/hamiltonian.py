@@ -85,10 +85,29 @@ def pauli_groups(labels, coeffs):
             np.array(coef, dtype=np.complex128))
 
 
+def constant_group_entries(toff, sm, coef):
+    """Per group, the entry rounded to complex64 (as FP64 pairs) when no term
+    has a Y|Z mask, so that it is the same in every row: the FP64 sum of the
+    coefficients in term order, as the device evaluates it; NaN where the
+    entry depends on the row."""
+    out = np.full((len(toff) - 1, 2), np.nan)
+    for g in range(len(toff) - 1):
+        t0, t1 = int(toff[g]), int(toff[g + 1])
+        if np.any(sm[t0:t1] != 0):
+            continue
+        re = im = 0.0
+        for t in range(t0, t1):
+            re += float(coef[t].real)
+            im += float(coef[t].imag)
+        out[g] = (float(np.float32(re)), float(np.float32(im)))
+    return out
+
+
 class PauliTables:
     """pauli_groups() on a device: n_groups and the tensors xm (uint64 as
     int64), toff (int32), sm (uint64 as int64), coef (FP64 pairs), the
-    layout qsim_pauli_csr_count and qsim_cheb_pauli_step take."""
+    layout qsim_pauli_csr_count and qsim_cheb_pauli_step take, plus gconst
+    (constant_group_entries) for the matrix-free Chebyshev term."""
 
     def __init__(self, labels, coeffs, device):
         xm, toff, sm, coef = pauli_groups(labels, coeffs)
@@ -98,10 +117,18 @@ class PauliTables:
         self.toff = torch.from_numpy(toff).to(dev)
         self.sm = torch.from_numpy(sm.view(np.int64) if sm.size else np.zeros(1, np.int64)).to(dev)
         self.coef = torch.from_numpy(coef.view(np.float64) if coef.size else np.zeros(2)).to(dev)
+        gc = constant_group_entries(toff, sm, coef)
+        self.gconst = torch.from_numpy(gc.reshape(-1) if gc.size else np.zeros(2)).to(dev)
 
     def args(self):
+        """(n_groups, xm, toff, sm, coef) pointers for qsim_pauli_csr_count / _fill."""
         from blackwell_kernels import _lib
         return (self.n_groups, _lib.ptr(self.xm), _lib.ptr(self.toff), _lib.ptr(self.sm), _lib.ptr(self.coef))
+
+    def step_args(self):
+        """args() plus gconst, for qsim_cheb_pauli_step."""
+        from blackwell_kernels import _lib
+        return self.args() + (_lib.ptr(self.gconst),)
 
 
 def pauli_to_csr_device(labels, coeffs, n, device, tables=None):

@@ -91,7 +91,7 @@ PROTOTYPES = {
     "qsim_csr_split_fill": (_i, [_vp, _vp, _vp, _vp, _i64, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "qsim_cheb_spmm_step_split": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i, _i, _d, _vp, _vp, _vp,
                                        _vp, _vp, _d, _d, _i]),
-    "qsim_cheb_pauli_step": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp, _d, _d, _i]),
+    "qsim_cheb_pauli_step": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp, _d, _d, _i]),
     "qsim_cheb_init_acc": (_i, [_vp, _vp, _vp, _vp, _i64, _d, _d]),
     "qsim_cheb_finish": (_i, [_vp, _vp, _vp, _vp, _i64]),
 }

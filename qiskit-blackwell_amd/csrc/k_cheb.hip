@@ -196,8 +196,10 @@ struct RowSum;
 template <>
 struct RowSum<kAccF64> {
   double r = 0.0, i = 0.0;
-  __device__ __forceinline__ void add(float2 v, float2 x) {
-    const double vr = v.x, vi = v.y, xr = x.x, xi = x.y;
+  __device__ __forceinline__ void add(float2 v, float2 x) { add_d(v.x, v.y, x); }
+  // vr, vi: an entry already rounded to complex64, held in FP64
+  __device__ __forceinline__ void add_d(double vr, double vi, float2 x) {
+    const double xr = x.x, xi = x.y;
     r = __builtin_fma(vr, xr, r);
     r = __builtin_fma(-vi, xi, r);
     i = __builtin_fma(vr, xi, i);
@@ -595,13 +597,16 @@ __global__ __launch_bounds__(kThreads) void k_cheb_spmv1(const int32_t* __restri
 // device CSR stores them) at columns r ^ xm[g], summed in group order as
 // FP64 sums of exact products.  One thread per row; the group and term loops
 // are uniform across the wave, so the tables come in through scalar loads.
+// gconst (optional) gives, per group, the entry already rounded to complex64
+// when it is the same in every row (no term has a Y|Z mask), or NaN: such a
+// group costs its gather and four FMAs, nothing else.
 // No index or value stream is read: per term the only HBM traffic is the
 // five state vectors, and the gathers of T_k hit the caches (T_k is 32 MiB
 // at 22 qubits).  Each term of an entry costs an AND, a popcount, one
 // shift-or that puts the parity into the sign of 1.0, and two FP64 FMAs
 // (per-term selects of +-c took 18 instructions); a 22-qubit chain term
-// takes 0.16 ms against 0.70 ms for the CSR term.  Walking XCD-contiguous
-// row ranges measured the same.
+// takes 0.15 ms (with gconst) against 0.70 ms for the CSR term.  Walking
+// XCD-contiguous row ranges measured the same.
 constexpr int kPauliRun = 8;   // 4: same time, 16: 3 % slower, 1: 10 % slower (22-qubit chain)
 // rows per thread, kThreads apart: the scalar table loads and the loop
 // control serve two rows (22-qubit chain: 1 row 0.206 ms, 2 rows 0.164 ms,
@@ -634,6 +639,7 @@ __global__ __launch_bounds__(kThreads) void k_cheb_pauli1(int n_groups, const ui
                                                           const int* __restrict__ toff,
                                                           const uint64_t* __restrict__ sm,
                                                           const double2* __restrict__ coef,
+                                                          const double2* __restrict__ gconst,
                                                           int64_t n_rows, double scale,
                                                           const float2* __restrict__ tc,
                                                           float2* __restrict__ tpn,
@@ -660,6 +666,16 @@ __global__ __launch_bounds__(kThreads) void k_cheb_pauli1(int n_groups, const ui
       for (int q = 0; q < RR; ++q) x[j][q] = tc[(r[q] ^ (R)xm[g + j]) & rmask];
 #pragma unroll
     for (int j = 0; j < kPauliRun; ++j) {
+      if (gconst) {
+        const double2 v = gconst[g + j];
+        if (v.x == v.x) {                   // the same entry in every row (uniform branch)
+          if (v.x != 0.0 || v.y != 0.0) {
+#pragma unroll
+            for (int q = 0; q < RR; ++q) sum[q].add_d(v.x, v.y, x[j][q]);
+          }
+          continue;
+        }
+      }
       double re[RR], im[RR];
       pauli_group_rows<R, RR>(r, toff[g + j], toff[g + j + 1], sm, coef, re, im);
 #pragma unroll
@@ -668,6 +684,16 @@ __global__ __launch_bounds__(kThreads) void k_cheb_pauli1(int n_groups, const ui
     }
   }
   for (; g < n_groups; ++g) {
+    if (gconst) {
+      const double2 v = gconst[g];
+      if (v.x == v.x) {
+        if (v.x != 0.0 || v.y != 0.0) {
+#pragma unroll
+          for (int q = 0; q < RR; ++q) sum[q].add_d(v.x, v.y, tc[(r[q] ^ (R)xm[g]) & rmask]);
+        }
+        continue;
+      }
+    }
     double re[RR], im[RR];
     pauli_group_rows<R, RR>(r, toff[g], toff[g + 1], sm, coef, re, im);
 #pragma unroll
@@ -824,7 +850,7 @@ int qsim_cheb_spmm_step(void* stream, const void* row_ptr, const void* col_idx, 
 }
 
 int qsim_cheb_pauli_step(void* stream, int n_qubits, int n_groups, const void* xm, const void* toff,
-                         const void* sm, const void* coef, double scale, const void* t_cur,
+                         const void* sm, const void* coef, const void* gconst, double scale, const void* t_cur,
                          void* t_prev_next, void* acc, void* comp, double c_re, double c_im,
                          int first) {
   if (n_qubits < 1 || n_qubits > QSIM_MAX_QUBITS)
@@ -838,7 +864,8 @@ int qsim_cheb_pauli_step(void* stream, int n_qubits, int n_groups, const void* x
                              : (nt ? k_cheb_pauli1<uint32_t, true> : k_cheb_pauli1<uint32_t, false>);
   hipLaunchKernelGGL(kern, grid1d(ceil_div((uint64_t)n_rows, kThreads * kPauliRows)), dim3(kThreads), 0,
                      as_stream(stream), n_groups, (const uint64_t*)xm, (const int*)toff,
-                     (const uint64_t*)sm, (const double2*)coef, n_rows, scale, (const float2*)t_cur,
+                     (const uint64_t*)sm, (const double2*)coef, (const double2*)gconst, n_rows, scale,
+                     (const float2*)t_cur,
                      (float2*)t_prev_next, (float2*)acc, (float2*)comp, f2(c_re, c_im), first);
   return launch_status("cheb_pauli_step");
 }

@@ -580,7 +580,8 @@ def test_cheb_spmm_step_matches_fp64(bk, dev, accumulate, n_rows):
 def _rand_pauli_sum(n, n_terms, seed):
     """Random Pauli labels over I/X/Y/Z with real coefficients; every X mask
     appears with several Z/Y patterns (groups of more than one term), one
-    term is the identity, and one pair of terms cancels on half the rows."""
+    term is the identity, one pair of terms cancels on half the rows, and
+    some X-only groups have the same entry in every row (one of them zero)."""
     rng = np.random.default_rng(seed)
     labels, coeffs = [], []
     for _ in range(n_terms):
@@ -595,6 +596,11 @@ def _rand_pauli_sum(n, n_terms, seed):
     coeffs.append(0.75)
     labels += ["X" + "I" * (n - 1), "X" + "Z" + "I" * (n - 2)]   # 0.5 X (1 + Z): zero where bit n-2 is set
     coeffs += [0.5, 0.5]
+    # groups whose entry is the same in every row (X only, no Y or Z): one
+    # term, two terms, and two that cancel exactly
+    labels += ["I" * (n - 2) + "XX", "I" * (n - 3) + "X" + "I" + "X", "I" * (n - 3) + "X" + "I" + "X",
+               "I" * (n - 4) + "XX" + "I" * 2, "I" * (n - 4) + "XX" + "I" * 2]
+    coeffs += [0.3, 0.2, -0.45, 0.6, -0.6]
     return labels, coeffs
 
 
@@ -612,6 +618,8 @@ def test_cheb_pauli_step_matches_csr_and_fp64(bk, dev, n):
     labels, coeffs = _rand_pauli_sum(n, 3 * n, 100 + n)
     h = HamiltonianCSR.from_sparse_pauli_op(SparsePauliOp.from_list(list(zip(labels, coeffs))), device=dev)
     assert h.pauli is not None and h.pauli.n_groups > 1
+    gc = h.pauli.gconst.cpu().numpy().reshape(-1, 2)
+    assert np.isfinite(gc[:, 0]).any() and np.isnan(gc[:, 0]).any()   # both kinds of group occur
     A64 = OH.pauli_sparse(labels, coeffs)
     A64 = A64.astype(np.complex64).astype(np.complex128)     # the entries rounded as the device stores them
     absA = abs(A64)
@@ -623,11 +631,14 @@ def test_cheb_pauli_step_matches_csr_and_fp64(bk, dev, n):
     s = stream_handle(dev)
     for first in (1, 0):
         outs = {}
-        for how in ("pauli", "csr"):
+        for how in ("pauli", "pauli_no_const", "csr"):
             tpd, accd = to_dev(Tp, dev), to_dev(acc0, dev)
             compd = torch.zeros_like(accd)
-            if how == "pauli":
-                call("qsim_cheb_pauli_step", s, n, *h.pauli.args(), scale, ptr(to_dev(T, dev)), ptr(tpd),
+            if how.startswith("pauli"):
+                pa = h.pauli.step_args()
+                if how == "pauli_no_const":
+                    pa = pa[:-1] + (None,)           # every group evaluated per row
+                call("qsim_cheb_pauli_step", s, n, *pa, scale, ptr(to_dev(T, dev)), ptr(tpd),
                      ptr(accd), ptr(compd), c.real, c.imag, first)
             else:
                 call("qsim_cheb_spmm_step", s, ptr(h.row_ptr), ptr(h.col_idx), ptr(h.values), N, 1, scale,
@@ -643,6 +654,8 @@ def test_cheb_pauli_step_matches_csr_and_fp64(bk, dev, n):
             assert np.all(np.abs(tn - tn_ref) <= tol), (how, first, float(np.max(np.abs(tn - tn_ref) - tol)))
             want_acc = acc0.astype(np.complex128) + c * tn
             assert np.max(np.abs(acc - want_acc)) <= 1e-6 * np.max(np.abs(want_acc)), (how, first)
+        # constant-group entries from the host table: the same bits as evaluating them per row
+        assert np.array_equal(outs["pauli"][0], outs["pauli_no_const"][0]), first
         d = np.abs(outs["pauli"][0] - outs["csr"][0])
         assert np.all(d <= 2.0 ** -22 * np.abs(outs["csr"][0]) + 1e-15 * bound + 1e-30), (first, float(d.max()))
 

@@ -430,3 +430,36 @@ def test_c_chebyshev_matches_expm_multiply():
     # H = Z on one qubit: exact phases
     out, _ = OH.chebyshev_evolve_c(["Z"], [1.0], 1, np.array([0.6, 0.8j]), 1.3, 1.0)
     assert np.allclose(out[0], [0.6 * cmath.exp(-1.3j), 0.8j * cmath.exp(1.3j)], atol=1e-12)
+
+
+def test_constant_group_entries_match_oracle_csr():
+    """The host table of row-independent group entries (X-only groups) used by
+    the matrix-free Chebyshev term: every such group's entry is, in every
+    row, the oracle CSR's value at column row ^ X mask rounded to complex64
+    (absent where the sum is zero), and every group with a Y or Z term is
+    marked NaN."""
+    from blackwell_backend.hamiltonian import pauli_groups, constant_group_entries
+    n = 5
+    labels = ["IIIXX", "IIIXX", "IXIXI", "XIIII", "XIIIZ", "IIIII", "ZZIII", "IYYII", "IXXII",
+              "XXXXX", "XXXXX"]
+    coeffs = [0.3, 0.45, -0.2, 0.7, 0.1, 1.25, -0.4, 0.35, 0.35, 0.6, -0.6]
+    xm, toff, sm, coef = pauli_groups(labels, coeffs)
+    gc = constant_group_entries(toff, sm, coef)
+    rp, ci, v = OH.pauli_csr_grouped(labels, coeffs, n)
+    seen_const = seen_zero = seen_row = 0
+    for g in range(len(xm)):
+        if np.any(sm[toff[g]:toff[g + 1]] != 0):
+            assert np.isnan(gc[g, 0])
+            seen_row += 1
+            continue
+        want = complex(gc[g, 0], gc[g, 1])
+        seen_const += 1
+        seen_zero += want == 0
+        for r in range(1 << n):
+            cols = ci[rp[r]:rp[r + 1]]
+            hit = np.nonzero(cols == (r ^ int(xm[g])))[0]
+            if want == 0:
+                assert hit.size == 0
+            else:
+                assert hit.size == 1 and np.complex64(v[rp[r] + hit[0]]) == np.complex64(want)
+    assert seen_const >= 3 and seen_zero == 1 and seen_row >= 3
