@@ -705,7 +705,9 @@ int qsim_apply_block_mfma(void* stream, void* state, int n_qubits, const void* b
   //   no block qubit in 0..3: prefetch, two workgroups per CU, nontemporal
   //   above the MALL size (cached below it, so the state stays resident);
   //   lowest block qubit in 1..3: prefetch, cached accesses, one per CU;
-  //   qubit 0 in the block: no prefetch, cached accesses, two per CU.
+  //   qubit 0 in the block: cached accesses; above the MALL size prefetch
+  //   and one per CU (3.45 against 3.61 ms on five patterns at 30 qubits),
+  //   below it no prefetch and two per CU.
   if (n_qubits >= 11 && n_qubits <= 32 && low <= 2 && (qmin != 0 || a.q[0] == 0)) {
     const uint64_t n_sets = (1ull << (n_qubits - 5)) / 64;
     hipStream_t s = as_stream(stream);
@@ -715,7 +717,7 @@ int qsim_apply_block_mfma(void* stream, void* state, int n_qubits, const void* b
     const float2* U = (const float2*)block_dev;
     float2* sv = (float2*)state;
     if (qmin == 0) {
-      if (il) hipLaunchKernelGGL((k_block5w<0, 0, 1, 1>), grid(2), dim3(256), 0, s, sv, n_sets, a, U);
+      if (il) hipLaunchKernelGGL((k_block5w<1, 0, 1, 1>), grid(1), dim3(256), 0, s, sv, n_sets, a, U);
       else hipLaunchKernelGGL((k_block5w<0, 0, 1, 0>), grid(2), dim3(256), 0, s, sv, n_sets, a, U);
     } else if (qmin < 4) {
       if (il) hipLaunchKernelGGL((k_block5w<1, 0, 0, 1>), grid(1), dim3(256), 0, s, sv, n_sets, a, U);
