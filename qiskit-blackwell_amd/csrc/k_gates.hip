@@ -283,7 +283,10 @@ __global__ __launch_bounds__(kThreads) void k_gate2q(float4* __restrict__ st, I 
   float2 g[16];
   load_m4(gdev, gv, g);
   bool tail = false;
-  const I blk = NT ? pass_block<I, (S > 0)>(po, tail) : (S > 0 ? xcd_block<I>() : gblock<I>());
+  // streamed states take the XCD-major pass order for every S, so that a
+  // both-low-target gate starts where the stream gates before it ended (the
+  // 30-qubit SU(4) step measured 2.8 ms shorter than with natural order for S = 0)
+  const I blk = NT ? pass_block<I, true>(po, tail) : (S > 0 ? xcd_block<I>() : gblock<I>());
   const I w0 = blk * kThreads + threadIdx.x;
   const bool live = w0 < n_work;
   const I w = live ? w0 : n_work - 1;
