@@ -172,7 +172,8 @@ int qsim_apply_block_mfma(void* stream, void* state, int n_qubits, const void* b
  * Packs them greedily into blocks of exactly min(k, n) qubits (k <= 6) such
  * that every op still follows the earlier ops it shares a qubit with.
  * Outputs, in execution order, *n_out entries out_ref[e]: b >= 0 names block
- * b (qubits block_qubits[8*b ..], ascending, -1 padded; matrix block_mats +
+ * b (qubits block_qubits[8*b ..], ascending, -1 padded; a block short of
+ * k qubits takes the lowest unused qubits from 12 up, then below; matrix block_mats +
  * b*2^(2k)*2 doubles, row bit j <-> j-th block qubit) and -1-i passes op i
  * through (wider than k).  Capacity: n_ops entries and blocks.            */
 int qsim_plan_blocks(int n_qubits, int k, int n_ops, const int32_t* op_nq, const int32_t* op_qubits,

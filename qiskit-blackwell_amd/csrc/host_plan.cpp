@@ -21,7 +21,8 @@
 //    time, a 20-op window 112 and 96.
 // The block matrix is the FP64 product of the members in circuit order, each
 // applied to the rows of the running 2^k x 2^k matrix.  Blocks are padded with
-// the lowest unused qubits to exactly min(k, n) qubits.  An op wider than k
+// the lowest unused qubits from kPadFrom up (then below it) to exactly
+// min(k, n) qubits.  An op wider than k
 // becomes a passthrough entry for the caller to run on its own kernel.
 #include "qsim_gfx950.h"
 #include "host_error.hpp"
@@ -35,6 +36,12 @@ namespace qsim {
 constexpr int kSeeds = 32;
 constexpr int kSearchWindow = 30;     // ops after the seed whose qubits may join S (effort 1)
 constexpr int kSearchQubits = 16;     // at most this many candidate qubits per seed
+// First padding qubit.  A padded qubit only selects the block kernel: a low one
+// can put qubit 0 or a third qubit of 0..4 into the block, whose kernels are
+// slower, and a lowest block qubit of 12 streams well.  The 30-qubit SU(4)
+// step ran 363-364 ms against 369 ms with padding from qubit 0 (108 blocks
+// either way; 368 ms from 16, 20 or 24).
+constexpr int kPadFrom = 12;
 
 // Ops of rem (in order) that the qubit set S absorbs: an op joins when its
 // qubits lie in S and no earlier op left out touched any of them.  With
@@ -286,8 +293,10 @@ int qsim_plan_blocks_search(int n_qubits, int k, int effort, int n_ops, const in
     int bqs[8], nb = 0;
     for (int q = 0; q < n_qubits && nb < 8; ++q)
       if (bq >> q & 1) bqs[nb++] = q;
-    for (int q = 0; q < n_qubits && nb < kk; ++q)
-      if (!(bq >> q & 1)) { bqs[nb++] = q; bq |= 1ull << q; }
+    // padding qubits: the lowest unused from kPadFrom up, then below it
+    for (int pass = 0; pass < 2; ++pass)
+      for (int q = pass ? 0 : std::min(kPadFrom, n_qubits); q < n_qubits && nb < kk; ++q)
+        if (!(bq >> q & 1)) { bqs[nb++] = q; bq |= 1ull << q; }
     // ascending order
     for (int a = 0; a < nb; ++a)
       for (int b = a + 1; b < nb; ++b)
