@@ -9,15 +9,16 @@
 // them was touched by an earlier op left out of the block (so every op still
 // follows its predecessors).  A seed is any remaining op that no earlier
 // remaining op touches; up to kSeeds of them are tried and the block that
-// absorbs the most ops is kept (ties go to the earliest).
+// absorbs the most ops per unit of kernel time (block_score) is kept (ties go
+// to the earliest).
 //  * effort 0: S grows greedily from the seed's qubits, an op joining while
 //    the union stays <= k; a depth-28 random SU(4) circuit on 28 qubits
 //    takes 100 blocks (120 with the first op as the only seed).
 //  * effort 1: in addition, every S made of the seed's qubits and up to
 //    k - |seed| of the qubits that the next kSearchWindow ops connect to the
-//    seed is scored.  The depth-30 SU(4) circuit on 30 qubits then takes 108
-//    blocks instead of 119 (28 qubits, depth 28: 92 instead of 100), for about
-//    5 ms of host time.  A 40-op window gave 110 and 93 blocks in twice the
+//    seed is scored.  The depth-30 SU(4) circuit on 30 qubits then took 108
+//    blocks instead of 119 (28 qubits, depth 28: 92 instead of 100) when
+//    scored by count, for about 5 ms of host time.  A 40-op window gave 110 and 93 blocks in twice the
 //    time, a 20-op window 112 and 96.
 // The block matrix is the FP64 product of the members in circuit order, each
 // applied to the rows of the running 2^k x 2^k matrix.  Blocks are padded with
@@ -42,6 +43,21 @@ constexpr int kSearchQubits = 16;     // at most this many candidate qubits per 
 // step ran 363-364 ms against 369 ms with padding from qubit 0 (108 blocks
 // either way; 368 ms from 16, 20 or 24).
 constexpr int kPadFrom = 12;
+// Blocks are compared by ops absorbed per unit of kernel time, in thousandths
+// of a block with no qubit in 0..3: qubit 0 in the block 1.10, three or more
+// block qubits in 0..4 (the LDS-staged kernel) 1.31, the ratios of their mean
+// launch times in the 30-qubit SU(4) trace.  Scored by count alone, that
+// circuit took 108 blocks (20 with qubit 0, 3 LDS-staged) and 367 ms per
+// fused step; scored per cost, 107 blocks (15 and 2) and 361 ms.  The ratios
+// hold for states above the 256 MiB MALL (n > 25); smaller states and the
+// other widths (one kernel for every qubit set) are scored by count.
+constexpr long long kCostQ0 = 1100, kCostLow3 = 1310;
+
+static inline long long block_score(int n_ops, uint64_t S, bool weigh) {
+  long long c = 1000;
+  if (weigh) c = (S & 1ull) ? kCostQ0 : (__builtin_popcountll(S & 31ull) >= 3 ? kCostLow3 : 1000);
+  return (long long)n_ops * 1000000 / c;
+}
 
 // Ops of rem (in order) that the qubit set S absorbs: an op joins when its
 // qubits lie in S and no earlier op left out touched any of them.  With
@@ -175,6 +191,7 @@ int qsim_plan_blocks_search(int n_qubits, int k, int effort, int n_ops, const in
     return fail(QSIM_EINVAL, "NULL pointer argument");
   const int kk = k < n_qubits ? k : n_qubits;
   const int D = 1 << kk;
+  const bool weigh = kk == 5 && n_qubits > 25;
   std::vector<uint64_t> mask(n_ops);
   std::vector<size_t> moff(n_ops);
   size_t off = 0;
@@ -233,7 +250,7 @@ int qsim_plan_blocks_search(int n_qubits, int k, int effort, int n_ops, const in
         // a full block whose qubits are all blocked can take nothing more
         if (t > sc && __builtin_popcountll(bq) == kk && (blocked & bq) == bq) break;
       }
-      if (chosen.size() > best.size()) {
+      if (best.empty() || block_score((int)chosen.size(), bq, weigh) > block_score((int)best.size(), best_bq, weigh)) {
         best = chosen;
         best_bq = bq;
       }
@@ -249,16 +266,16 @@ int qsim_plan_blocks_search(int n_qubits, int k, int effort, int n_ops, const in
           if ((conn >> q & 1) && !(ms >> q & 1)) extra[ne++] = q;
         const int room = kk - __builtin_popcountll(ms);
         // subsets of extra with 1..room members (Gosper's next-combination)
-        int best_cnt = (int)best.size();
+        long long best_sc = block_score((int)best.size(), best_bq, weigh);
         uint64_t best_s = 0;
         for (int r = 1; r <= room && r <= ne; ++r) {
           for (uint32_t sub = (1u << r) - 1; sub < (1u << ne);) {
             uint64_t S = ms;
             for (int e = 0; e < ne; ++e)
               if (sub >> e & 1) S |= 1ull << extra[e];
-            const int cnt = absorb_set(rem, mask, S, nullptr);
-            if (cnt > best_cnt) {
-              best_cnt = cnt;
+            const long long sc = block_score(absorb_set(rem, mask, S, nullptr), S, weigh);
+            if (sc > best_sc) {
+              best_sc = sc;
               best_s = S;
             }
             const uint32_t c = sub & (0u - sub), nx = sub + c;
